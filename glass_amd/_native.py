@@ -1,0 +1,180 @@
+"""ctypes binding of libglass_hip.so (the C ABI in include/glass_hip.h).
+
+The library is loaded from glass_amd/lib/ (built in-tree by glass_amd.build).
+There is no fallback: if the library is missing or fails to load, every entry
+point raises, so nothing silently runs on the CPU.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_int32, c_int64, c_size_t, c_uint32, c_void_p
+from pathlib import Path
+
+LIBRARY_PATH = Path(__file__).resolve().parent / "lib" / "libglass_hip.so"
+
+GH_OK = 0
+GH_ERR_VALUE = 1
+GH_ERR_INDEX = 2
+GH_ERR_RUNTIME = 3
+
+GH_FMT_F32 = 0
+GH_FMT_U16 = 1
+GH_FMT_FITS_U16BE_BZERO = 2
+
+GH_INTERP_BILINEAR = 1
+GH_INTERP_LANCZOS3 = 3
+
+GH_WARP_TRACK_COVERAGE = 1
+GH_WARP_SKIP_NONFINITE_TAPS = 2
+GH_WARP_AFFINE = 4
+
+GH_REJECT_NONE = 0
+GH_REJECT_WINSORIZED_SIGMA = 1
+
+GH_COUNT_F32 = 0
+GH_COUNT_U16 = 1
+
+
+class CalibPolicy(ctypes.Structure):
+    _fields_ = [
+        ("master_dark_includes_bias", c_int32),
+        ("flat_floor", c_float),
+        ("pedestal", c_float),
+        ("accumulate_stats", c_int32),
+    ]
+
+
+class IntegrateParams(ctypes.Structure):
+    _fields_ = [
+        ("rejection", c_int32),
+        ("low_sigma", c_float),
+        ("high_sigma", c_float),
+        ("min_samples", c_int32),
+        ("max_reject_fraction", c_float),
+        ("count_dtype", c_int32),
+        ("statistics_variant", c_int32),
+        ("require_positive_scale", c_int32),
+        ("tile_size", c_int32),
+    ]
+
+
+class IntegrateOutputs(ctypes.Structure):
+    _fields_ = [
+        ("master", c_void_p),
+        ("weight", c_void_p),
+        ("coverage", c_void_p),
+        ("low", c_void_p),
+        ("high", c_void_p),
+        ("dq", c_void_p),
+        ("dq_use_geometric", c_int32),
+        ("dq_active_frames", c_int32),
+    ]
+
+
+class FrameStatistics(ctypes.Structure):
+    _fields_ = [
+        ("mean", c_double),
+        ("std", c_double),
+        ("sum", c_double),
+        ("sum_sq", c_double),
+        ("valid_pixels", c_int64),
+        ("total_pixels", c_int64),
+        ("nonfinite_pixels", c_int64),
+        ("median", c_double),
+        ("mad", c_double),
+        ("min", c_float),
+        ("max", c_float),
+    ]
+
+
+# name -> (argtypes); every function returns int
+_SIGNATURES = {
+    "gh_last_error": [c_char_p, c_size_t],
+    "gh_version": [],
+    "gh_device_count": [POINTER(c_int)],
+    "gh_stack_create": [c_int64, c_int64, c_int64, POINTER(c_void_p)],
+    "gh_stack_destroy": [c_void_p],
+    "gh_stack_info": [c_void_p, POINTER(c_int64)],
+    "gh_stack_frame_ptr": [c_void_p, c_int64, POINTER(c_void_p)],
+    "gh_stack_frame_nonfinite": [c_void_p, c_int64, POINTER(c_int64)],
+    "gh_set_masters": [c_void_p, c_void_p, c_void_p, c_void_p],
+    "gh_upload_frame": [c_void_p, c_int64, c_void_p],
+    "gh_download_tile": [c_void_p, POINTER(c_int64), c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p],
+    "gh_calibrate_frames": [
+        c_void_p, POINTER(c_int64), c_int64, c_void_p, c_int64, c_int32, POINTER(c_float),
+        POINTER(CalibPolicy), c_void_p,
+    ],
+    "gh_normalize_frames": [
+        c_void_p, POINTER(c_int64), c_int64, POINTER(c_float), POINTER(c_float), c_void_p,
+    ],
+    "gh_warp_frames": [
+        c_void_p, POINTER(c_int64), c_int64, POINTER(c_double), c_int32, c_float, c_float, c_uint32, c_void_p,
+    ],
+    "gh_reset_warp_coverage": [c_void_p],
+    "gh_accumulate_full_warp_coverage": [c_void_p],
+    "gh_warp_coverage_map": [c_void_p, c_void_p],
+    "gh_integrate": [c_void_p, POINTER(c_float), POINTER(IntegrateParams), POINTER(IntegrateOutputs), c_void_p],
+    "gh_frame_stats": [c_void_p, c_int64, c_int32, POINTER(FrameStatistics)],
+    "gh_calibration_stats": [c_void_p, c_int64, POINTER(c_double), POINTER(c_double), POINTER(c_int64)],
+    "gh_synchronize": [c_void_p],
+}
+
+EXPORTED_SYMBOLS = tuple(_SIGNATURES)
+
+_lib = None
+
+
+def load_library(path: Path | str | None = None) -> ctypes.CDLL:
+    """Load libglass_hip.so (cached).  Raises if it is missing."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    target = Path(path) if path is not None else LIBRARY_PATH
+    if not target.exists():
+        raise RuntimeError(
+            f"libglass_hip.so not found at {target}; build it with `python -m glass_amd.build` "
+            "(the resident stacking path has no CPU fallback)"
+        )
+    # One HIP runtime per process: PyTorch-ROCm ships its own libamdhip64.so.7.
+    # Loading torch first makes our library bind to that copy (same SONAME), so
+    # device pointers and streams are shared with torch tensors; loading ours
+    # first would bring a second runtime and torch would then see no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:  # pragma: no cover - torch is optional for the C ABI
+        pass
+    lib = ctypes.CDLL(str(target))
+    for name, argtypes in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argtypes
+        fn.restype = c_int
+    if path is None:
+        _lib = lib
+    return lib
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def last_error() -> str:
+    lib = load_library()
+    buf = ctypes.create_string_buffer(4096)
+    lib.gh_last_error(buf, len(buf))
+    return buf.value.decode("utf-8", errors="replace")
+
+
+def call(name: str, *args) -> None:
+    """Call a gh_* entry point and map its status onto Python exceptions the
+    way the reference bindings do (ValueError / IndexError / RuntimeError)."""
+    lib = load_library()
+    status = getattr(lib, name)(*args)
+    if status == GH_OK:
+        return
+    message = last_error()
+    if status == GH_ERR_VALUE:
+        raise ValueError(message)
+    if status == GH_ERR_INDEX:
+        raise IndexError(message)
+    raise RuntimeError(message)

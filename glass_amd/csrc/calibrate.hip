@@ -1,0 +1,439 @@
+// Light calibration and per-frame statistics kernels for gfx950.
+//
+// Calibration arithmetic follows the CPU oracle src/glass/cpu/calibration.py:51-76
+// operation for operation in f32 (no contraction: the library is built with
+// -ffp-contract=off):  x = raw; x -= dark*s; [x -= bias]; x /= max(flat, floor);
+// [x += pedestal].  Raw FITS u16 decode as cpp/cuda/calibration_kernels.cu:133-137.
+//
+// HBM bytes per frame-pixel: 2 (u16 raw) or 4 (f32 raw) read + 4 written; the
+// master tiles (12 B/px) are shared by every frame of a batch and stay in L2/MALL
+// for the batch.  Each thread handles 4 consecutive pixels (8 B / 16 B loads,
+// 16 B stores); the batch index runs on gridDim.y so one launch covers up to 16
+// frames.  Optional fused FP64 moments replace a separate stats pass over the stack.
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace gh {
+namespace {
+
+constexpr int kFmtF32 = 0;
+constexpr int kFmtU16 = 1;
+constexpr int kFmtBE = 2;
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__device__ __forceinline__ float calibrate_one(float v, float bias, float dark, float flat, const CalibParams& p,
+                                               float dark_scale) {
+  if (p.has_dark) {
+    v = v - dark * dark_scale;
+    if (!p.dark_includes_bias && p.has_bias) v = v - bias;
+  } else if (p.has_bias) {
+    v = v - bias;
+  }
+  if (p.has_flat) {
+    // np.maximum keeps NaN; a NaN flat must not turn into the floor.
+    const float d = (flat < p.flat_floor) ? p.flat_floor : flat;
+    v = v / d;
+  }
+  if (p.pedestal != 0.0f) v = v + p.pedestal;
+  return v;
+}
+
+__device__ __forceinline__ float decode_be(unsigned short word) {
+  // bytes b0 b1 in memory; the little-endian load gives b0 | b1<<8.
+  const unsigned b0 = word & 0xffu;
+  const unsigned b1 = (word >> 8) & 0xffu;
+  return static_cast<float>(((b0 << 8) | b1) ^ 0x8000u);
+}
+
+template <int FMT>
+__device__ __forceinline__ void load_raw4(const void* src, long long i, float (&v)[4]) {
+  if constexpr (FMT == kFmtF32) {
+    const float4 q = reinterpret_cast<const float4*>(src)[i];
+    v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+  } else {
+    const uint2 q = reinterpret_cast<const uint2*>(src)[i];
+    const unsigned short s0 = q.x & 0xffffu, s1 = q.x >> 16, s2 = q.y & 0xffffu, s3 = q.y >> 16;
+    if constexpr (FMT == kFmtU16) {
+      v[0] = static_cast<float>(s0); v[1] = static_cast<float>(s1);
+      v[2] = static_cast<float>(s2); v[3] = static_cast<float>(s3);
+    } else {
+      v[0] = decode_be(s0); v[1] = decode_be(s1); v[2] = decode_be(s2); v[3] = decode_be(s3);
+    }
+  }
+}
+
+template <int FMT>
+__device__ __forceinline__ float load_raw1(const void* src, long long i) {
+  if constexpr (FMT == kFmtF32) {
+    return reinterpret_cast<const float*>(src)[i];
+  } else {
+    const unsigned short s = reinterpret_cast<const unsigned short*>(src)[i];
+    if constexpr (FMT == kFmtU16) return static_cast<float>(s);
+    return decode_be(s);
+  }
+}
+
+template <int FMT, int VEC>
+__global__ __launch_bounds__(kBlock) void calibrate_kernel(CalibBatch batch, CalibParams p, long long npix,
+                                                           StatPartial* partials) {
+  const int f = blockIdx.y;
+  const void* src = batch.src[f];
+  float* dst = batch.dst[f];
+  const float s = batch.dark_scale[f];
+  double sum = 0.0, sum_sq = 0.0;
+  unsigned long long finite = 0, nonfinite = 0;
+  const long long groups = npix / VEC;
+  const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+  for (long long g = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; g < groups; g += stride) {
+    float v[VEC];
+    float b[VEC] = {}, d[VEC] = {}, fl[VEC] = {};
+    if constexpr (VEC == 4) {
+      load_raw4<FMT>(src, g, v);
+      if (p.has_bias) {
+        const float4 q = reinterpret_cast<const float4*>(p.bias)[g];
+        b[0] = q.x; b[1] = q.y; b[2] = q.z; b[3] = q.w;
+      }
+      if (p.has_dark) {
+        const float4 q = reinterpret_cast<const float4*>(p.dark)[g];
+        d[0] = q.x; d[1] = q.y; d[2] = q.z; d[3] = q.w;
+      }
+      if (p.has_flat) {
+        const float4 q = reinterpret_cast<const float4*>(p.flat)[g];
+        fl[0] = q.x; fl[1] = q.y; fl[2] = q.z; fl[3] = q.w;
+      }
+    } else {
+      v[0] = load_raw1<FMT>(src, g);
+      if (p.has_bias) b[0] = p.bias[g];
+      if (p.has_dark) d[0] = p.dark[g];
+      if (p.has_flat) fl[0] = p.flat[g];
+    }
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) {
+      v[j] = calibrate_one(v[j], b[j], d[j], fl[j], p, s);
+      if (isfinite(v[j])) {
+        if (p.accumulate_stats) {
+          const double x = static_cast<double>(v[j]);
+          sum += x;
+          sum_sq += x * x;
+          ++finite;
+        }
+      } else {
+        ++nonfinite;
+      }
+    }
+    if constexpr (VEC == 4) {
+      reinterpret_cast<float4*>(dst)[g] = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+      dst[g] = v[0];
+    }
+  }
+  // Any non-finite output marks the frame (one atomic per wave that saw one).
+  if (__any(nonfinite != 0) && (threadIdx.x & 63) == 0) atomicOr(batch.nonfinite[f], 1u);
+  if (!p.accumulate_stats) return;
+  __shared__ double s_sum[kBlock / 64], s_sq[kBlock / 64];
+  __shared__ unsigned long long s_fin[kBlock / 64], s_nf[kBlock / 64];
+  sum = wave_sum(sum);
+  sum_sq = wave_sum(sum_sq);
+  finite = wave_sum(finite);
+  nonfinite = wave_sum(nonfinite);
+  const int wave = threadIdx.x / 64;
+  if ((threadIdx.x & 63) == 0) {
+    s_sum[wave] = sum; s_sq[wave] = sum_sq; s_fin[wave] = finite; s_nf[wave] = nonfinite;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    StatPartial out{0.0, 0.0, 0, 0};
+    for (int w = 0; w < kBlock / 64; ++w) {
+      out.sum += s_sum[w]; out.sum_sq += s_sq[w]; out.finite += s_fin[w]; out.nonfinite += s_nf[w];
+    }
+    partials[static_cast<long long>(f) * gridDim.x + blockIdx.x] = out;
+  }
+}
+
+__global__ void clear_flags_kernel(CalibBatch batch, int count) {
+  const int i = threadIdx.x;
+  if (i < count) *batch.nonfinite[i] = 0u;
+}
+
+__global__ __launch_bounds__(kBlock) void stats_reduce_kernel(const StatPartial* partials, unsigned blocks_per_row,
+                                                               CalibBatch batch, StatPartial* out) {
+  const int row = blockIdx.x;
+  double sum = 0.0, sum_sq = 0.0;
+  unsigned long long finite = 0, nonfinite = 0;
+  for (unsigned b = threadIdx.x; b < blocks_per_row; b += kBlock) {
+    const StatPartial q = partials[static_cast<long long>(row) * blocks_per_row + b];
+    sum += q.sum; sum_sq += q.sum_sq; finite += q.finite; nonfinite += q.nonfinite;
+  }
+  __shared__ double s_sum[kBlock / 64], s_sq[kBlock / 64];
+  __shared__ unsigned long long s_fin[kBlock / 64], s_nf[kBlock / 64];
+  sum = wave_sum(sum); sum_sq = wave_sum(sum_sq); finite = wave_sum(finite); nonfinite = wave_sum(nonfinite);
+  const int wave = threadIdx.x / 64;
+  if ((threadIdx.x & 63) == 0) {
+    s_sum[wave] = sum; s_sq[wave] = sum_sq; s_fin[wave] = finite; s_nf[wave] = nonfinite;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    StatPartial r{0.0, 0.0, 0, 0};
+    for (int w = 0; w < kBlock / 64; ++w) {
+      r.sum += s_sum[w]; r.sum_sq += s_sq[w]; r.finite += s_fin[w]; r.nonfinite += s_nf[w];
+    }
+    out[batch.frame[row]] = r;
+  }
+}
+
+// ------------------------------------------------------------ frame moments
+__global__ __launch_bounds__(kBlock) void frame_moments_kernel(const float* frame, long long npix,
+                                                                FrameMoments* partials) {
+  double sum = 0.0, sum_sq = 0.0;
+  unsigned long long finite = 0, nonfinite = 0;
+  float mn = INFINITY, mx = -INFINITY;
+  const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+  for (long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; i < npix; i += stride) {
+    const float v = frame[i];
+    if (isfinite(v)) {
+      const double x = static_cast<double>(v);
+      sum += x; sum_sq += x * x; ++finite;
+      mn = fminf(mn, v); mx = fmaxf(mx, v);
+    } else {
+      ++nonfinite;
+    }
+  }
+  __shared__ FrameMoments s[kBlock / 64];
+  sum = wave_sum(sum); sum_sq = wave_sum(sum_sq); finite = wave_sum(finite); nonfinite = wave_sum(nonfinite);
+  mn = wave_min(mn); mx = wave_max(mx);
+  const int wave = threadIdx.x / 64;
+  if ((threadIdx.x & 63) == 0) s[wave] = FrameMoments{sum, sum_sq, finite, nonfinite, mn, mx};
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    FrameMoments r{0.0, 0.0, 0, 0, INFINITY, -INFINITY};
+    for (int w = 0; w < kBlock / 64; ++w) {
+      r.sum += s[w].sum; r.sum_sq += s[w].sum_sq; r.finite += s[w].finite; r.nonfinite += s[w].nonfinite;
+      r.min = fminf(r.min, s[w].min); r.max = fmaxf(r.max, s[w].max);
+    }
+    partials[blockIdx.x] = r;
+  }
+}
+
+__global__ __launch_bounds__(64) void frame_moments_final(const FrameMoments* partials, unsigned blocks,
+                                                          FrameMoments* out) {
+  double sum = 0.0, sum_sq = 0.0;
+  unsigned long long finite = 0, nonfinite = 0;
+  float mn = INFINITY, mx = -INFINITY;
+  for (unsigned b = threadIdx.x; b < blocks; b += 64) {
+    const FrameMoments q = partials[b];
+    sum += q.sum; sum_sq += q.sum_sq; finite += q.finite; nonfinite += q.nonfinite;
+    mn = fminf(mn, q.min); mx = fmaxf(mx, q.max);
+  }
+  sum = wave_sum(sum); sum_sq = wave_sum(sum_sq); finite = wave_sum(finite); nonfinite = wave_sum(nonfinite);
+  mn = wave_min(mn); mx = wave_max(mx);
+  if (threadIdx.x == 0) *out = FrameMoments{sum, sum_sq, finite, nonfinite, mn, mx};
+}
+
+// ------------------------------------------------------------ radix select
+// Order-preserving key of a finite float: negative values flip all bits,
+// non-negative values set the sign bit.
+__device__ __forceinline__ unsigned sort_key(float v) {
+  const unsigned b = __float_as_uint(v);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__global__ void zero_u32_kernel(unsigned* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// state[0] = prefix, state[1] = resolved-bit mask, state[2..3] = remaining rank (u64)
+__global__ __launch_bounds__(kBlock) void radix_hist_kernel(const float* frame, long long npix, int abs_mode,
+                                                             float center, int shift, const unsigned* state,
+                                                             unsigned* hist) {
+  __shared__ unsigned local[256];
+  local[threadIdx.x] = 0u;
+  __syncthreads();
+  const unsigned prefix = state[0];
+  const unsigned mask = state[1];
+  const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+  for (long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; i < npix; i += stride) {
+    float v = frame[i];
+    if (!isfinite(v)) continue;
+    if (abs_mode) v = fabsf(v - center);
+    const unsigned k = sort_key(v);
+    if (((k ^ prefix) & mask) == 0u) atomicAdd(&local[(k >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  const unsigned c = local[threadIdx.x];
+  if (c) atomicAdd(&hist[threadIdx.x], c);
+}
+
+__global__ __launch_bounds__(256) void radix_scan_kernel(unsigned* hist, int shift, unsigned* state) {
+  if (threadIdx.x != 0) return;
+  unsigned long long rank = static_cast<unsigned long long>(state[2]) |
+                            (static_cast<unsigned long long>(state[3]) << 32);
+  unsigned long long cum = 0;
+  int bucket = 255;
+  for (int b = 0; b < 256; ++b) {
+    const unsigned long long c = hist[b];
+    if (rank < cum + c) {
+      bucket = b;
+      break;
+    }
+    cum += c;
+  }
+  rank -= cum;
+  state[0] |= static_cast<unsigned>(bucket) << shift;
+  state[1] |= 255u << shift;
+  state[2] = static_cast<unsigned>(rank & 0xffffffffu);
+  state[3] = static_cast<unsigned>(rank >> 32);
+  for (int b = 0; b < 256; ++b) hist[b] = 0u;
+}
+
+__global__ void radix_init_kernel(unsigned* state, unsigned long long rank) {
+  state[0] = 0u;
+  state[1] = 0u;
+  state[2] = static_cast<unsigned>(rank & 0xffffffffu);
+  state[3] = static_cast<unsigned>(rank >> 32);
+}
+
+// --------------------------------------------------------------- utilities
+__global__ void fill_kernel(float* dst, long long n, float value) {
+  const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+  for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = value;
+}
+
+__global__ void add_scalar_kernel(float* dst, long long n, float value) {
+  const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+  for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = dst[i] + value;
+}
+
+__global__ __launch_bounds__(kBlock) void normalize_kernel(float* frame, long long npix, float offset, float divisor,
+                                                           unsigned* flag) {
+  int bad = 0;
+  const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+  for (long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; i < npix; i += stride) {
+    const float v = (frame[i] - offset) / divisor;
+    frame[i] = v;
+    bad |= isfinite(v) ? 0 : 1;
+  }
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
+}
+
+__global__ __launch_bounds__(kBlock) void flag_nonfinite_kernel(const float* frame, long long npix, unsigned* flag) {
+  int bad = 0;
+  const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+  for (long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; i < npix; i += stride)
+    bad |= isfinite(frame[i]) ? 0 : 1;
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
+}
+
+unsigned stream_blocks(long long n) {
+  long long g = (n + kBlock - 1) / kBlock;
+  if (g > 4096) g = 4096;
+  if (g < 1) g = 1;
+  return static_cast<unsigned>(g);
+}
+
+}  // namespace
+
+unsigned calibrate_blocks_per_frame(long long npix) {
+  const int vec = (npix % 4 == 0) ? 4 : 1;
+  long long groups = npix / vec;
+  long long g = (groups + kBlock - 1) / kBlock;
+  // Enough blocks per frame to fill the chip even for one frame, bounded so
+  // each thread loops a few times and the partials stay small.
+  if (g > 2048) g = 2048;
+  if (g < 1) g = 1;
+  return static_cast<unsigned>(g);
+}
+
+void launch_calibrate(const CalibBatch& batch, int count, const CalibParams& p, long long npix,
+                      StatPartial* partials, hipStream_t stream) {
+  const bool vec4 = (npix % 4 == 0);
+  const dim3 grid(calibrate_blocks_per_frame(npix), count);
+#define GH_CAL(FMT, VEC) calibrate_kernel<FMT, VEC><<<grid, kBlock, 0, stream>>>(batch, p, npix, partials)
+  if (vec4) {
+    if (p.format == kFmtF32) GH_CAL(kFmtF32, 4);
+    else if (p.format == kFmtU16) GH_CAL(kFmtU16, 4);
+    else GH_CAL(kFmtBE, 4);
+  } else {
+    if (p.format == kFmtF32) GH_CAL(kFmtF32, 1);
+    else if (p.format == kFmtU16) GH_CAL(kFmtU16, 1);
+    else GH_CAL(kFmtBE, 1);
+  }
+#undef GH_CAL
+  check(hipGetLastError(), "calibrate kernel launch");
+}
+
+void launch_stats_reduce(const StatPartial* partials, const CalibBatch& batch, int rows, unsigned blocks_per_row,
+                         StatPartial* out, hipStream_t stream) {
+  stats_reduce_kernel<<<rows, kBlock, 0, stream>>>(partials, blocks_per_row, batch, out);
+  check(hipGetLastError(), "stats reduce launch");
+}
+
+void launch_clear_flags(unsigned* /*flags*/, const CalibBatch& batch, int count, hipStream_t stream) {
+  clear_flags_kernel<<<1, 64, 0, stream>>>(batch, count);
+  check(hipGetLastError(), "clear flags launch");
+}
+
+unsigned frame_moments_blocks(long long npix) { return stream_blocks(npix); }
+
+void launch_frame_moments(const float* frame, long long npix, FrameMoments* scratch_partials, unsigned blocks,
+                          FrameMoments* out, hipStream_t stream) {
+  frame_moments_kernel<<<blocks, kBlock, 0, stream>>>(frame, npix, scratch_partials);
+  frame_moments_final<<<1, 64, 0, stream>>>(scratch_partials, blocks, out);
+  check(hipGetLastError(), "frame moments launch");
+}
+
+void launch_radix_select(const float* frame, long long npix, int abs_mode, float center,
+                         unsigned long long rank, unsigned* hist, unsigned* state, hipStream_t stream) {
+  zero_u32_kernel<<<1, 256, 0, stream>>>(hist, 256);
+  radix_init_kernel<<<1, 1, 0, stream>>>(state, rank);
+  const unsigned blocks = stream_blocks(npix);
+  for (int shift = 24; shift >= 0; shift -= 8) {
+    radix_hist_kernel<<<blocks, kBlock, 0, stream>>>(frame, npix, abs_mode, center, shift, state, hist);
+    radix_scan_kernel<<<1, 256, 0, stream>>>(hist, shift, state);
+  }
+  check(hipGetLastError(), "radix select launch");
+}
+
+void launch_fill(float* dst, long long n, float value, hipStream_t stream) {
+  fill_kernel<<<stream_blocks(n), kBlock, 0, stream>>>(dst, n, value);
+  check(hipGetLastError(), "fill launch");
+}
+
+void launch_add_scalar(float* dst, long long n, float value, hipStream_t stream) {
+  add_scalar_kernel<<<stream_blocks(n), kBlock, 0, stream>>>(dst, n, value);
+  check(hipGetLastError(), "add scalar launch");
+}
+
+void launch_normalize(float* frame, long long npix, float offset, float divisor, unsigned* flag,
+                      hipStream_t stream) {
+  normalize_kernel<<<stream_blocks(npix), kBlock, 0, stream>>>(frame, npix, offset, divisor, flag);
+  check(hipGetLastError(), "normalize launch");
+}
+
+void launch_flag_nonfinite(const float* frame, long long npix, unsigned* flag, hipStream_t stream) {
+  flag_nonfinite_kernel<<<stream_blocks(npix), kBlock, 0, stream>>>(frame, npix, flag);
+  check(hipGetLastError(), "flag nonfinite launch");
+}
+
+}  // namespace gh

@@ -1,0 +1,703 @@
+// C ABI of libglass_hip.so (declared in include/glass_hip.h).
+//
+// gh_stack mirrors the device state of the reference ResidentCalibratedStack
+// (cpp/src/native_bindings.cpp:2915-2996): an F x H x W float32 frame stack in
+// HBM, optional masters, a geometric warp-coverage accumulator and a stream.
+// Frames are addressed through a slot table so a warp can write a frame into a
+// spare slot and swap, instead of warping into scratch and copying back
+// (native_bindings.cpp:15442-15524 + cpp/cuda/warp_kernels.cu:806-914).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/glass_hip.h"
+#include "common.hpp"
+#include "kernels.hpp"
+
+using gh::check;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& message) {
+  g_last_error = message;
+  return code;
+}
+
+template <typename F>
+int guarded(F&& body) {
+  try {
+    body();
+    return GH_OK;
+  } catch (const gh::ValueError& e) {
+    return fail(GH_ERR_VALUE, e.what());
+  } catch (const gh::IndexError& e) {
+    return fail(GH_ERR_INDEX, e.what());
+  } catch (const std::bad_alloc& e) {
+    return fail(GH_ERR_RUNTIME, std::string("host allocation failed: ") + e.what());
+  } catch (const std::exception& e) {
+    return fail(GH_ERR_RUNTIME, e.what());
+  }
+}
+
+bool is_device_pointer(const void* ptr) {
+  if (ptr == nullptr) return false;
+  hipPointerAttribute_t attr;
+  const hipError_t err = hipPointerGetAttributes(&attr, ptr);
+  if (err != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+template <typename T>
+struct DeviceBuffer {
+  T* ptr = nullptr;
+  size_t count = 0;
+  void ensure(size_t n, const char* what) {
+    if (n <= count && ptr != nullptr) return;
+    release();
+    check(hipMalloc(reinterpret_cast<void**>(&ptr), std::max<size_t>(n, 1) * sizeof(T)), what);
+    count = n;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    count = 0;
+  }
+};
+
+}  // namespace
+
+struct gh_stack {
+  int64_t frames = 0, height = 0, width = 0, npix = 0;
+  float* pool = nullptr;                 // frames x npix allocation
+  std::vector<float*> slot;              // frame -> current slot
+  std::vector<float*> spare;             // free slots for out-of-place warps
+  std::vector<float*> spare_allocations; // separately allocated spare slots
+  std::vector<char> loaded;
+  int64_t loaded_count = 0;
+  unsigned* nonfinite = nullptr;         // [frames] flags
+  float* bias = nullptr;
+  float* dark = nullptr;
+  float* flat = nullptr;
+  float* geometric = nullptr;            // warp coverage accumulator
+  int64_t warp_coverage_frames = 0;
+  hipStream_t stream = nullptr;
+
+  DeviceBuffer<gh::StatPartial> partials;
+  DeviceBuffer<gh::StatPartial> calib_stats;  // [frames]
+  std::vector<char> calib_stats_valid;
+  DeviceBuffer<unsigned char> raw_staging;
+  DeviceBuffer<const float*> frame_table;
+  DeviceBuffer<float> frame_weights;
+  DeviceBuffer<unsigned char> tile_flags;
+  DeviceBuffer<float> out_f32[5];
+  DeviceBuffer<unsigned> out_dq;
+  DeviceBuffer<gh::FrameMoments> moments;
+  DeviceBuffer<unsigned> radix;
+
+  hipStream_t pick(void* s) const { return s ? static_cast<hipStream_t>(s) : stream; }
+
+  size_t frame_bytes() const { return static_cast<size_t>(npix) * sizeof(float); }
+
+  void require_index(int64_t i) const {
+    if (i < 0 || i >= frames) throw gh::IndexError("resident frame index is out of range");
+  }
+  void require_loaded(int64_t i, const char* op) const {
+    require_index(i);
+    if (!loaded[i]) throw std::runtime_error(std::string("resident frame must be loaded before ") + op);
+  }
+  void mark_loaded(int64_t i) {
+    if (!loaded[i]) {
+      loaded[i] = 1;
+      ++loaded_count;
+    }
+  }
+
+  int64_t bytes_allocated() const {
+    int64_t total = static_cast<int64_t>(frames) * static_cast<int64_t>(frame_bytes());
+    total += static_cast<int64_t>(spare_allocations.size()) * static_cast<int64_t>(frame_bytes());
+    if (bias) total += frame_bytes();
+    if (dark) total += frame_bytes();
+    if (flat) total += frame_bytes();
+    if (geometric) total += frame_bytes();
+    return total;
+  }
+
+  void ensure_geometric() {
+    if (geometric) return;
+    check(hipMalloc(reinterpret_cast<void**>(&geometric), frame_bytes()), "hipMalloc(warp coverage)");
+    gh::launch_fill(geometric, npix, 0.0f, stream);
+    warp_coverage_frames = 0;
+  }
+
+  void ensure_spares(size_t n) {
+    while (spare.size() < n) {
+      float* p = nullptr;
+      check(hipMalloc(reinterpret_cast<void**>(&p), frame_bytes()), "hipMalloc(warp spare slot)");
+      spare.push_back(p);
+      spare_allocations.push_back(p);
+    }
+  }
+
+  void upload_master(const float* src, float** dst, const char* what) {
+    if (src == nullptr) {
+      if (*dst) (void)hipFree(*dst);
+      *dst = nullptr;
+      return;
+    }
+    if (*dst == nullptr) check(hipMalloc(reinterpret_cast<void**>(dst), frame_bytes()), what);
+    check(hipMemcpyAsync(*dst, src, frame_bytes(), is_device_pointer(src) ? hipMemcpyDeviceToDevice
+                                                                            : hipMemcpyHostToDevice,
+                         stream),
+          what);
+  }
+
+  ~gh_stack() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    (void)hipFree(pool);
+    for (float* p : spare_allocations) (void)hipFree(p);
+    (void)hipFree(nonfinite);
+    (void)hipFree(bias);
+    (void)hipFree(dark);
+    (void)hipFree(flat);
+    (void)hipFree(geometric);
+    partials.release();
+    calib_stats.release();
+    raw_staging.release();
+    frame_table.release();
+    frame_weights.release();
+    tile_flags.release();
+    for (auto& b : out_f32) b.release();
+    out_dq.release();
+    moments.release();
+    radix.release();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+extern "C" {
+
+int gh_last_error(char* buffer, size_t size) {
+  if (buffer && size) {
+    const size_t n = std::min(size - 1, g_last_error.size());
+    std::memcpy(buffer, g_last_error.data(), n);
+    buffer[n] = '\0';
+  }
+  return static_cast<int>(g_last_error.size());
+}
+
+int gh_version(void) { return 1; }
+
+int gh_device_count(int* count) {
+  return guarded([&] {
+    int n = 0;
+    check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    *count = n;
+  });
+}
+
+int gh_stack_create(int64_t frame_count, int64_t height, int64_t width, gh_stack** out) {
+  return guarded([&] {
+    if (!out) throw gh::ValueError("output handle pointer is NULL");
+    *out = nullptr;
+    if (frame_count <= 0 || height <= 0 || width <= 0)
+      throw gh::ValueError("resident stack dimensions must be non-empty");
+    if (height > (1 << 30) || width > (1 << 30)) throw gh::ValueError("resident stack dimensions are too large");
+    auto* s = new gh_stack();
+    try {
+      s->frames = frame_count;
+      s->height = height;
+      s->width = width;
+      s->npix = height * width;
+      check(hipStreamCreate(&s->stream), "hipStreamCreate(resident stack)");
+      check(hipMalloc(reinterpret_cast<void**>(&s->pool), static_cast<size_t>(frame_count) * s->frame_bytes()),
+            "hipMalloc(resident calibrated stack)");
+      check(hipMalloc(reinterpret_cast<void**>(&s->nonfinite), static_cast<size_t>(frame_count) * sizeof(unsigned)),
+            "hipMalloc(nonfinite flags)");
+      check(hipMemsetAsync(s->nonfinite, 0, static_cast<size_t>(frame_count) * sizeof(unsigned), s->stream),
+            "hipMemset(nonfinite flags)");
+      s->slot.resize(frame_count);
+      for (int64_t i = 0; i < frame_count; ++i) s->slot[i] = s->pool + i * s->npix;
+      s->loaded.assign(frame_count, 0);
+      s->calib_stats_valid.assign(frame_count, 0);
+      check(hipStreamSynchronize(s->stream), "resident stack init");
+    } catch (...) {
+      delete s;
+      throw;
+    }
+    *out = s;
+  });
+}
+
+int gh_stack_destroy(gh_stack* stack) {
+  return guarded([&] { delete stack; });
+}
+
+int gh_stack_info(const gh_stack* s, int64_t out[8]) {
+  return guarded([&] {
+    if (!s) throw gh::ValueError("stack handle is NULL");
+    out[0] = s->frames;
+    out[1] = s->height;
+    out[2] = s->width;
+    out[3] = s->npix;
+    out[4] = s->loaded_count;
+    out[5] = s->bytes_allocated();
+    out[6] = s->warp_coverage_frames;
+    out[7] = static_cast<int64_t>(s->spare_allocations.size()) * static_cast<int64_t>(s->frame_bytes());
+  });
+}
+
+int gh_stack_frame_ptr(gh_stack* s, int64_t index, float** device_ptr) {
+  return guarded([&] {
+    s->require_index(index);
+    *device_ptr = s->slot[index];
+  });
+}
+
+int gh_stack_frame_nonfinite(gh_stack* s, int64_t index, int64_t* count) {
+  return guarded([&] {
+    s->require_index(index);
+    unsigned flag = 0;
+    check(hipStreamSynchronize(s->stream), "synchronize(nonfinite flag)");
+    check(hipMemcpy(&flag, s->nonfinite + index, sizeof(unsigned), hipMemcpyDeviceToHost), "hipMemcpy(nonfinite)");
+    *count = flag;
+  });
+}
+
+int gh_set_masters(gh_stack* s, const float* bias, const float* dark, const float* flat) {
+  return guarded([&] {
+    s->upload_master(bias, &s->bias, "upload bias master");
+    s->upload_master(dark, &s->dark, "upload dark master");
+    s->upload_master(flat, &s->flat, "upload flat master");
+    check(hipStreamSynchronize(s->stream), "set_calibration_masters");
+  });
+}
+
+int gh_upload_frame(gh_stack* s, int64_t index, const float* frame) {
+  return guarded([&] {
+    s->require_index(index);
+    if (!frame) throw gh::ValueError("frame pointer is NULL");
+    check(hipMemcpyAsync(s->slot[index], frame, s->frame_bytes(),
+                         is_device_pointer(frame) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s->stream),
+          "hipMemcpy(resident calibrated frame)");
+    check(hipMemsetAsync(s->nonfinite + index, 0, sizeof(unsigned), s->stream), "reset nonfinite flag");
+    gh::launch_flag_nonfinite(s->slot[index], s->npix, s->nonfinite + index, s->stream);
+    s->calib_stats_valid[index] = 0;
+    s->mark_loaded(index);
+    check(hipStreamSynchronize(s->stream), "upload_calibrated_frame");
+  });
+}
+
+int gh_download_tile(gh_stack* s, const int64_t* indices, int64_t count, int64_t x0, int64_t y0, int64_t x1,
+                     int64_t y1, float* out) {
+  return guarded([&] {
+    if (count <= 0) throw gh::ValueError("indices must contain at least one frame index");
+    if (x0 < 0 || y0 < 0 || x0 >= x1 || y0 >= y1 || x1 > s->width || y1 > s->height)
+      throw gh::IndexError("resident frame tile bounds are invalid");
+    const size_t tw = static_cast<size_t>(x1 - x0), th = static_cast<size_t>(y1 - y0);
+    check(hipStreamSynchronize(s->stream), "synchronize(download)");
+    for (int64_t k = 0; k < count; ++k) {
+      s->require_loaded(indices[k], "resident frame tile download");
+      const float* src = s->slot[indices[k]] + y0 * s->width + x0;
+      check(hipMemcpy2D(out + k * tw * th, tw * sizeof(float), src, static_cast<size_t>(s->width) * sizeof(float),
+                        tw * sizeof(float), th, hipMemcpyDeviceToHost),
+            "hipMemcpy2D(resident frame tile)");
+    }
+  });
+}
+
+int gh_calibrate_frames(gh_stack* s, const int64_t* indices, int64_t count, const void* raw,
+                        int64_t raw_frame_stride_bytes, int32_t format, const float* dark_scales,
+                        const gh_calib_policy* policy, void* stream_ptr) {
+  return guarded([&] {
+    if (count <= 0) return;
+    if (!raw) throw gh::ValueError("raw frame pointer is NULL");
+    if (format != GH_FMT_F32 && format != GH_FMT_U16 && format != GH_FMT_FITS_U16BE_BZERO)
+      throw gh::ValueError("unsupported raw sample format");
+    const int64_t sample_bytes = (format == GH_FMT_F32) ? 4 : 2;
+    const int64_t frame_raw_bytes = s->npix * sample_bytes;
+    if (raw_frame_stride_bytes < frame_raw_bytes) throw gh::ValueError("raw frame stride is smaller than a frame");
+    for (int64_t k = 0; k < count; ++k) s->require_index(indices[k]);
+    gh_calib_policy pol{1, 1.0e-6f, 0.0f, 0};
+    if (policy) pol = *policy;
+    hipStream_t st = s->pick(stream_ptr);
+    const bool device_raw = is_device_pointer(raw);
+    gh::CalibParams p{};
+    p.bias = s->bias;
+    p.dark = s->dark;
+    p.flat = s->flat;
+    p.has_bias = s->bias != nullptr;
+    p.has_dark = s->dark != nullptr;
+    p.has_flat = s->flat != nullptr;
+    p.dark_includes_bias = pol.master_dark_includes_bias;
+    p.flat_floor = pol.flat_floor;
+    p.pedestal = pol.pedestal;
+    p.format = format;
+    p.accumulate_stats = pol.accumulate_stats;
+    const bool aligned = (s->npix % 4 == 0) && (raw_frame_stride_bytes % 16 == 0) &&
+                         (reinterpret_cast<uintptr_t>(raw) % 16 == 0);
+    if (!aligned && s->npix % 4 == 0 && device_raw)
+      throw gh::ValueError("device raw frames must be 16-byte aligned with a 16-byte multiple stride");
+    const unsigned blocks = gh::calibrate_blocks_per_frame(s->npix);
+    if (pol.accumulate_stats) {
+      s->partials.ensure(static_cast<size_t>(gh::kCalibBatch) * blocks, "hipMalloc(calibration partials)");
+      s->calib_stats.ensure(static_cast<size_t>(s->frames), "hipMalloc(calibration stats)");
+    }
+    const int64_t batch_cap = device_raw ? gh::kCalibBatch : 1;
+    if (!device_raw) s->raw_staging.ensure(static_cast<size_t>(frame_raw_bytes + 16), "hipMalloc(raw staging)");
+    for (int64_t start = 0; start < count; start += batch_cap) {
+      const int n = static_cast<int>(std::min<int64_t>(batch_cap, count - start));
+      gh::CalibBatch b{};
+      for (int j = 0; j < n; ++j) {
+        const int64_t k = start + j;
+        const int64_t f = indices[k];
+        const unsigned char* src = static_cast<const unsigned char*>(raw) + k * raw_frame_stride_bytes;
+        if (!device_raw) {
+          check(hipMemcpyAsync(s->raw_staging.ptr, src, frame_raw_bytes, hipMemcpyHostToDevice, st),
+                "hipMemcpy(raw light)");
+          src = s->raw_staging.ptr;
+        }
+        b.src[j] = src;
+        b.dst[j] = s->slot[f];
+        b.dark_scale[j] = dark_scales ? dark_scales[k] : 1.0f;
+        b.nonfinite[j] = s->nonfinite + f;
+        b.frame[j] = static_cast<int>(f);
+      }
+      gh::launch_clear_flags(s->nonfinite, b, n, st);
+      gh::launch_calibrate(b, n, p, s->npix, s->partials.ptr, st);
+      if (pol.accumulate_stats) gh::launch_stats_reduce(s->partials.ptr, b, n, blocks, s->calib_stats.ptr, st);
+      for (int j = 0; j < n; ++j) {
+        const int64_t f = indices[start + j];
+        s->calib_stats_valid[f] = pol.accumulate_stats ? 1 : 0;
+        s->mark_loaded(f);
+      }
+      if (!device_raw) check(hipStreamSynchronize(st), "calibrate (host raw)");
+    }
+  });
+}
+
+int gh_normalize_frames(gh_stack* s, const int64_t* indices, int64_t count, const float* offsets,
+                        const float* divisors, void* stream_ptr) {
+  return guarded([&] {
+    if (count <= 0) return;
+    if (!offsets || !divisors) throw gh::ValueError("offsets and divisors must not be NULL");
+    hipStream_t st = s->pick(stream_ptr);
+    for (int64_t k = 0; k < count; ++k) {
+      const int64_t f = indices[k];
+      s->require_loaded(f, "frame normalisation");
+      // The frame keeps its non-finite flag; normalisation can only add to it.
+      gh::launch_normalize(s->slot[f], s->npix, offsets[k], divisors[k], s->nonfinite + f, st);
+      s->calib_stats_valid[f] = 0;
+    }
+    if (!stream_ptr) check(hipStreamSynchronize(st), "normalize_frames");
+  });
+}
+
+int gh_warp_frames(gh_stack* s, const int64_t* indices, int64_t count, const double* inverse, int32_t interpolation,
+                   float fill, float clamping_threshold, uint32_t flags, void* stream_ptr) {
+  return guarded([&] {
+    if (count <= 0) return;
+    if (interpolation != GH_INTERP_BILINEAR && interpolation != GH_INTERP_LANCZOS3)
+      throw gh::ValueError("unsupported warp interpolation");
+    if (!inverse) throw gh::ValueError("inverse matrices pointer is NULL");
+    for (int64_t k = 0; k < count; ++k) {
+      s->require_loaded(indices[k], "resident matrix warp");
+      for (int64_t j = 0; j < k; ++j)
+        if (indices[j] == indices[k]) throw gh::ValueError("warp indices must be unique");
+      for (int e = 0; e < 9; ++e)
+        if (!std::isfinite(inverse[k * 9 + e])) throw gh::ValueError("warp inverse matrix must be finite");
+    }
+    hipStream_t st = s->pick(stream_ptr);
+    gh::WarpParams p{};
+    p.height = static_cast<int>(s->height);
+    p.width = static_cast<int>(s->width);
+    p.interpolation = interpolation;
+    p.fill = fill;
+    p.clamp = clamping_threshold;
+    p.skip_nonfinite = (flags & GH_WARP_SKIP_NONFINITE_TAPS) ? 1 : 0;
+    p.affine = (flags & GH_WARP_AFFINE) ? 1 : 0;
+    if (p.affine) {
+      for (int64_t k = 0; k < count; ++k) {
+        const double* m = inverse + k * 9;
+        if (!(m[6] == 0.0 && m[7] == 0.0 && m[8] == 1.0))
+          throw gh::ValueError("GH_WARP_AFFINE requires an inverse whose last row is (0, 0, 1)");
+      }
+    }
+    if (flags & GH_WARP_TRACK_COVERAGE) {
+      s->ensure_geometric();
+      p.coverage = s->geometric;
+    }
+    s->ensure_spares(static_cast<size_t>(std::min<int64_t>(count, gh::kWarpBatch)));
+    for (int64_t start = 0; start < count; start += gh::kWarpBatch) {
+      const int n = static_cast<int>(std::min<int64_t>(gh::kWarpBatch, count - start));
+      gh::WarpBatch b{};
+      for (int j = 0; j < n; ++j) {
+        const int64_t f = indices[start + j];
+        b.src[j] = s->slot[f];
+        b.dst[j] = s->spare[j];
+        std::memcpy(b.inv[j], inverse + (start + j) * 9, 9 * sizeof(double));
+        b.nonfinite[j] = s->nonfinite + f;
+      }
+      gh::CalibBatch clear{};
+      for (int j = 0; j < n; ++j) clear.nonfinite[j] = b.nonfinite[j];
+      gh::launch_clear_flags(s->nonfinite, clear, n, st);
+      gh::launch_warp(b, n, p, st);
+      for (int j = 0; j < n; ++j) {
+        const int64_t f = indices[start + j];
+        std::swap(s->slot[f], s->spare[j]);
+        s->calib_stats_valid[f] = 0;
+      }
+      if (flags & GH_WARP_TRACK_COVERAGE) s->warp_coverage_frames += n;
+    }
+  });
+}
+
+int gh_reset_warp_coverage(gh_stack* s) {
+  return guarded([&] {
+    s->ensure_geometric();
+    gh::launch_fill(s->geometric, s->npix, 0.0f, s->stream);
+    s->warp_coverage_frames = 0;
+    check(hipStreamSynchronize(s->stream), "reset_warp_coverage");
+  });
+}
+
+int gh_accumulate_full_warp_coverage(gh_stack* s) {
+  return guarded([&] {
+    s->ensure_geometric();
+    gh::launch_add_scalar(s->geometric, s->npix, 1.0f, s->stream);
+    ++s->warp_coverage_frames;
+    check(hipStreamSynchronize(s->stream), "accumulate_full_warp_coverage_frame");
+  });
+}
+
+int gh_warp_coverage_map(gh_stack* s, float* out) {
+  return guarded([&] {
+    if (!out) throw gh::ValueError("output pointer is NULL");
+    check(hipStreamSynchronize(s->stream), "synchronize(warp coverage)");
+    if (!s->geometric) {
+      if (is_device_pointer(out)) {
+        gh::launch_fill(out, s->npix, 0.0f, s->stream);
+        check(hipStreamSynchronize(s->stream), "warp_coverage_map");
+      } else {
+        std::fill(out, out + s->npix, 0.0f);
+      }
+      return;
+    }
+    check(hipMemcpy(out, s->geometric, s->frame_bytes(), is_device_pointer(out) ? hipMemcpyDeviceToDevice
+                                                                                 : hipMemcpyDeviceToHost),
+          "hipMemcpy(warp coverage map)");
+  });
+}
+
+int gh_integrate(gh_stack* s, const float* weights, const gh_integrate_params* params,
+                 const gh_integrate_outputs* outputs, void* stream_ptr) {
+  return guarded([&] {
+    if (!params || !outputs) throw gh::ValueError("params and outputs must not be NULL");
+    if (s->loaded_count != s->frames)
+      throw std::runtime_error("all resident frames must be loaded before integration");
+    const gh_integrate_params& prm = *params;
+    if (prm.rejection != GH_REJECT_NONE && prm.rejection != GH_REJECT_WINSORIZED_SIGMA)
+      throw gh::ValueError("unsupported rejection mode");
+    if (prm.rejection == GH_REJECT_WINSORIZED_SIGMA) {
+      if (!(prm.low_sigma > 0.0f) || !(prm.high_sigma > 0.0f))
+        throw gh::ValueError("low_sigma and high_sigma must be positive");
+      if (prm.min_samples < 1) throw gh::ValueError("min_samples must be at least 1");
+      if (!(prm.max_reject_fraction >= 0.0f && prm.max_reject_fraction <= 1.0f))
+        throw gh::ValueError("max_reject_fraction must be between 0 and 1");
+    }
+    if (prm.count_dtype != GH_COUNT_F32 && prm.count_dtype != GH_COUNT_U16)
+      throw gh::ValueError("count_map_dtype must be float32 or uint16");
+    hipStream_t st = s->pick(stream_ptr);
+
+    // Active frames: finite, positive weights (cpu/integration.py:143), frame order.
+    std::vector<const float*> table;
+    std::vector<float> w;
+    bool unit = true;
+    for (int64_t f = 0; f < s->frames; ++f) {
+      const float wf = weights ? weights[f] : 1.0f;
+      if (!(std::isfinite(wf) && wf > 0.0f)) continue;
+      table.push_back(s->slot[f]);
+      w.push_back(wf);
+      unit = unit && (wf == 1.0f);
+    }
+    const int n_active = static_cast<int>(table.size());
+    if (prm.rejection == GH_REJECT_WINSORIZED_SIGMA && n_active > gh::integrate_max_frames())
+      throw gh::ValueError("winsorized integration supports at most 256 positive-weight frames");
+    if (s->npix >= (int64_t(1) << 30)) throw gh::ValueError("frames of 2^30 pixels or more are not supported");
+    // The winsorized kernel reads table entries [0, capacity) unconditionally
+    // (compile-time indices); pad to the largest capacity with a valid frame
+    // (the last active one, or frame 0 when none is active) and weight 0.
+    const size_t padded = static_cast<size_t>(std::max(gh::integrate_max_frames(), n_active));
+    const float* pad_ptr = n_active ? table.back() : s->slot[0];
+    table.resize(padded, pad_ptr);
+    w.resize(padded, 0.0f);
+    s->frame_table.ensure(padded, "hipMalloc(frame table)");
+    s->frame_weights.ensure(padded, "hipMalloc(frame weights)");
+    check(hipMemcpy(s->frame_table.ptr, table.data(), padded * sizeof(const float*), hipMemcpyHostToDevice),
+          "hipMemcpy(frame table)");
+    check(hipMemcpy(s->frame_weights.ptr, w.data(), padded * sizeof(float), hipMemcpyHostToDevice),
+          "hipMemcpy(frame weights)");
+
+    // Statistics arithmetic (rejection.py:39-44): all-valid only when every
+    // weight is positive and every sample of every frame is finite.
+    bool any_nonfinite = false;
+    {
+      std::vector<unsigned> flags(static_cast<size_t>(s->frames));
+      check(hipStreamSynchronize(st), "synchronize(nonfinite flags)");
+      check(hipMemcpy(flags.data(), s->nonfinite, flags.size() * sizeof(unsigned), hipMemcpyDeviceToHost),
+            "hipMemcpy(nonfinite flags)");
+      for (int64_t f = 0; f < s->frames; ++f) {
+        const float wf = weights ? weights[f] : 1.0f;
+        if (std::isfinite(wf) && wf > 0.0f && flags[f]) any_nonfinite = true;
+      }
+    }
+    gh::IntegrateArgs a{};
+    a.frame_ptr = s->frame_table.ptr;
+    a.frame_w = s->frame_weights.ptr;
+    a.n_active = n_active;
+    a.npix = s->npix;
+    a.width = static_cast<int>(s->width);
+    a.rejection = prm.rejection;
+    a.low_sigma = prm.low_sigma;
+    a.high_sigma = prm.high_sigma;
+    a.min_samples = prm.min_samples;
+    a.max_frac = prm.max_reject_fraction;
+    a.unit_weights = unit ? 1 : 0;
+    a.require_positive_scale = prm.require_positive_scale;
+    const bool all_weights_positive = (n_active == s->frames);
+    if (prm.statistics_variant >= 0) {
+      a.variant_all_valid = prm.statistics_variant;
+    } else if (prm.tile_size > 0 && all_weights_positive && any_nonfinite) {
+      const int tiles_x = static_cast<int>((s->width + prm.tile_size - 1) / prm.tile_size);
+      const int tiles_y = static_cast<int>((s->height + prm.tile_size - 1) / prm.tile_size);
+      s->tile_flags.ensure(static_cast<size_t>(tiles_x) * tiles_y, "hipMalloc(tile flags)");
+      gh::launch_tile_all_finite(s->frame_table.ptr, n_active, static_cast<int>(s->height),
+                                 static_cast<int>(s->width), prm.tile_size, s->tile_flags.ptr, st);
+      a.tile_flags = s->tile_flags.ptr;
+      a.tile_size = prm.tile_size;
+      a.tiles_x = tiles_x;
+    } else {
+      a.variant_all_valid = (all_weights_positive && !any_nonfinite) ? 1 : 0;
+    }
+
+    // Outputs: device pointers are written in place; host pointers go through
+    // device temporaries and are copied back.
+    const gh_integrate_outputs& o = *outputs;
+    const size_t count_bytes = (prm.count_dtype == GH_COUNT_U16) ? 2 : 4;
+    void* host_dst[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    size_t host_bytes[6] = {0, 0, 0, 0, 0, 0};
+    auto route = [&](void* user, int slot, size_t elem_bytes) -> void* {
+      if (!user) return nullptr;
+      if (is_device_pointer(user)) return user;
+      host_dst[slot] = user;
+      host_bytes[slot] = elem_bytes * static_cast<size_t>(s->npix);
+      if (slot < 5) {
+        s->out_f32[slot].ensure(static_cast<size_t>(s->npix), "hipMalloc(integration output)");
+        return s->out_f32[slot].ptr;
+      }
+      s->out_dq.ensure(static_cast<size_t>(s->npix), "hipMalloc(dq output)");
+      return s->out_dq.ptr;
+    };
+    a.master = static_cast<float*>(route(o.master, 0, 4));
+    a.weight = static_cast<float*>(route(o.weight, 1, 4));
+    a.coverage = route(o.coverage, 2, count_bytes);
+    a.low = route(o.low, 3, count_bytes);
+    a.high = route(o.high, 4, count_bytes);
+    a.dq = static_cast<unsigned*>(route(o.dq, 5, 4));
+    a.count_u16 = (prm.count_dtype == GH_COUNT_U16) ? 1 : 0;
+    if (o.dq && o.dq_use_geometric) {
+      if (!s->geometric) throw gh::ValueError("DQ with geometric coverage requested but no warp coverage exists");
+      a.geometric = s->geometric;
+    }
+    a.dq_active_frames = o.dq_active_frames;
+    gh::launch_integrate(a, st);
+    const void* dev_src[6] = {a.master, a.weight, a.coverage, a.low, a.high, a.dq};
+    bool any_host = false;
+    for (int i = 0; i < 6; ++i) {
+      if (!host_dst[i]) continue;
+      any_host = true;
+      check(hipMemcpyAsync(host_dst[i], dev_src[i], host_bytes[i], hipMemcpyDeviceToHost, st),
+            "hipMemcpy(integration output)");
+    }
+    if (any_host || !stream_ptr) check(hipStreamSynchronize(st), "integrate synchronize");
+  });
+}
+
+int gh_frame_stats(gh_stack* s, int64_t index, int32_t exact_median, gh_frame_statistics* out) {
+  return guarded([&] {
+    s->require_loaded(index, "global frame statistics");
+    if (!out) throw gh::ValueError("output pointer is NULL");
+    const float* frame = s->slot[index];
+    const unsigned blocks = gh::frame_moments_blocks(s->npix);
+    s->moments.ensure(blocks + 1, "hipMalloc(frame moments)");
+    gh::launch_frame_moments(frame, s->npix, s->moments.ptr + 1, blocks, s->moments.ptr, s->stream);
+    gh::FrameMoments m{};
+    check(hipMemcpyAsync(&m, s->moments.ptr, sizeof(m), hipMemcpyDeviceToHost, s->stream), "hipMemcpy(moments)");
+    check(hipStreamSynchronize(s->stream), "frame stats");
+    const int64_t n = static_cast<int64_t>(m.finite);
+    out->sum = m.sum;
+    out->sum_sq = m.sum_sq;
+    out->valid_pixels = n;
+    out->total_pixels = s->npix;
+    out->nonfinite_pixels = s->npix - n;
+    out->mean = n ? m.sum / static_cast<double>(n) : 0.0;
+    double var = n ? m.sum_sq / static_cast<double>(n) - out->mean * out->mean : 0.0;
+    if (var < 0.0) var = 0.0;
+    out->std = std::sqrt(var);
+    out->min = n ? m.min : 0.0f;
+    out->max = n ? m.max : 0.0f;
+    out->median = 0.0;
+    out->mad = 0.0;
+    if (!exact_median || n == 0) return;
+    s->radix.ensure(256 + 4, "hipMalloc(radix select)");
+    auto select = [&](int abs_mode, float center, uint64_t rank) -> float {
+      gh::launch_radix_select(frame, s->npix, abs_mode, center, rank, s->radix.ptr, s->radix.ptr + 256, s->stream);
+      unsigned key = 0;
+      check(hipMemcpyAsync(&key, s->radix.ptr + 256, sizeof(unsigned), hipMemcpyDeviceToHost, s->stream),
+            "hipMemcpy(radix key)");
+      check(hipStreamSynchronize(s->stream), "radix select");
+      const unsigned bits = (key & 0x80000000u) ? (key & 0x7fffffffu) : ~key;
+      float v;
+      std::memcpy(&v, &bits, sizeof(v));
+      return v;
+    };
+    // quality.py:170-180: partition median, even counts average the middle pair in FP64.
+    const uint64_t mid = static_cast<uint64_t>(n / 2);
+    double median = select(0, 0.0f, mid);
+    if (n % 2 == 0) median = (static_cast<double>(select(0, 0.0f, mid - 1)) + median) / 2.0;
+    const float center = static_cast<float>(median);
+    double mad = select(1, center, mid);
+    if (n % 2 == 0) mad = (static_cast<double>(select(1, center, mid - 1)) + mad) / 2.0;
+    out->median = median;
+    out->mad = mad;
+  });
+}
+
+int gh_calibration_stats(gh_stack* s, int64_t index, double* sum, double* sum_sq, int64_t* finite) {
+  return guarded([&] {
+    s->require_index(index);
+    if (!s->calib_stats_valid[index] || !s->calib_stats.ptr)
+      throw gh::ValueError("no fused calibration statistics for this frame");
+    gh::StatPartial r{};
+    check(hipStreamSynchronize(s->stream), "synchronize(calibration stats)");
+    check(hipMemcpy(&r, s->calib_stats.ptr + index, sizeof(r), hipMemcpyDeviceToHost), "hipMemcpy(calib stats)");
+    *sum = r.sum;
+    *sum_sq = r.sum_sq;
+    *finite = static_cast<int64_t>(r.finite);
+  });
+}
+
+int gh_synchronize(gh_stack* s) {
+  return guarded([&] { check(hipStreamSynchronize(s->stream), "synchronize"); });
+}
+
+}  // extern "C"

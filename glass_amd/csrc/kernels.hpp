@@ -1,0 +1,141 @@
+// Host-side launchers of the gfx950 kernels (one .hip translation unit each).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gh {
+
+// ---------------------------------------------------------------- calibrate
+constexpr int kCalibBatch = 16;
+
+struct CalibBatch {
+  const void* src[kCalibBatch];  // raw frame k
+  float* dst[kCalibBatch];       // destination slot of frame k
+  float dark_scale[kCalibBatch];
+  unsigned* nonfinite[kCalibBatch];  // frame k's "holds a non-finite sample" flag (pre-cleared)
+  int frame[kCalibBatch];            // frame index of batch entry k (statistics row)
+};
+
+struct CalibParams {
+  const float* bias;
+  const float* dark;
+  const float* flat;
+  int has_bias, has_dark, has_flat;
+  int dark_includes_bias;
+  float flat_floor;
+  float pedestal;
+  int format;  // GH_FMT_*
+  int accumulate_stats;
+};
+
+// Per-block partial statistics, reduced in fixed order by frame_stats_reduce.
+struct StatPartial {
+  double sum;
+  double sum_sq;
+  unsigned long long finite;
+  unsigned long long nonfinite;
+};
+
+// Blocks per frame used by the calibrate kernels for `npix` pixels.
+unsigned calibrate_blocks_per_frame(long long npix);
+
+void launch_calibrate(const CalibBatch& batch, int count, const CalibParams& p, long long npix,
+                      StatPartial* partials, hipStream_t stream);
+
+// Reduces rows [0, rows) of `partials` (blocks_per_row each) into
+// out[batch.frame[row]], in a fixed order (deterministic FP64 sums).
+void launch_stats_reduce(const StatPartial* partials, const CalibBatch& batch, int rows, unsigned blocks_per_row,
+                         StatPartial* out, hipStream_t stream);
+
+// Clears flags[i] for the listed frames (kernel, so it is graph-capturable).
+void launch_clear_flags(unsigned* flags, const CalibBatch& batch, int count, hipStream_t stream);
+
+// Per-frame sum / sum^2 / finite / nonfinite / min / max of an f32 frame.
+struct FrameMoments {
+  double sum;
+  double sum_sq;
+  unsigned long long finite;
+  unsigned long long nonfinite;
+  float min;
+  float max;
+};
+void launch_frame_moments(const float* frame, long long npix, FrameMoments* scratch_partials,
+                          unsigned blocks, FrameMoments* out, hipStream_t stream);
+unsigned frame_moments_blocks(long long npix);
+
+// Exact k-th smallest finite value of |x - center| (abs_mode) or x by 4-pass
+// 8-bit radix select over order-preserving keys; hist must hold 256 u32.
+void launch_radix_select(const float* frame, long long npix, int abs_mode, float center,
+                         unsigned long long rank, unsigned* hist, unsigned* state, hipStream_t stream);
+
+// -------------------------------------------------------------------- warp
+constexpr int kWarpBatch = 8;
+
+struct WarpBatch {
+  const float* src[kWarpBatch];
+  float* dst[kWarpBatch];
+  double inv[kWarpBatch][9];
+  unsigned* nonfinite[kWarpBatch];  // per-frame "holds a non-finite sample" flag (device)
+};
+
+struct WarpParams {
+  int height, width;
+  int interpolation;  // 1 bilinear, 3 lanczos3
+  float fill;
+  float clamp;        // < 0: off
+  int skip_nonfinite;
+  int affine;         // 1: no projective divide
+  float* coverage;    // geometric coverage accumulator (nullable)
+};
+
+void launch_warp(const WarpBatch& batch, int count, const WarpParams& p, hipStream_t stream);
+
+void launch_fill(float* dst, long long n, float value, hipStream_t stream);
+void launch_add_scalar(float* dst, long long n, float value, hipStream_t stream);
+// x = (x - offset) / divisor in place (f32), flagging non-finite results.
+void launch_normalize(float* frame, long long npix, float offset, float divisor, unsigned* nonfinite_flag,
+                      hipStream_t stream);
+
+// Sets *flag to 1 when the frame holds a non-finite sample (never clears it).
+void launch_flag_nonfinite(const float* frame, long long npix, unsigned* flag, hipStream_t stream);
+
+// --------------------------------------------------------------- integrate
+struct IntegrateArgs {
+  const float* const* frame_ptr; // [n_active] device slot pointers, frame order
+  const float* frame_w;          // [n_active] positive finite weights
+  int n_active;
+  long long npix;
+  int width;
+  int rejection;                 // 0 none, 1 winsorized
+  float low_sigma, high_sigma;
+  int min_samples;
+  float max_frac;
+  int unit_weights;
+  int variant_all_valid;         // statistics arithmetic when tile_flags == nullptr
+  const unsigned char* tile_flags;  // per tile: 1 all-valid arithmetic (nullable)
+  int tile_size, tiles_x;
+  int require_positive_scale;
+  // outputs (device, nullable)
+  float* master;
+  float* weight;
+  void* coverage;
+  void* low;
+  void* high;
+  int count_u16;
+  unsigned* dq;
+  const float* geometric;        // nullable
+  int dq_active_frames;
+};
+
+// Returns the register capacity (max frames) the winsorized kernel supports.
+int integrate_max_frames();
+void launch_integrate(const IntegrateArgs& a, hipStream_t stream);
+
+// Per-tile "any non-finite sample" flags over n_active frames (for the
+// CPUStackEngine per-tile statistics choice); flags[t] = 1 if all finite.
+void launch_tile_all_finite(const float* const* frame_ptr, int n_active, int height, int width, int tile_size,
+                            unsigned char* flags, hipStream_t stream);
+
+}  // namespace gh

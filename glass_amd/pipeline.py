@@ -1,0 +1,231 @@
+"""Host driver of the resident stacking path (masters -> calibrate -> warp ->
+weights -> reject/integrate -> DQ), mirroring the stage order of the
+reference's ``run_resident_calibration_integration`` (src/glass/engine/
+resident_cuda.py:7119) on top of ``ResidentCalibratedStack``.
+
+Semantics follow the CPU path, which is the oracle:
+* masters: src/glass/engine/pipeline.py:517-1047 -- winsorized mean stack with
+  the master rejection policy, per-tile statistics choice; darks minus bias
+  when they exclude it; flats reduced by the bias master, divided by their own
+  exact median, stacked, floored;
+* light calibration: src/glass/cpu/calibration.py:51-76;
+* warp: identity matrices skip resampling (resident_cuda.py:2751-2754), others
+  are resampled on the reference grid; geometric coverage is accumulated;
+* weights: engine/integration.py:80-106 (none / variance_aware / simple_snr),
+  with zero weight for masked frames;
+* integration: cpu/integration.py:112-194; DQ: resident_cuda.py:3110-3297.
+All arithmetic runs in libglass_hip.so; numpy only shapes host inputs/outputs.
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+from typing import Sequence
+
+import numpy as np
+
+from .resident import ResidentCalibratedStack
+
+DEFAULT_POLICY = {
+    "master_dark_includes_bias": True,
+    "dark_scaling_enabled": True,
+    "flat_floor": 1.0e-6,
+    "pedestal": 0.0,
+    "master_rejection_low_sigma": 3.0,
+    "master_rejection_high_sigma": 3.0,
+    "master_rejection_min_samples": 3,
+    "master_rejection_max_fraction": 0.5,
+    "tile_size": 512,
+}
+
+
+def _policy(policy: dict | None) -> dict:
+    out = dict(DEFAULT_POLICY)
+    out.update(policy or {})
+    return out
+
+
+def _stack_frames(frames) -> ResidentCalibratedStack:
+    frames = list(frames)
+    if not frames:
+        raise ValueError("cannot stack an empty frame list")
+    h, w = np.asarray(frames[0]).shape if not hasattr(frames[0], "shape") else tuple(frames[0].shape)
+    stack = ResidentCalibratedStack(len(frames), int(h), int(w))
+    for i, frame in enumerate(frames):
+        stack.upload_calibrated_frame(i, frame)
+    return stack
+
+
+def _master_integrate(stack: ResidentCalibratedStack, pol: dict) -> np.ndarray:
+    master, *_ = stack.integrate_stack_engine(
+        low_sigma=float(pol["master_rejection_low_sigma"]),
+        high_sigma=float(pol["master_rejection_high_sigma"]),
+        min_samples=int(pol["master_rejection_min_samples"]),
+        max_reject_fraction=float(pol["master_rejection_max_fraction"]),
+        tile_size=int(pol["tile_size"]),
+    )
+    return master
+
+
+def build_master_bias(frames, policy: dict | None = None) -> np.ndarray:
+    """pipeline.py:1186-1200 -> _stack_mean_master (:647, :529-610)."""
+    pol = _policy(policy)
+    with _stack_frames(frames) as stack:
+        return _master_integrate(stack, pol)
+
+
+def build_master_dark(frames, master_bias=None, policy: dict | None = None) -> np.ndarray:
+    """pipeline.py:1222-1246: darks stacked, then minus the bias master when
+    the darks do not include bias (f32, :589-593)."""
+    pol = _policy(policy)
+    with _stack_frames(frames) as stack:
+        if not pol["master_dark_includes_bias"] and master_bias is not None:
+            master = stack_minus_master(stack, pol, master_bias)
+        else:
+            master = _master_integrate(stack, pol)
+    return master
+
+
+def stack_minus_master(stack: ResidentCalibratedStack, pol: dict, subtract) -> np.ndarray:
+    master = _master_integrate(stack, pol)
+    # the subtraction of the stacked dark and the bias master runs on device
+    with ResidentCalibratedStack(1, stack.height, stack.width) as tmp:
+        tmp.set_calibration_masters(bias=subtract)
+        tmp.calibrate_frame(0, master, 0.0, None, {"master_dark_includes_bias": False})
+        return tmp.download_frame(0)
+
+
+def build_master_flat(frames, master_bias=None, policy: dict | None = None):
+    """pipeline.py:828-955: per flat (flat - bias) / exact median (FP64
+    partition median, :721-775), stacked with the master rejection policy,
+    then floored.  Returns (master_flat, per-flat scalars)."""
+    pol = _policy(policy)
+    frames = list(frames)
+    h, w = np.asarray(frames[0]).shape
+    floor = float(pol["flat_floor"])
+    with ResidentCalibratedStack(len(frames), h, w) as stack:
+        stack.set_calibration_masters(bias=master_bias)
+        for i, frame in enumerate(frames):
+            stack.calibrate_frame(i, frame, 0.0, None, {"master_dark_includes_bias": True})
+        scalars = []
+        for i in range(len(frames)):
+            median = stack.frame_global_stats(i, exact_median=True)["median"]
+            if abs(median) < floor:
+                raise ValueError("flat normalization is below flat_floor")
+            scalars.append(float(median))
+        stack.normalize_frames(list(range(len(frames))), np.zeros(len(frames)), np.asarray(scalars, np.float32))
+        master = _master_integrate(stack, pol)
+    return np.maximum(master, np.float32(floor)).astype(np.float32), scalars
+
+
+def variance_aware_weights(rms: Sequence[float], active: Sequence[bool] | None = None) -> np.ndarray:
+    """engine/integration.py:80-106 ``variance_aware`` from per-frame rms
+    (``background_rms``, quality.py:294): 1/max(rms^2, 1e-12), floored at 1e-6,
+    divided by the f32 median of the positive raw weights; inactive frames get
+    weight 0 (resident_cuda.py:12759-12790)."""
+    raw = [max(1.0 / max(float(r) ** 2, 1.0e-12), 1.0e-6) for r in rms]
+    positive = np.asarray([v for v in raw if np.isfinite(v) and v > 0.0], dtype=np.float32)
+    scale = max(float(np.median(positive)) if positive.size else 1.0, 1.0e-6)
+    weights = np.asarray([max(v / scale, 1.0e-6) for v in raw], dtype=np.float32)
+    if active is not None:
+        weights = np.where(np.asarray(active, dtype=bool), weights, np.float32(0.0)).astype(np.float32)
+    return weights
+
+
+def _is_identity(matrix) -> bool:
+    return bool(np.allclose(np.asarray(matrix, dtype=np.float64), np.eye(3), atol=0.0, rtol=0.0))
+
+
+@dataclass
+class StackResult:
+    master: np.ndarray
+    weight: np.ndarray
+    coverage: np.ndarray
+    low: np.ndarray
+    high: np.ndarray
+    dq: np.ndarray
+    geometric: np.ndarray
+    weights: np.ndarray
+    timing: dict = field(default_factory=dict)
+
+
+def run_resident_stack(
+    lights,
+    light_exposures,
+    masters: dict,
+    matrices,
+    *,
+    dark_exposure_s=None,
+    interpolation: str = "lanczos3",
+    weighting: str = "none",
+    active=None,
+    low_sigma: float = 3.0,
+    high_sigma: float = 3.0,
+    min_samples: int = 3,
+    max_reject_fraction: float = 0.5,
+    policy: dict | None = None,
+    raw_format: str = "float32",
+    warp_semantics: str = "cpu",
+) -> StackResult:
+    """One resident stacking job from host frames (the drop-in driver)."""
+    pol = _policy(policy)
+    lights = list(lights)
+    f = len(lights)
+    first = np.asarray(lights[0])
+    if raw_format == "fits_u16be_bzero":
+        raise ValueError("pass decoded frames or use calibrate_frames_* for raw bytes")
+    h, w = first.shape
+    t0 = time.perf_counter()
+    stack = ResidentCalibratedStack(f, h, w)
+    try:
+        stack.set_calibration_masters(masters.get("bias"), masters.get("dark"), masters.get("flat"))
+        for i, light in enumerate(lights):
+            stack.calibrate_frame(i, light, light_exposures[i], dark_exposure_s, pol)
+        t_cal = time.perf_counter()
+        stats = [stack.frame_global_stats(i) for i in range(f)] if weighting != "none" else None
+        stack.reset_warp_coverage()
+        moving = [i for i in range(f) if not _is_identity(matrices[i])]
+        for i in range(f):
+            if i not in moving:
+                stack.accumulate_full_warp_coverage_frame()
+        if moving:
+            mats = np.stack([np.asarray(matrices[i], dtype=np.float64) for i in moving])
+            if warp_semantics == "cpu":
+                stack.apply_matrix_frames_cpu_semantics(moving, mats, interpolation, fill=np.nan)
+            elif interpolation == "lanczos3":
+                stack.apply_matrix_lanczos3_frames(moving, mats)
+            else:
+                stack.apply_matrix_bilinear_frames(moving, mats)
+        t_warp = time.perf_counter()
+        if weighting == "none":
+            weights = np.ones(f, dtype=np.float32)
+            if active is not None:
+                weights = np.where(np.asarray(active, bool), weights, np.float32(0.0)).astype(np.float32)
+        elif weighting == "variance_aware":
+            weights = variance_aware_weights([s["std"] for s in stats], active)
+        else:
+            raise ValueError(f"unsupported weighting: {weighting}")
+        n_active = int(np.count_nonzero(weights > 0))
+        master, weight, coverage, low, high, prof = stack.integrate_hardened_winsorized_sigma_timed(
+            weights, low_sigma, high_sigma, min_samples, max_reject_fraction)
+        geometric = stack.warp_coverage_map()
+        t_int = time.perf_counter()
+        dq = dq_map(stack, weights, low_sigma, high_sigma, min_samples, max_reject_fraction, n_active)
+        return StackResult(master, weight, coverage, low, high, dq, geometric, weights,
+                           {"calibrate_s": t_cal - t0, "warp_s": t_warp - t_cal, "integrate_s": t_int - t_warp})
+    finally:
+        stack.close()
+
+
+def dq_map(stack: ResidentCalibratedStack, weights, low_sigma, high_sigma, min_samples, max_reject_fraction,
+           active_frames: int) -> np.ndarray:
+    """Resident DQ map computed by the integration kernel itself (fused)."""
+    from . import _native as N
+    import ctypes
+
+    params = N.IntegrateParams(N.GH_REJECT_WINSORIZED_SIGMA, float(low_sigma), float(high_sigma), int(min_samples),
+                               float(max_reject_fraction), N.GH_COUNT_F32, -1, 0, 0)
+    out = stack._integrate(weights, params, set(), N.GH_COUNT_F32, dq=True, dq_geometric=True,
+                           dq_active_frames=active_frames)
+    return out["dq"]

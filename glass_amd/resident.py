@@ -1,0 +1,528 @@
+"""Drop-in ``ResidentCalibratedStack`` for the resident stacking path.
+
+Mirrors the hot-path surface of the reference operator API
+``glass_cuda.ResidentCalibratedStack`` (src/glass_cuda.py:2653) / the pybind11
+class of cpp/src/native_bindings.cpp:20624-21213: same method names, argument
+order, defaults, return shapes and exception types.  Every call goes through
+the C ABI of libglass_hip.so (glass_amd/_native.py); there is no CPU path.
+
+Inputs may be numpy arrays (copied in, like pybind ``c_style | forcecast``) or
+torch tensors already on the GPU (zero-copy: their device pointer is passed).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+from typing import Any, Iterable, Sequence
+
+import numpy as np
+
+from . import _native as N
+
+_DEFAULT_POLICY = {
+    "master_dark_includes_bias": True,
+    "dark_scaling_enabled": True,
+    "flat_floor": 1.0e-6,
+    "pedestal": 0.0,
+}
+
+
+def _is_torch_cuda(obj) -> bool:
+    return type(obj).__module__.startswith("torch") and getattr(obj, "is_cuda", False)
+
+
+def _buffer(obj, dtype, shape=None, name="array"):
+    """(pointer, keepalive) for a host array or a device tensor."""
+    if _is_torch_cuda(obj):
+        import torch
+
+        want = {np.float32: torch.float32, np.uint16: torch.int16, np.uint8: torch.uint8}.get(np.dtype(dtype).type)
+        if not obj.is_contiguous():
+            raise ValueError(f"{name} device tensor must be contiguous")
+        if want is not None and obj.dtype not in (want, torch.uint8) and not (
+            np.dtype(dtype) == np.uint16 and obj.dtype in (torch.int16, torch.uint16 if hasattr(torch, "uint16") else torch.int16)
+        ):
+            raise ValueError(f"{name} device tensor has dtype {obj.dtype}, expected {np.dtype(dtype)}")
+        if shape is not None and obj.numel() * obj.element_size() != int(np.prod(shape)) * np.dtype(dtype).itemsize:
+            raise ValueError(f"{name} must have shape {tuple(shape)}")
+        return ctypes.c_void_p(obj.data_ptr()), obj
+    if type(obj).__module__.startswith("torch"):
+        obj = obj.detach().cpu().numpy()
+    arr = np.ascontiguousarray(obj, dtype=dtype)
+    if shape is not None and arr.size != int(np.prod(shape)):
+        raise ValueError(f"{name} must have shape {tuple(shape)}, got {arr.shape}")
+    if shape is not None and arr.ndim == 2 and tuple(arr.shape) != tuple(shape):
+        raise ValueError(f"{name} must have shape {tuple(shape)}, got {arr.shape}")
+    return ctypes.c_void_p(arr.ctypes.data), arr
+
+
+def _index_array(indices) -> np.ndarray:
+    arr = np.ascontiguousarray(np.asarray(list(indices) if not isinstance(indices, np.ndarray) else indices), dtype=np.int64)
+    if arr.ndim != 1:
+        raise ValueError("indices must be a 1-D sequence")
+    return arr
+
+
+def _i64p(arr: np.ndarray):
+    return arr.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+
+
+def dark_scale(light_exposure_s, dark_exposure_s, policy: dict | None, has_dark: bool) -> float:
+    """Dark scaling factor of the CPU path (src/glass/cpu/calibration.py:63-66):
+    FP64 ratio rounded to f32, 1.0 without scaling, without a dark exposure or
+    with a zero dark exposure."""
+    pol = dict(_DEFAULT_POLICY)
+    pol.update(policy or {})
+    if not has_dark:
+        return 1.0
+    if pol["dark_scaling_enabled"] and dark_exposure_s not in (None, 0):
+        return float(np.float32(float(light_exposure_s) / float(dark_exposure_s)))
+    return 1.0
+
+
+def invert_affine(matrix) -> np.ndarray:
+    """FP64 inverse of a 3x3 moving->reference matrix, as the CPU warp does
+    (src/glass/engine/warp.py:169-176).  Singular matrices raise ValueError."""
+    m = np.asarray(matrix, dtype=np.float64).reshape(3, 3)
+    if not np.all(np.isfinite(m)):
+        raise ValueError("warp matrix must be finite")
+    if abs(float(np.linalg.det(m))) <= 1.0e-20:
+        raise ValueError("warp matrix is singular")
+    try:
+        return np.linalg.inv(m)
+    except np.linalg.LinAlgError as exc:
+        raise ValueError("warp matrix is singular") from exc
+
+
+class ResidentCalibratedStack:
+    """F x H x W float32 calibrated light stack resident in HBM."""
+
+    def __init__(self, frame_count: int, height: int, width: int):
+        handle = ctypes.c_void_p()
+        N.call("gh_stack_create", int(frame_count), int(height), int(width), ctypes.byref(handle))
+        self._h = handle
+        self._frame_count = int(frame_count)
+        self._height = int(height)
+        self._width = int(width)
+        self._has = {"bias": False, "dark": False, "flat": False}
+
+    # ------------------------------------------------------------ lifecycle
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            N.call("gh_stack_destroy", self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _info(self) -> np.ndarray:
+        out = (ctypes.c_int64 * 8)()
+        N.call("gh_stack_info", self._h, out)
+        return np.frombuffer(out, dtype=np.int64).copy()
+
+    # ----------------------------------------------------------- properties
+    @property
+    def frame_count(self) -> int:
+        return self._frame_count
+
+    @property
+    def height(self) -> int:
+        return self._height
+
+    @property
+    def width(self) -> int:
+        return self._width
+
+    @property
+    def pixels_per_frame(self) -> int:
+        return self._height * self._width
+
+    @property
+    def loaded_count(self) -> int:
+        return int(self._info()[4])
+
+    @property
+    def bytes_allocated(self) -> int:
+        return int(self._info()[5])
+
+    @property
+    def warp_coverage_frame_count(self) -> int:
+        return int(self._info()[6])
+
+    @property
+    def warp_scratch_bytes(self) -> int:
+        return int(self._info()[7])
+
+    @property
+    def warp_copy_mode(self) -> str:
+        return "slot_swap"
+
+    def frame_device_ptr(self, index: int) -> int:
+        """Device address of frame `index` (valid until that frame is warped)."""
+        ptr = ctypes.c_void_p()
+        N.call("gh_stack_frame_ptr", self._h, int(index), ctypes.byref(ptr))
+        return int(ptr.value or 0)
+
+    def frame_has_nonfinite(self, index: int) -> bool:
+        out = ctypes.c_int64()
+        N.call("gh_stack_frame_nonfinite", self._h, int(index), ctypes.byref(out))
+        return bool(out.value)
+
+    # -------------------------------------------------------------- masters
+    def set_calibration_masters(self, bias=None, dark=None, flat=None) -> None:
+        shape = (self._height, self._width)
+        keep = []
+        ptrs = []
+        for name, value in (("bias", bias), ("dark", dark), ("flat", flat)):
+            if value is None:
+                ptrs.append(None)
+                self._has[name] = False
+                continue
+            ptr, ref = _buffer(value, np.float32, shape, name)
+            keep.append(ref)
+            ptrs.append(ptr)
+            self._has[name] = True
+        N.call("gh_set_masters", self._h, *ptrs)
+
+    # ---------------------------------------------------------- upload/down
+    def upload_calibrated_frame(self, index: int, frame) -> None:
+        ptr, ref = _buffer(frame, np.float32, (self._height, self._width), "frame")
+        N.call("gh_upload_frame", self._h, int(index), ptr)
+
+    def download_frame_tile(self, index: int, x0: int, y0: int, x1: int, y1: int) -> np.ndarray:
+        out = self.download_frames_tile([index], x0, y0, x1, y1)
+        return out[0]
+
+    def download_frames_tile(self, indices, x0: int, y0: int, x1: int, y1: int) -> np.ndarray:
+        idx = _index_array(indices)
+        if idx.size == 0:
+            raise ValueError("indices must contain at least one frame index")
+        if x0 < 0 or y0 < 0 or x0 >= x1 or y0 >= y1 or x1 > self._width or y1 > self._height:
+            raise IndexError("resident frame tile bounds are invalid")
+        out = np.empty((idx.size, int(y1) - int(y0), int(x1) - int(x0)), dtype=np.float32)
+        N.call("gh_download_tile", self._h, _i64p(idx), int(idx.size), int(x0), int(y0), int(x1), int(y1),
+               ctypes.c_void_p(out.ctypes.data))
+        return out
+
+    def download_frame(self, index: int) -> np.ndarray:
+        return self.download_frame_tile(index, 0, 0, self._width, self._height)
+
+    # ------------------------------------------------------------ calibrate
+    def _policy(self, policy: dict | None, accumulate_stats: bool = False) -> N.CalibPolicy:
+        pol = dict(_DEFAULT_POLICY)
+        pol.update(policy or {})
+        return N.CalibPolicy(
+            int(bool(pol["master_dark_includes_bias"])),
+            float(pol["flat_floor"]),
+            float(pol["pedestal"] or 0.0),
+            int(bool(accumulate_stats)),
+        )
+
+    def _calibrate(self, indices, raw_ptr, stride, fmt, light_exposures, dark_exposures, policy, accumulate_stats,
+                   stream=None):
+        idx = _index_array(indices)
+        lights = list(light_exposures)
+        if isinstance(dark_exposures, (list, tuple, np.ndarray)):
+            darks = list(dark_exposures)
+        else:
+            darks = [dark_exposures] * idx.size
+        if len(lights) != idx.size or len(darks) != idx.size:
+            raise ValueError("light_exposures and dark_exposures must match indices")
+        scales = np.asarray(
+            [dark_scale(lights[k], darks[k], policy, self._has["dark"]) for k in range(idx.size)], dtype=np.float32
+        )
+        pol = self._policy(policy, accumulate_stats)
+        N.call(
+            "gh_calibrate_frames", self._h, _i64p(idx), int(idx.size), raw_ptr, int(stride), int(fmt),
+            scales.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(pol),
+            ctypes.c_void_p(stream) if stream else None,
+        )
+
+    def calibrate_frame(self, index, light, light_exposure_s, dark_exposure_s=None, policy=None) -> None:
+        """calibrate_frame (native_bindings.cpp:5979) -- f32 light frame."""
+        ptr, ref = _buffer(light, np.float32, (self._height, self._width), "light")
+        self._calibrate([index], ptr, self.pixels_per_frame * 4, N.GH_FMT_F32, [light_exposure_s],
+                        [dark_exposure_s], policy, False)
+
+    def calibrate_frame_timed(self, index, light, light_exposure_s, dark_exposure_s=None, policy=None) -> dict:
+        start = time.perf_counter()
+        self.calibrate_frame(index, light, light_exposure_s, dark_exposure_s, policy)
+        total = time.perf_counter() - start
+        return {"schema_version": 1, "h2d_mode": "pageable", "total_s": total}
+
+    def calibrate_frames_host_async_timed(self, indices, lights, light_exposures, dark_exposures, policy=None) -> dict:
+        start = time.perf_counter()
+        for k, index in enumerate(_index_array(indices)):
+            darks = dark_exposures[k] if isinstance(dark_exposures, (list, tuple, np.ndarray)) else dark_exposures
+            self.calibrate_frame(int(index), lights[k], light_exposures[k], darks, policy)
+        return {"schema_version": 1, "frame_count": len(lights), "total_s": time.perf_counter() - start}
+
+    def calibrate_frames_fits_u16be_bzero_host_async_multistream_callback_release_timed(
+        self, indices, raw_lights, light_exposures, dark_exposures, stream_count, wave_frames, release_callback,
+        policy=None,
+    ) -> dict:
+        """Raw FITS BITPIX=16/BZERO=32768 big-endian light frames
+        (native_bindings.cpp:6839; kernel calibration_kernels.cu:110-153)."""
+        idx = _index_array(indices)
+        start = time.perf_counter()
+        for k, index in enumerate(idx):
+            ptr, ref = _buffer(raw_lights[k], np.uint8, (self.pixels_per_frame * 2,), "raw_light")
+            darks = dark_exposures[k] if isinstance(dark_exposures, (list, tuple, np.ndarray)) else dark_exposures
+            self._calibrate([int(index)], ptr, self.pixels_per_frame * 2, N.GH_FMT_FITS_U16BE_BZERO,
+                            [light_exposures[k]], [darks], policy, False)
+            if release_callback is not None:
+                release_callback(int(k))
+        return {"schema_version": 1, "frame_count": int(idx.size), "stream_count": int(stream_count),
+                "wave_frames": int(wave_frames), "total_s": time.perf_counter() - start}
+
+    def calibrate_frames_device(self, indices, raw, raw_format: str, light_exposures, dark_exposures, policy=None,
+                                frame_stride_bytes: int | None = None, accumulate_stats: bool = False,
+                                stream: int | None = None) -> None:
+        """Resident-raw extension: calibrate frames whose raw samples already
+        sit in HBM (a torch tensor), without a host round trip."""
+        fmt = {"float32": N.GH_FMT_F32, "uint16": N.GH_FMT_U16, "fits_u16be_bzero": N.GH_FMT_FITS_U16BE_BZERO}[raw_format]
+        sample = 4 if fmt == N.GH_FMT_F32 else 2
+        stride = int(frame_stride_bytes or self.pixels_per_frame * sample)
+        if not _is_torch_cuda(raw):
+            raise ValueError("calibrate_frames_device expects a GPU tensor")
+        self._calibrate(indices, ctypes.c_void_p(raw.data_ptr()), stride, fmt, light_exposures, dark_exposures,
+                        policy, accumulate_stats, stream)
+
+    def calibration_stats(self, index: int) -> dict:
+        s, ss, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        N.call("gh_calibration_stats", self._h, int(index), ctypes.byref(s), ctypes.byref(ss), ctypes.byref(n))
+        count = int(n.value)
+        mean = s.value / count if count else 0.0
+        var = max(ss.value / count - mean * mean, 0.0) if count else 0.0
+        return {"sum": s.value, "sum_sq": ss.value, "valid_pixels": count, "mean": mean, "std": math.sqrt(var)}
+
+    def normalize_frames(self, indices, offsets, divisors, stream: int | None = None) -> None:
+        """x = (x - offset) / divisor per frame, f32 (per-flat normalisation)."""
+        idx = _index_array(indices)
+        off = np.ascontiguousarray(np.asarray(offsets, dtype=np.float32).reshape(-1))
+        div = np.ascontiguousarray(np.asarray(divisors, dtype=np.float32).reshape(-1))
+        if off.size != idx.size or div.size != idx.size:
+            raise ValueError("offsets and divisors must match indices")
+        N.call("gh_normalize_frames", self._h, _i64p(idx), int(idx.size),
+               off.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), div.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+               ctypes.c_void_p(stream) if stream else None)
+
+    # ----------------------------------------------------------------- warp
+    def _warp(self, indices, matrices, interpolation, fill, clamping_threshold, track_coverage, skip_nonfinite,
+              stream=None) -> dict:
+        idx = _index_array(indices)
+        mats = np.asarray(matrices, dtype=np.float64).reshape(-1, 3, 3)
+        if mats.shape[0] != idx.size:
+            raise ValueError("matrices must have shape (len(indices), 3, 3)")
+        inv = np.ascontiguousarray(np.stack([invert_affine(m) for m in mats]) if idx.size else np.zeros((0, 3, 3)))
+        affine = bool(idx.size) and bool(np.all(inv[:, 2, 0] == 0.0) and np.all(inv[:, 2, 1] == 0.0)
+                                         and np.all(inv[:, 2, 2] == 1.0))
+        flags = (N.GH_WARP_TRACK_COVERAGE if track_coverage else 0) | (N.GH_WARP_AFFINE if affine else 0)
+        if skip_nonfinite:
+            flags |= N.GH_WARP_SKIP_NONFINITE_TAPS
+        start = time.perf_counter()
+        N.call("gh_warp_frames", self._h, _i64p(idx), int(idx.size),
+               inv.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), int(interpolation), float(fill),
+               float(clamping_threshold), int(flags), ctypes.c_void_p(stream) if stream else None)
+        if stream is None:
+            N.call("gh_synchronize", self._h)
+        return {
+            "schema_version": 1,
+            "frame_count": int(idx.size),
+            "interpolation": "lanczos3" if interpolation == N.GH_INTERP_LANCZOS3 else "bilinear",
+            "track_coverage": bool(track_coverage),
+            "copy_mode": "slot_swap",
+            "total_s": time.perf_counter() - start,
+        }
+
+    def apply_matrix_bilinear_frame(self, index, matrix, fill=math.nan) -> None:
+        self._warp([index], [matrix], N.GH_INTERP_BILINEAR, fill, -1.0, False, True)
+
+    def apply_matrix_lanczos3_frame(self, index, matrix, fill=math.nan, clamping_threshold=-1.0) -> None:
+        self._warp([index], [matrix], N.GH_INTERP_LANCZOS3, fill, clamping_threshold, False, True)
+
+    def apply_matrix_bilinear_frames(self, indices, matrices, fill=math.nan, max_chunk_capacity_frames=0,
+                                     track_coverage=True) -> dict:
+        return self._warp(indices, matrices, N.GH_INTERP_BILINEAR, fill, -1.0, track_coverage, True)
+
+    def apply_matrix_lanczos3_frames(self, indices, matrices, fill=math.nan, clamping_threshold=-1.0,
+                                     max_chunk_capacity_frames=0, track_coverage=True) -> dict:
+        return self._warp(indices, matrices, N.GH_INTERP_LANCZOS3, fill, clamping_threshold, track_coverage, True)
+
+    def apply_matrix_frames_cpu_semantics(self, indices, matrices, interpolation="lanczos3", fill=0.0,
+                                          track_coverage=True, stream=None) -> dict:
+        """Matrix warp with the CPU-path tap semantics (engine/warp.py:121-152:
+        non-finite taps are not skipped, sinc threshold 1e-8)."""
+        interp = N.GH_INTERP_LANCZOS3 if interpolation == "lanczos3" else N.GH_INTERP_BILINEAR
+        return self._warp(indices, matrices, interp, fill, -1.0, track_coverage, False, stream)
+
+    def reset_warp_coverage(self) -> None:
+        N.call("gh_reset_warp_coverage", self._h)
+
+    def accumulate_full_warp_coverage_frame(self) -> None:
+        N.call("gh_accumulate_full_warp_coverage", self._h)
+
+    def warp_coverage_map(self) -> np.ndarray:
+        out = np.empty((self._height, self._width), dtype=np.float32)
+        N.call("gh_warp_coverage_map", self._h, ctypes.c_void_p(out.ctypes.data))
+        return out
+
+    # ---------------------------------------------------------------- stats
+    def frame_global_stats(self, index: int, exact_median: bool = False) -> dict:
+        st = N.FrameStatistics()
+        N.call("gh_frame_stats", self._h, int(index), int(bool(exact_median)), ctypes.byref(st))
+        out = {
+            "mean": st.mean,
+            "std": st.std,
+            "valid_pixels": int(st.valid_pixels),
+            "total_pixels": int(st.total_pixels),
+            "nonfinite_pixels": int(st.nonfinite_pixels),
+            "model": "resident_global_mean_std",
+            "min": float(st.min),
+            "max": float(st.max),
+        }
+        if exact_median:
+            out["median"] = st.median
+            out["mad"] = st.mad
+        return out
+
+    # ------------------------------------------------------------ integrate
+    def _integrate(self, weights, params: N.IntegrateParams, want, count_dtype, dq=False, dq_geometric=False,
+                   dq_active_frames=0, device_outputs: dict | None = None, stream=None):
+        if weights is None:
+            w = None
+            wptr = None
+        else:
+            w = np.ascontiguousarray(np.asarray(weights, dtype=np.float32))
+            if w.shape != (self._frame_count,):
+                raise ValueError("weights must have shape (frame_count,)")
+            wptr = w.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        shape = (self._height, self._width)
+        host: dict[str, Any] = {}
+        outs = N.IntegrateOutputs()
+        cdt = np.uint16 if count_dtype == N.GH_COUNT_U16 else np.float32
+        for name, dtype in (("master", np.float32), ("weight", np.float32), ("coverage", cdt), ("low", cdt),
+                            ("high", cdt), ("dq", np.uint32)):
+            if name == "dq" and not dq:
+                continue
+            if device_outputs and name in device_outputs:
+                setattr(outs, name, device_outputs[name].data_ptr())
+                continue
+            if name in want or name == "dq":
+                arr = np.empty(shape, dtype=dtype)
+                host[name] = arr
+                setattr(outs, name, arr.ctypes.data)
+        outs.dq_use_geometric = int(bool(dq_geometric))
+        outs.dq_active_frames = int(dq_active_frames)
+        N.call("gh_integrate", self._h, wptr, ctypes.byref(params), ctypes.byref(outs),
+               ctypes.c_void_p(stream) if stream else None)
+        return host
+
+    def integrate_mean(self, weights=None):
+        """integrate_mean(weights=None) -> (master, weight_map)
+        (native_bindings.cpp:12935)."""
+        params = N.IntegrateParams(N.GH_REJECT_NONE, 3.0, 3.0, 3, 0.5, N.GH_COUNT_F32, -1, 0, 0)
+        out = self._integrate(weights, params, {"master", "weight"}, N.GH_COUNT_F32)
+        return out["master"], out["weight"]
+
+    def integrate_hardened_winsorized_sigma(self, weights=None, low_sigma=3.0, high_sigma=3.0, min_samples=3,
+                                            max_reject_fraction=0.5, count_map_dtype="float32", profile=False,
+                                            download_mode="full"):
+        """integrate_hardened_winsorized_sigma (native_bindings.cpp:13395,
+        binding :21174-21184) -> (master, weight|None, coverage|None, low|None,
+        high|None[, profile])."""
+        result = self._winsorized(weights, low_sigma, high_sigma, min_samples, max_reject_fraction, count_map_dtype,
+                                  download_mode)
+        if profile:
+            return (*result[0], result[1])
+        return result[0]
+
+    def integrate_hardened_winsorized_sigma_timed(self, weights=None, low_sigma=3.0, high_sigma=3.0, min_samples=3,
+                                                  max_reject_fraction=0.5, count_map_dtype="float32",
+                                                  download_mode="full"):
+        maps, prof = self._winsorized(weights, low_sigma, high_sigma, min_samples, max_reject_fraction,
+                                      count_map_dtype, download_mode)
+        return (*maps, prof)
+
+    def _winsorized(self, weights, low_sigma, high_sigma, min_samples, max_reject_fraction, count_map_dtype,
+                    download_mode, statistics_variant=-1, tile_size=0, require_positive_scale=0):
+        if count_map_dtype not in ("float32", "uint16"):
+            raise ValueError("count_map_dtype must be 'float32' or 'uint16'")
+        modes = {
+            "full": {"master", "weight", "coverage", "low", "high"},
+            "master_weight": {"master", "weight"},
+            "master_only": {"master"},
+        }
+        if download_mode not in modes:
+            raise ValueError("download_mode must be 'full', 'master_weight' or 'master_only'")
+        if not (float(low_sigma) > 0 and float(high_sigma) > 0):
+            raise ValueError("low_sigma and high_sigma must be positive")
+        if int(min_samples) < 1:
+            raise ValueError("min_samples must be at least 1")
+        if not 0.0 <= float(max_reject_fraction) <= 1.0:
+            raise ValueError("max_reject_fraction must be between 0 and 1")
+        cdt = N.GH_COUNT_U16 if count_map_dtype == "uint16" else N.GH_COUNT_F32
+        params = N.IntegrateParams(N.GH_REJECT_WINSORIZED_SIGMA, float(low_sigma), float(high_sigma),
+                                   int(min_samples), float(max_reject_fraction), cdt, int(statistics_variant),
+                                   int(require_positive_scale), int(tile_size))
+        want = modes[download_mode]
+        start = time.perf_counter()
+        out = self._integrate(weights, params, want, cdt)
+        total = time.perf_counter() - start
+        maps = tuple(out.get(name) for name in ("master", "weight", "coverage", "low", "high"))
+        active = self._frame_count if weights is None else int(
+            np.count_nonzero(np.isfinite(np.asarray(weights, dtype=np.float32)) & (np.asarray(weights, dtype=np.float32) > 0))
+        )
+        prof = {
+            "native_method": "ResidentCalibratedStack.integrate_hardened_winsorized_sigma",
+            "resident_winsorized_mode": "hardened_cpu_parity",
+            "frame_count": self._frame_count,
+            "active_frame_count": active,
+            "pixel_count": self.pixels_per_frame,
+            "count_map_dtype_requested": count_map_dtype,
+            "count_map_dtype": count_map_dtype,
+            "download_mode": download_mode,
+            "native_profile": {
+                "schema_version": 1,
+                "percentile_strategy": "register_sorting_network_order_statistics",
+                "winsorized_accumulation_order": "fp64",
+                "kernel": "integrate_winsorized_kernel",
+            },
+            "total_s": total,
+        }
+        return maps, prof
+
+    def integrate_stack_engine(self, low_sigma=3.0, high_sigma=3.0, min_samples=3, max_reject_fraction=0.5,
+                               tile_size=512):
+        """CPUStackEngine semantics (master calibration): per-tile statistics
+        arithmetic and the positive-scale rejection rule
+        (src/glass/engine/stack_engine.py:39-120, :578-593)."""
+        maps, _ = self._winsorized(None, low_sigma, high_sigma, min_samples, max_reject_fraction, "float32", "full",
+                                   statistics_variant=-1, tile_size=int(tile_size), require_positive_scale=1)
+        return maps
+
+    def integrate_to_device(self, outputs: dict, weights=None, rejection="winsorized_sigma", low_sigma=3.0,
+                            high_sigma=3.0, min_samples=3, max_reject_fraction=0.5, count_map_dtype="uint16",
+                            dq_geometric=True, dq_active_frames=0, stream=None) -> None:
+        """Integrate into caller-owned GPU tensors (keys master, weight,
+        coverage, low, high, dq); nothing is copied to the host."""
+        cdt = N.GH_COUNT_U16 if count_map_dtype == "uint16" else N.GH_COUNT_F32
+        rej = N.GH_REJECT_WINSORIZED_SIGMA if rejection == "winsorized_sigma" else N.GH_REJECT_NONE
+        params = N.IntegrateParams(rej, float(low_sigma), float(high_sigma), int(min_samples),
+                                   float(max_reject_fraction), cdt, -1, 0, 0)
+        self._integrate(weights, params, set(), cdt, dq="dq" in outputs, dq_geometric=dq_geometric,
+                        dq_active_frames=dq_active_frames, device_outputs=outputs, stream=stream)
+
+    def synchronize(self) -> None:
+        N.call("gh_synchronize", self._h)

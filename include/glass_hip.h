@@ -1,0 +1,201 @@
+/*
+ * glass_hip.h -- C ABI of libglass_hip.so, the MI355X (gfx950) resident
+ * stacking path: master calibration -> affine warp (bilinear / Lanczos3) ->
+ * winsorized sigma-clip rejection with weighted-mean integration, plus the
+ * output maps (weight, coverage, low/high rejection counts, DQ).
+ *
+ * The entry points replace the hot-path methods of the reference's pybind11
+ * class ResidentCalibratedStack (cpp/src/native_bindings.cpp:2915, bound at
+ * :20624-21213), which the reference reaches through glass_cuda.py:2653.
+ * Each declaration below cites the reference method it replaces.
+ *
+ * Conventions
+ *  - Every function returns an int status: GH_OK (0) or one of the error codes
+ *    below; gh_last_error() returns the message of the last failure on the
+ *    calling thread.  GH_ERR_VALUE maps to Python ValueError
+ *    (std::invalid_argument in the reference), GH_ERR_INDEX to IndexError
+ *    (std::out_of_range), GH_ERR_RUNTIME to RuntimeError (check_cuda,
+ *    native_bindings.cpp:1011).
+ *  - Frames live in HBM as float32 row-major [height][width] slots owned by
+ *    the stack object.  Pointers marked "host or device" are classified with
+ *    hipPointerGetAttributes; everything else is host memory.
+ *  - `stream` is a hipStream_t (NULL = the stack's own stream).  Calls that
+ *    return host data synchronise before returning.
+ *  - Plain C types only: no torch or numpy types cross this boundary.
+ */
+#ifndef GLASS_HIP_H
+#define GLASS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GH_OK 0
+#define GH_ERR_VALUE 1
+#define GH_ERR_INDEX 2
+#define GH_ERR_RUNTIME 3
+
+typedef struct gh_stack gh_stack;
+
+/* Raw sample formats accepted by the calibration entry points. */
+#define GH_FMT_F32 0            /* float32 */
+#define GH_FMT_U16 1            /* little-endian uint16 ADU */
+#define GH_FMT_FITS_U16BE_BZERO 2 /* FITS BITPIX=16, BZERO=32768, big-endian */
+
+/* Interpolators of the matrix warp. */
+#define GH_INTERP_BILINEAR 1
+#define GH_INTERP_LANCZOS3 3
+
+/* Warp flags. */
+#define GH_WARP_TRACK_COVERAGE 1u     /* add per-pixel footprint to the geometric coverage map */
+#define GH_WARP_SKIP_NONFINITE_TAPS 2u /* resident semantics: non-finite Lanczos taps skipped */
+#define GH_WARP_AFFINE 4u             /* last inverse row is exactly (0,0,1): no projective divide */
+
+/* Rejection modes of gh_integrate. */
+#define GH_REJECT_NONE 0
+#define GH_REJECT_WINSORIZED_SIGMA 1
+
+/* Count-map dtypes of gh_integrate. */
+#define GH_COUNT_F32 0
+#define GH_COUNT_U16 1
+
+typedef struct gh_calib_policy {
+  int32_t master_dark_includes_bias; /* default 1 (src/glass/models.py:68) */
+  float flat_floor;                  /* default 1e-6 */
+  float pedestal;                    /* default 0 */
+  int32_t accumulate_stats;          /* 1: fused FP64 sum / sum^2 / finite count per frame */
+} gh_calib_policy;
+
+typedef struct gh_integrate_params {
+  int32_t rejection;          /* GH_REJECT_* */
+  float low_sigma;            /* default 3 */
+  float high_sigma;           /* default 3 */
+  int32_t min_samples;        /* default 3, >= 1 */
+  float max_reject_fraction;  /* default 0.5, in [0,1] */
+  int32_t count_dtype;        /* GH_COUNT_F32 or GH_COUNT_U16 */
+  int32_t statistics_variant; /* -1 auto, 0 per-pixel valid-count arithmetic
+                                 (rejection.py:242-296), 1 all-valid arithmetic
+                                 (rejection.py:219-239) */
+  int32_t require_positive_scale; /* 1: StackEngine rule, no rejection when the
+                                     scale is not finite and > 0 (stack_engine.py:586) */
+  int32_t tile_size;          /* 0: the statistics variant is chosen once for the
+                                 whole frame (weighted_integrate_stack); >0: per
+                                 tile of this size (CPUStackEngine tiles) */
+} gh_integrate_params;
+
+/* Output pointers of gh_integrate.  Each may be NULL (not produced).  Device
+ * pointers are written on the stream; host pointers are filled before return. */
+typedef struct gh_integrate_outputs {
+  float* master;           /* [H*W] f32 */
+  float* weight;           /* [H*W] f32 */
+  void* coverage;          /* [H*W] f32 or u16 (post-rejection sample count) */
+  void* low;               /* [H*W] f32 or u16 */
+  void* high;              /* [H*W] f32 or u16 */
+  uint32_t* dq;            /* [H*W] u32 DQ flags (resident_cuda.py:3110-3297) */
+  int32_t dq_use_geometric; /* 1: include the geometric warp coverage map in DQ */
+  int32_t dq_active_frames; /* active frame count for partial-coverage WARP_EDGE */
+} gh_integrate_outputs;
+
+typedef struct gh_frame_statistics {
+  double mean;          /* FP64, finite pixels */
+  double std;           /* sqrt(max(E[x^2] - mean^2, 0)) */
+  double sum;
+  double sum_sq;
+  int64_t valid_pixels;
+  int64_t total_pixels;
+  int64_t nonfinite_pixels;
+  double median;        /* exact (radix select), even counts average the middle pair */
+  double mad;           /* exact median of |x - f32(median)| */
+  float min;
+  float max;
+} gh_frame_statistics;
+
+/* ---- library ------------------------------------------------------------ */
+int gh_last_error(char* buffer, size_t size);
+int gh_version(void);
+/* Number of visible HIP devices (cudaGetDeviceCount, native_bindings.cpp:15572). */
+int gh_device_count(int* count);
+
+/* ---- lifecycle: ResidentCalibratedStack(frame_count, height, width)
+ *      (native_bindings.cpp:2917-2934, destructor :2939-2996) ----------------- */
+int gh_stack_create(int64_t frame_count, int64_t height, int64_t width, gh_stack** out);
+int gh_stack_destroy(gh_stack* stack);
+/* frame_count / height / width / loaded_count / bytes_allocated /
+ * warp_coverage_frame_count properties (native_bindings.cpp:20626-20639). */
+int gh_stack_info(const gh_stack* stack, int64_t out[8]);
+/* Device address of frame `index` (for zero-copy tensor handoff); valid until
+ * the next warp of that frame (warps write into a fresh slot). */
+int gh_stack_frame_ptr(gh_stack* stack, int64_t index, float** device_ptr);
+/* 1 if frame `index` holds any non-finite sample (tracked by every writer). */
+int gh_stack_frame_nonfinite(gh_stack* stack, int64_t index, int64_t* count);
+
+/* set_calibration_masters(bias=None, dark=None, flat=None)
+ * (native_bindings.cpp:3152, :15398-15414).  NULL = absent.  host or device. */
+int gh_set_masters(gh_stack* stack, const float* bias, const float* dark, const float* flat);
+
+/* upload_calibrated_frame(i, f32[H,W]) (native_bindings.cpp:3158). host or device. */
+int gh_upload_frame(gh_stack* stack, int64_t index, const float* frame);
+
+/* download_frame_tile / download_frames_tile (native_bindings.cpp:3082-3150):
+ * frames indices[0..count) tile [y0,y1)x[x0,x1) into host `out`
+ * ([count][y1-y0][x1-x0] f32). */
+int gh_download_tile(gh_stack* stack, const int64_t* indices, int64_t count, int64_t x0, int64_t y0,
+                     int64_t x1, int64_t y1, float* out);
+
+/* calibrate_frame / calibrate_frames_* / ..._fits_u16be_bzero_...
+ * (native_bindings.cpp:5979-8778; kernels cpp/cuda/calibration_kernels.cu:5-186):
+ * frame k of the batch is read from `raw` + k*raw_frame_stride_bytes (host or
+ * device, format GH_FMT_*) and written, calibrated, into slot indices[k].
+ * dark_scales[k] multiplies the dark (1.0 when unscaled). */
+int gh_calibrate_frames(gh_stack* stack, const int64_t* indices, int64_t count, const void* raw,
+                        int64_t raw_frame_stride_bytes, int32_t format, const float* dark_scales,
+                        const gh_calib_policy* policy, void* stream);
+
+/* Per-frame affine normalisation x = (x - offset[k]) / divisor[k] in f32 for
+ * frames indices[k]: the per-flat median normalisation of the CPU master-flat
+ * build (src/glass/engine/pipeline.py:880-884, `data / np.float32(scalar)`;
+ * resident form rc.py:6736-6755). */
+int gh_normalize_frames(gh_stack* stack, const int64_t* indices, int64_t count, const float* offsets,
+                        const float* divisors, void* stream);
+
+/* apply_matrix_{bilinear,lanczos3}_frame[s] (native_bindings.cpp:8835-9922;
+ * kernels cpp/cuda/warp_kernels.cu:206-914): frame indices[k] is resampled onto
+ * the reference grid through inverse[k] (row-major 3x3 FP64 inverse of the
+ * moving->reference matrix).  Out-of-footprint pixels get `fill`.
+ * clamping_threshold < 0 disables the Lanczos3 clamp. */
+int gh_warp_frames(gh_stack* stack, const int64_t* indices, int64_t count, const double* inverse,
+                   int32_t interpolation, float fill, float clamping_threshold, uint32_t flags, void* stream);
+
+/* reset_warp_coverage / accumulate_full_warp_coverage_frame / warp_coverage_map
+ * (native_bindings.cpp:3051-3080).  `out` host or device, [H*W] f32. */
+int gh_reset_warp_coverage(gh_stack* stack);
+int gh_accumulate_full_warp_coverage(gh_stack* stack);
+int gh_warp_coverage_map(gh_stack* stack, float* out);
+
+/* integrate_mean(weights) and integrate_hardened_winsorized_sigma(weights, low,
+ * high, min_samples, max_reject_fraction, count_map_dtype, ...)
+ * (native_bindings.cpp:12935-12990, :13395-14335; kernel
+ * cpp/cuda/integration_kernels.cu:2292-2779).  weights: host f32[frame_count]
+ * or NULL (unit).  Zero / non-finite weights exclude the frame. */
+int gh_integrate(gh_stack* stack, const float* weights, const gh_integrate_params* params,
+                 const gh_integrate_outputs* outputs, void* stream);
+
+/* frame_global_stats(i) (native_bindings.cpp:10569-10643) plus exact median /
+ * MAD (src/glass/engine/quality.py:126-200).  FP64. */
+int gh_frame_stats(gh_stack* stack, int64_t index, int32_t exact_median, gh_frame_statistics* out);
+
+/* Fused calibration statistics of the last gh_calibrate_frames call with
+ * accumulate_stats=1: sum, sum_sq (FP64) and finite count per frame index. */
+int gh_calibration_stats(gh_stack* stack, int64_t index, double* sum, double* sum_sq, int64_t* finite);
+
+/* Wait for all work queued on the stack's stream. */
+int gh_synchronize(gh_stack* stack);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GLASS_HIP_H */

@@ -1,0 +1,186 @@
+"""Affine warp oracle (test infrastructure; see oracle/__init__.py).
+
+``warp_matrix`` restates the CPU path src/glass/engine/warp.py:155-228
+(``_warp_tile_matrix``) with ``_sample_bilinear`` (:99-118) and
+``_sample_windowed`` (:121-152, Lanczos3 weights :73-84), vectorised over
+output pixels but with the same per-pixel arithmetic and tap order:
+* source coordinates in FP64 from ``np.linalg.inv`` of the moving->reference
+  matrix: ``(i00*x + i01*y) + i02`` divided by the projective denominator;
+* bilinear: valid 0 <= s <= dim-1, f32 lerps, x1 = min(x0+1, W-1);
+* Lanczos3: valid 2 <= s < dim-3, taps floor(s)-2 .. floor(s)+3, FP64
+  weights sinc(x)*sinc(x/3) with sinc(|x|<1e-8)=1, taps with weight exactly 0
+  skipped, FP64 sums, result 0 when |sum w| <= 1e-12; non-finite taps are NOT
+  skipped on this path.
+
+``warp_matrix_resident`` restates the resident-backend semantics that the
+reference GPU tests pin with their own brute force
+(tests/test_gpu_warp_vs_cpu.py:35-128): non-finite taps skipped, sinc
+threshold 1e-6, optional clamp to [min - t*range, max + t*range] of the taps.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+def invert_matrix(matrix):
+    """FP64 inverse as engine/warp.py:169-176 (np.linalg.inv)."""
+    m = np.asarray(matrix, dtype=np.float64)
+    if m.shape != (3, 3):
+        raise ValueError(f"warp matrix must be 3x3, got {m.shape}")
+    try:
+        return np.linalg.inv(m)
+    except np.linalg.LinAlgError as exc:  # pragma: no cover - same message as reference
+        raise ValueError("warp matrix is singular") from exc
+
+
+def source_coordinates(inverse, height, width):
+    """engine/warp.py:177-195: FP64 source coordinates, NaN where the
+    projective denominator is <= 1e-12 in magnitude."""
+    ys, xs = np.mgrid[0:height, 0:width].astype(np.float64)
+    inv = np.asarray(inverse, dtype=np.float64)
+    denom = inv[2, 0] * xs + inv[2, 1] * ys + inv[2, 2]
+    ok = np.abs(denom) > 1.0e-12
+    sx = np.divide(inv[0, 0] * xs + inv[0, 1] * ys + inv[0, 2], denom, out=np.full_like(xs, np.nan), where=ok)
+    sy = np.divide(inv[1, 0] * xs + inv[1, 1] * ys + inv[1, 2], denom, out=np.full_like(ys, np.nan), where=ok)
+    return sx, sy
+
+
+def _sinc(x, threshold):
+    x = np.asarray(x, dtype=np.float64)
+    pix = np.pi * x
+    with np.errstate(divide="ignore", invalid="ignore"):
+        value = np.sin(pix) / pix
+    return np.where(np.abs(x) < threshold, 1.0, value)
+
+
+def lanczos3_weight(x, threshold=1.0e-8):
+    """engine/warp.py:73-84 (threshold 1e-8) / test oracle :67-77 (1e-6)."""
+    ax = np.abs(np.asarray(x, dtype=np.float64))
+    w = _sinc(ax, threshold) * _sinc(ax / 3.0, threshold)
+    return np.where(ax >= 3.0, 0.0, w)
+
+
+def _bilinear(image, sx, sy):
+    h, w = image.shape
+    x0 = np.floor(sx).astype(np.int64)
+    y0 = np.floor(sy).astype(np.int64)
+    x1 = np.minimum(x0 + 1, w - 1)
+    y1 = np.minimum(y0 + 1, h - 1)
+    tx = (sx - x0).astype(np.float32)
+    ty = (sy - y0).astype(np.float32)
+    one = np.float32(1.0)
+    top = image[y0, x0] * (one - tx) + image[y0, x1] * tx
+    bottom = image[y1, x0] * (one - tx) + image[y1, x1] * tx
+    return (top * (one - ty) + bottom * ty).astype(np.float32)
+
+
+def _lanczos_cpu(image, sx, sy):
+    """engine/warp.py:121-152 vectorised: yy outer, xx inner, FP64 sums."""
+    bx = np.floor(sx).astype(np.int64)
+    by = np.floor(sy).astype(np.int64)
+    acc = np.zeros(sx.shape, dtype=np.float64)
+    wsum = np.zeros(sx.shape, dtype=np.float64)
+    for dy in range(-2, 4):
+        yy = by + dy
+        wy = lanczos3_weight(sy - yy.astype(np.float64), 1.0e-8)
+        for dx in range(-2, 4):
+            xx = bx + dx
+            wx = lanczos3_weight(sx - xx.astype(np.float64), 1.0e-8)
+            take = (wy != 0.0) & (wx != 0.0)
+            weight = wx * wy
+            value = image[yy, xx].astype(np.float64)
+            acc = np.where(take, acc + value * weight, acc)
+            wsum = np.where(take, wsum + weight, wsum)
+    out = np.zeros(sx.shape, dtype=np.float32)
+    ok = np.abs(wsum) > 1.0e-12
+    with np.errstate(divide="ignore", invalid="ignore"):
+        out[ok] = (acc[ok] / wsum[ok]).astype(np.float32)
+    return out
+
+
+def _lanczos_resident(image, sx, sy, clamping_threshold):
+    """tests/test_gpu_warp_vs_cpu.py:80-128 vectorised."""
+    bx = np.floor(sx).astype(np.int64)
+    by = np.floor(sy).astype(np.int64)
+    acc = np.zeros(sx.shape, dtype=np.float64)
+    wsum = np.zeros(sx.shape, dtype=np.float64)
+    vmin = np.full(sx.shape, np.inf)
+    vmax = np.full(sx.shape, -np.inf)
+    for dy in range(-2, 4):
+        yy = by + dy
+        wy = lanczos3_weight(sy - yy.astype(np.float64), 1.0e-6)
+        for dx in range(-2, 4):
+            xx = bx + dx
+            value = image[yy, xx].astype(np.float64)
+            fin = np.isfinite(value)
+            weight = wy * lanczos3_weight(sx - xx.astype(np.float64), 1.0e-6)
+            acc = np.where(fin, acc + np.where(fin, value, 0.0) * weight, acc)
+            wsum = np.where(fin, wsum + weight, wsum)
+            vmin = np.where(fin, np.minimum(vmin, value), vmin)
+            vmax = np.where(fin, np.maximum(vmax, value), vmax)
+    ok = np.abs(wsum) > 1.0e-12
+    with np.errstate(divide="ignore", invalid="ignore"):
+        value = acc / wsum
+    if clamping_threshold >= 0.0:
+        rng = vmax - vmin
+        clamped = np.minimum(vmax + clamping_threshold * rng, np.maximum(vmin - clamping_threshold * rng, value))
+        value = np.where(np.isfinite(vmin), clamped, value)
+    return value.astype(np.float32), ok
+
+
+def warp_matrix(image, matrix, fill=0.0, interpolation="bilinear"):
+    """CPU-path matrix warp of a whole frame (engine/warp.py:155-228).
+
+    Returns (warped f32, coverage f32 0/1)."""
+    img = np.asarray(image, dtype=np.float32)
+    h, w = img.shape
+    out = np.full((h, w), fill, dtype=np.float32)
+    cov = np.zeros((h, w), dtype=np.float32)
+    sx, sy = source_coordinates(invert_matrix(matrix), h, w)
+    valid = np.isfinite(sx) & np.isfinite(sy)
+    if interpolation == "bilinear":
+        valid &= (sx >= 0.0) & (sx <= float(w - 1)) & (sy >= 0.0) & (sy <= float(h - 1))
+    elif interpolation == "lanczos3":
+        valid &= (sx >= 2.0) & (sx < float(w - 3)) & (sy >= 2.0) & (sy < float(h - 3))
+    else:
+        raise ValueError(f"unsupported warp interpolation: {interpolation}")
+    if not np.any(valid):
+        return out, cov
+    vx, vy = sx[valid], sy[valid]
+    if interpolation == "bilinear":
+        sampled = _bilinear(img, vx, vy)
+    else:
+        sampled = _lanczos_cpu(img, vx, vy)
+    out[valid] = sampled
+    cov[valid] = 1.0
+    return out, cov
+
+
+def warp_matrix_resident(image, matrix, fill=np.nan, interpolation="lanczos3", clamping_threshold=-1.0):
+    """Resident-backend semantics (tests/test_gpu_warp_vs_cpu.py:35-128).
+
+    Bilinear is identical to the CPU path; Lanczos3 skips non-finite taps,
+    uses the 1e-6 sinc threshold and the optional clamp.  A pixel whose finite
+    tap weights sum to <= 1e-12 keeps the fill value and coverage 0.
+    """
+    img = np.asarray(image, dtype=np.float32)
+    h, w = img.shape
+    out = np.full((h, w), fill, dtype=np.float32)
+    cov = np.zeros((h, w), dtype=np.float32)
+    sx, sy = source_coordinates(invert_matrix(matrix), h, w)
+    valid = np.isfinite(sx) & np.isfinite(sy)
+    if interpolation == "bilinear":
+        valid &= (sx >= 0.0) & (sx <= float(w - 1)) & (sy >= 0.0) & (sy <= float(h - 1))
+        if np.any(valid):
+            out[valid] = _bilinear(img, sx[valid], sy[valid])
+            cov[valid] = 1.0
+        return out, cov
+    valid &= (sx >= 2.0) & (sx < float(w - 3)) & (sy >= 2.0) & (sy < float(h - 3))
+    if not np.any(valid):
+        return out, cov
+    value, ok = _lanczos_resident(img, sx[valid], sy[valid], float(clamping_threshold))
+    idx = np.flatnonzero(valid.ravel())[ok]
+    out.ravel()[idx] = value[ok]
+    cov.ravel()[idx] = 1.0
+    return out, cov

@@ -1,0 +1,333 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle on the
+same seeded inputs.  Bit-exact where the arithmetic is (calibration, bilinear,
+rejection counts, rejection-free means), tolerance-based where the reference
+itself only pins a tolerance (Lanczos3 weights in f32, FP64 sums in a
+different order)."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from glass_amd import ResidentCalibratedStack
+from glass_amd import pipeline as P
+from glass_amd.synthetic import encode_fits_u16be_bzero, make_set, to_adu
+
+pytestmark = pytest.mark.gpu
+
+
+def _stack_of(frames):
+    frames = np.asarray(frames, np.float32)
+    s = ResidentCalibratedStack(frames.shape[0], frames.shape[1], frames.shape[2])
+    for i, f in enumerate(frames):
+        s.upload_calibrated_frame(i, f)
+    return s
+
+
+# ------------------------------------------------------------- calibration
+def test_calibrate_golden_314_bit_exact(gpu, golden_314):
+    g = golden_314
+    with ResidentCalibratedStack(3, 24, 24) as s:
+        s.set_calibration_masters(g["master_bias"], g["master_dark"], g["master_flat"])
+        for i in range(3):
+            s.calibrate_frame(i, g["raw_light"][i], float(g["light_exposure_s"][i]), float(g["dark_exposure_s"][i]),
+                              g["policy"])
+        got = s.download_frames_tile([0, 1, 2], 0, 0, 24, 24)
+    assert np.array_equal(got, g["calibrated_light"])
+
+
+@pytest.mark.parametrize("includes_bias,scaling,pedestal", [(True, True, 0.0), (False, True, 0.0), (True, False, 5.0)])
+def test_calibrate_formats_bit_exact(gpu, includes_bias, scaling, pedestal):
+    rng = np.random.default_rng(42)
+    h, w = 33, 47  # odd size exercises the scalar path
+    bias = rng.uniform(90, 110, (h, w)).astype(np.float32)
+    dark = rng.uniform(100, 140, (h, w)).astype(np.float32)
+    flat = rng.uniform(0.5, 1.5, (h, w)).astype(np.float32)
+    flat[3, 4] = 0.0  # below the floor
+    adu = rng.integers(0, 65536, size=(3, h, w), dtype=np.uint16)
+    policy = {"master_dark_includes_bias": includes_bias, "dark_scaling_enabled": scaling, "pedestal": pedestal,
+              "flat_floor": 1e-6}
+    with ResidentCalibratedStack(3, h, w) as s:
+        s.set_calibration_masters(bias, dark, flat)
+        s.calibrate_frame(0, adu[0].astype(np.float32), 37.0, 60.0, policy)
+        raw = encode_fits_u16be_bzero(adu[1])
+        s.calibrate_frames_fits_u16be_bzero_host_async_multistream_callback_release_timed(
+            [1], [raw], [37.0], [60.0], 1, 1, None, policy)
+        s.calibrate_frame(2, adu[2], 37.0, 60.0, policy)
+        got = s.download_frames_tile([0, 1, 2], 0, 0, w, h)
+    for i in range(3):
+        exp = oracle.calibrate_light(adu[i], bias, dark, flat, 37.0, 60.0,
+                                     master_dark_includes_bias=includes_bias, dark_scaling_enabled=scaling,
+                                     pedestal=pedestal)
+        assert np.array_equal(got[i], exp), i
+
+
+def test_calibrate_device_raw_with_fused_stats(gpu):
+    import torch
+
+    rng = np.random.default_rng(3)
+    h, w, f = 64, 96, 5
+    adu = rng.integers(1000, 5000, size=(f, h, w), dtype=np.uint16)
+    raw = torch.from_numpy(encode_fits_u16be_bzero(adu).reshape(f, -1).copy()).cuda()
+    bias = rng.uniform(900, 1000, (h, w)).astype(np.float32)
+    with ResidentCalibratedStack(f, h, w) as s:
+        s.set_calibration_masters(bias=bias)
+        s.calibrate_frames_device(list(range(f)), raw, "fits_u16be_bzero", [60.0] * f, None,
+                                  accumulate_stats=True)
+        got = s.download_frames_tile(list(range(f)), 0, 0, w, h)
+        stats = [s.calibration_stats(i) for i in range(f)]
+    for i in range(f):
+        exp = oracle.calibrate_light(adu[i], bias, None, None, 60.0, None)
+        assert np.array_equal(got[i], exp)
+        ref = oracle.frame_global_stats(exp)
+        assert stats[i]["valid_pixels"] == ref["valid_pixels"]
+        assert stats[i]["mean"] == pytest.approx(ref["mean"], rel=1e-14)
+        assert stats[i]["std"] == pytest.approx(ref["std"], rel=1e-9)
+
+
+# ----------------------------------------------------------------- masters
+def test_masters_golden_314_bit_exact(gpu, golden_314):
+    g = golden_314
+    pol = dict(g["policy"], tile_size=int(g["tile_size"]))
+    mb = P.build_master_bias(g["raw_bias"], pol)
+    md = P.build_master_dark(g["raw_dark"], mb, pol)
+    mf, _ = P.build_master_flat(g["raw_flat"], mb, pol)
+    assert np.array_equal(mb, g["master_bias"])
+    assert np.array_equal(md, g["master_dark"])
+    assert np.array_equal(mf, g["master_flat"])
+
+
+def test_masters_with_rejection_match_oracle(gpu):
+    rng = np.random.default_rng(17)
+    frames = rng.normal(1000, 5, size=(20, 40, 36)).astype(np.float32)
+    frames[3, 5, 5] = 5000.0
+    frames[7, 10:12, 3] = -100.0
+    frames[9, 20, 20] = np.nan  # one tile takes the per-pixel-count arithmetic
+    pol = {"tile_size": 16, "master_dark_includes_bias": False}
+    bias = rng.normal(100, 1, size=(40, 36)).astype(np.float32)
+    got_b = P.build_master_bias(frames, pol)
+    exp_b = oracle.build_master_bias(frames, pol)
+    assert np.array_equal(got_b, exp_b)
+    got_d = P.build_master_dark(frames, bias, pol)
+    exp_d = oracle.build_master_dark(frames, bias, pol)
+    assert np.array_equal(got_d, exp_d)
+    flats = rng.normal(1.0, 0.01, size=(6, 40, 36)).astype(np.float32) * 20000
+    got_f, sc = P.build_master_flat(flats, bias, pol)
+    exp_f, esc = oracle.build_master_flat(flats, bias, pol)
+    assert sc == esc
+    assert np.array_equal(got_f, exp_f)
+
+
+# -------------------------------------------------------------------- warp
+def _rot(deg, tx, ty, scale=1.0):
+    a = np.deg2rad(deg)
+    return np.array([[np.cos(a) * scale, -np.sin(a) * scale, tx], [np.sin(a) * scale, np.cos(a) * scale, ty],
+                     [0, 0, 1]], np.float32)
+
+
+@pytest.mark.parametrize("interp", ["bilinear", "lanczos3"])
+def test_warp_cpu_semantics_matches_oracle(gpu, interp):
+    rng = np.random.default_rng(5)
+    h, w = 57, 71
+    frames = rng.normal(100, 10, size=(3, h, w)).astype(np.float32)
+    mats = [_rot(0.7, 3.3, -2.6), _rot(-1.0, -0.4, 1.25, 1.0004), _rot(0.0, 2.0, 1.0)]
+    with _stack_of(frames) as s:
+        s.reset_warp_coverage()
+        s.apply_matrix_frames_cpu_semantics([0, 1, 2], np.stack(mats), interp, fill=0.0)
+        got = s.download_frames_tile([0, 1, 2], 0, 0, w, h)
+        geo = s.warp_coverage_map()
+    cov_sum = np.zeros((h, w), np.float32)
+    for i in range(3):
+        exp, cov = oracle.warp_matrix(frames[i], mats[i], fill=0.0, interpolation=interp)
+        cov_sum += cov
+        tol = 0.0 if interp == "bilinear" else 2e-4  # f32 Lanczos weights vs FP64 (values ~100)
+        assert np.allclose(got[i], exp, rtol=0, atol=tol), (interp, i, np.max(np.abs(got[i] - exp)))
+    assert np.array_equal(geo, cov_sum)
+
+
+def test_warp_resident_semantics_nan_taps_and_clamp(gpu):
+    yy, xx = np.indices((12, 13), dtype=np.float32)
+    data = 0.2 * xx + 0.4 * yy
+    data[:, 7:] += 5.0
+    data[6, 6] = np.nan
+    m = _rot(1.0, 0.35, -0.25)
+    with _stack_of(data[None]) as s:
+        s.apply_matrix_lanczos3_frame(0, m, np.nan, 0.30)
+        got = s.download_frame(0)
+    exp, cov = oracle.warp_matrix_resident(data, m, np.nan, "lanczos3", 0.30)
+    assert np.array_equal(np.isfinite(got), cov > 0)
+    assert np.allclose(got[cov > 0], exp[cov > 0], atol=3e-5)
+
+
+def test_warp_reference_case_bilinear(gpu):
+    # tests/test_gpu_warp_vs_cpu.py:154-173 on the resident class
+    data = np.arange(49, dtype=np.float32).reshape(7, 7)
+    m = _rot(3.0, 1.2, -0.4)
+    with _stack_of(data[None]) as s:
+        s.apply_matrix_bilinear_frame(0, m, -1.0)
+        got = s.download_frame(0)
+    exp, cov = oracle.warp_matrix_resident(data, m, -1.0, "bilinear")
+    assert np.array_equal(got, exp)
+
+
+# --------------------------------------------------------------- integrate
+def _random_stack(seed, f, h, w, nan_frac=0.02, outliers=True):
+    rng = np.random.default_rng(seed)
+    stack = rng.normal(1000.0, 4.0, size=(f, h, w)).astype(np.float32)
+    if outliers:
+        idx = rng.random(stack.shape) < 0.01
+        stack[idx] += rng.uniform(50, 500, size=int(idx.sum())).astype(np.float32)
+        idx = rng.random(stack.shape) < 0.005
+        stack[idx] -= rng.uniform(50, 500, size=int(idx.sum())).astype(np.float32)
+    if nan_frac:
+        stack[rng.random(stack.shape) < nan_frac] = np.nan
+    return stack
+
+
+def _compare(got, exp, exact_counts=True):
+    gm, gw, gc, gl, gh = got
+    em, ew, ec, el, eh = exp
+    assert np.array_equal(gc, ec)
+    assert np.array_equal(gl, el)
+    assert np.array_equal(gh, eh)
+    assert np.array_equal(gw, ew) or np.allclose(gw, ew, rtol=1e-7)
+    assert np.allclose(gm, em, rtol=2e-7, atol=0), np.max(np.abs(gm - em) / np.maximum(np.abs(em), 1))
+
+
+@pytest.mark.parametrize("frames", [1, 2, 3, 5, 16, 17, 33, 64, 65, 100, 129, 193, 208, 209, 256])
+def test_winsorized_unit_weights_matches_oracle(gpu, frames):
+    stack = _random_stack(frames, frames, 9, 31)
+    print(f"winsorized unit weights frames={frames}", flush=True)
+    with _stack_of(stack) as s:
+        got = s.integrate_hardened_winsorized_sigma(None, 3.0, 3.0, 3, 0.5)
+    exp = oracle.weighted_integrate_stack(stack, rejection="winsorized_sigma")
+    _compare(got, exp)
+
+
+@pytest.mark.parametrize("frames", [4, 20, 40, 120])
+def test_winsorized_weights_and_zero_weight_frames(gpu, frames):
+    rng = np.random.default_rng(100 + frames)
+    stack = _random_stack(frames, frames, 8, 24)
+    weights = rng.uniform(0.5, 2.0, frames).astype(np.float32)
+    weights[rng.random(frames) < 0.15] = 0.0
+    print(f"winsorized weighted frames={frames}", flush=True)
+    with _stack_of(stack) as s:
+        got = s.integrate_hardened_winsorized_sigma(weights, 2.5, 3.0, 2, 0.4)
+    exp = oracle.weighted_integrate_stack(stack, weights=weights, rejection="winsorized_sigma", low_sigma=2.5,
+                                          high_sigma=3.0, min_samples=2, max_reject_fraction=0.4)
+    _compare(got, exp)
+
+
+def test_winsorized_all_valid_variant(gpu):
+    stack = _random_stack(7, 25, 10, 12, nan_frac=0.0)
+    with _stack_of(stack) as s:
+        got = s.integrate_hardened_winsorized_sigma()
+    exp = oracle.weighted_integrate_stack(stack, rejection="winsorized_sigma")
+    _compare(got, exp)
+
+
+def test_winsorized_reference_literals(gpu):
+    frames = np.stack([
+        np.array([[1, 1], [10, 4]], np.float32), np.array([[1, 2], [10, 4]], np.float32),
+        np.array([[1, 3], [10, 4]], np.float32), np.array([[12, 4], [10, 40]], np.float32),
+    ])
+    weights = np.array([1.0, 2.0, 1.0, 1.0], np.float32)
+    with _stack_of(frames) as s:
+        master, weight, cov, low, high, prof = s.integrate_hardened_winsorized_sigma_timed(weights, 2.4, 2.4)
+    exp = oracle.weighted_integrate_stack(frames, weights=weights, rejection="winsorized_sigma", low_sigma=2.4,
+                                          high_sigma=2.4)
+    _compare((master, weight, cov, low, high), exp)
+    assert high[0, 0] == 1 and high[1, 1] == 1
+    frames = np.stack([np.full((2, 3), 10, np.float32)] * 4 + [np.array([[100, 10, 10], [10, -50, 10]], np.float32)])
+    with _stack_of(frames) as s:
+        got = s.integrate_hardened_winsorized_sigma(np.ones(5, np.float32), 2.0, 2.0, min_samples=1,
+                                                    max_reject_fraction=1.0, count_map_dtype="uint16")
+    assert got[2].dtype == np.uint16
+    assert int(got[4][0, 0]) == 1 and int(got[3][1, 1]) == 1
+    exp = oracle.weighted_integrate_stack(frames, weights=np.ones(5, np.float32), rejection="winsorized_sigma",
+                                          low_sigma=2.0, high_sigma=2.0, min_samples=1, max_reject_fraction=1.0)
+    _compare(tuple(np.asarray(x, np.float32) for x in got), exp)
+
+
+def test_winsorized_empty_and_guard_edge_cases(gpu):
+    stack = np.full((6, 3, 4), np.nan, np.float32)
+    stack[:, 0, 0] = [1, 1, 1, 1, 1, 50]
+    stack[:2, 1, 1] = [3, 4]
+    with _stack_of(stack) as s:
+        got = s.integrate_hardened_winsorized_sigma(None, 3.0, 3.0, 3, 0.5)
+        mean = s.integrate_mean()
+    exp = oracle.weighted_integrate_stack(stack, rejection="winsorized_sigma")
+    _compare(got, exp)
+    em, ew = oracle.mean_integrate([np.nan_to_num(f, nan=0.0) for f in stack])  # only for shape
+    assert mean[0].shape == em.shape
+
+
+def test_integrate_mean_matches_oracle(gpu):
+    stack = _random_stack(9, 11, 7, 9)
+    weights = np.linspace(0.5, 1.5, 11).astype(np.float32)
+    with _stack_of(stack) as s:
+        master, weight = s.integrate_mean(weights)
+    em, ew, *_ = oracle.weighted_integrate_stack(stack, weights=weights)
+    assert np.array_equal(master, em)
+    assert np.array_equal(weight, ew)
+
+
+def test_integrate_rejects_bad_arguments(gpu):
+    with _stack_of(np.ones((3, 2, 2), np.float32)) as s:
+        with pytest.raises(ValueError):
+            s.integrate_hardened_winsorized_sigma(np.ones(2, np.float32))
+        with pytest.raises(ValueError):
+            s.integrate_hardened_winsorized_sigma(None, -1.0)
+        with pytest.raises(ValueError):
+            s.integrate_hardened_winsorized_sigma(None, 3.0, 3.0, 0)
+        with pytest.raises(ValueError):
+            s.integrate_hardened_winsorized_sigma(None, 3.0, 3.0, 3, 1.5)
+        with pytest.raises(IndexError):
+            s.download_frame_tile(0, 0, 0, 3, 2)
+        with pytest.raises(IndexError):
+            s.upload_calibrated_frame(5, np.ones((2, 2), np.float32))
+
+
+# ------------------------------------------------------------ stats and DQ
+def test_frame_stats_exact_median_mad(gpu):
+    rng = np.random.default_rng(8)
+    for n_even in (0, 1):
+        frame = rng.normal(10, 3, size=(37, 41 + n_even)).astype(np.float32)
+        frame[0, :5] = np.nan
+        with _stack_of(frame[None]) as s:
+            st = s.frame_global_stats(0, exact_median=True)
+        ref = oracle.scan_quality_stats(frame)
+        assert st["median"] == ref["median"]
+        assert st["mad"] == ref["mad"]
+        assert st["valid_pixels"] == ref["pixel_count"]
+        assert st["mean"] == pytest.approx(ref["mean"], rel=1e-13)
+        assert st["std"] == pytest.approx(ref["rms"], rel=1e-9)
+
+
+def test_end_to_end_config0_and_dq(gpu):
+    data = make_set(seed=0)
+    pol = {"tile_size": 512}
+    mb = P.build_master_bias(data.bias, pol)
+    md = P.build_master_dark(data.dark, mb, pol)
+    mf, _ = P.build_master_flat(data.flat, mb, pol)
+    assert np.array_equal(mb, oracle.build_master_bias(data.bias, pol))
+    res = P.run_resident_stack(data.light, [data.light_exposure_s] * len(data.light),
+                               {"bias": mb, "dark": md, "flat": mf}, data.matrices,
+                               dark_exposure_s=data.dark_exposure_s, interpolation="bilinear")
+    cal = [oracle.calibrate_light(l, mb, md, mf, data.light_exposure_s, data.dark_exposure_s) for l in data.light]
+    warped, covs = [], []
+    for i, c in enumerate(cal):
+        if np.array_equal(data.matrices[i], np.eye(3)):
+            warped.append(c)
+            covs.append(np.ones_like(c))
+        else:
+            wimg, cov = oracle.warp_matrix(c, data.matrices[i], fill=0.0, interpolation="bilinear")
+            warped.append(wimg)
+            covs.append(cov)
+    exp = oracle.weighted_integrate_stack(np.stack(warped), coverage=np.stack(covs), rejection="winsorized_sigma")
+    _compare((res.master, res.weight, res.coverage, res.low, res.high), exp)
+    geo = np.sum(covs, axis=0).astype(np.float32)
+    assert np.array_equal(res.geometric, geo)
+    edq = oracle.resident_dq_map(exp[0], exp[1], exp[2], exp[3], exp[4], geo, active_frame_count=len(cal))
+    assert np.array_equal(res.dq, edq)
