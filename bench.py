@@ -1,0 +1,251 @@
+#!/usr/bin/env python3
+"""Resident stacking benchmark: calibrate -> Lanczos3 warp -> winsorized
+reject + integrate (+ weight / coverage / low / high / DQ maps) on one MI355X.
+
+Workload (default): 200 lights at 9600 x 6422 from FITS u16 (BZERO 32768,
+big-endian) raw samples already resident in HBM (24.7 GB), 20 bias / 20 dark /
+20 flat frames (masters built on the GPU before timing), affine drift up to
++-60 px and +-1 deg, 3000 stars with FWHM 2-4 px, 7 blurred frames given zero
+weight, Lanczos3 warp, winsorized sigma clip 3/3 over the frame axis.
+
+One step = one pass of the path over the whole light set: calibrate all
+frames from raw (f32 stack, 49.3 GB), warp the 199 non-reference frames,
+integrate the 193 positive-weight frames into the output maps.
+
+Prints one JSON line (rank 0).  --gpus > 1 is refused: the path runs on one
+GPU; see DESIGN.md for how it shards.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def parse_args():
+    p = argparse.ArgumentParser(description=__doc__)
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--frames", type=int, default=200)
+    p.add_argument("--height", type=int, default=6422)
+    p.add_argument("--width", type=int, default=9600)
+    p.add_argument("--calib-frames", type=int, default=20)
+    p.add_argument("--zero-weight", type=int, default=7)
+    p.add_argument("--interpolation", default="lanczos3", choices=["lanczos3", "bilinear"])
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-rows", type=int, default=32, help="rows per frame of the CPU baseline sample")
+    p.add_argument("--seed", type=int, default=2024)
+    return p.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(args, light_exposure, dark_exposure):
+    """Reference CPU path restated in C/OpenMP (oracle/cpu_ref.c), timed on the
+    host cores over a bounded sample of the same workload: all frames, the
+    first --cpu-rows rows, sub-pixel drift so the warp footprint covers the
+    sample.  Returns the baseline dict (frame-Mpix/s)."""
+    import numpy as np
+
+    from glass_amd.synthetic import affine_matrix, encode_fits_u16be_bzero, make_set, to_adu
+    from oracle import cpu_ref
+
+    f, h, w = args.frames, args.cpu_rows, args.width
+    data = make_set(seed=args.seed, n_lights=f, height=h, width=w, n_cal=3, n_stars=max(4, 3000 * h // args.height),
+                    fwhm=3.0, max_shift=2.0, max_rot_deg=0.005, sky=0.02)
+    raw = encode_fits_u16be_bzero(to_adu(data.light)).reshape(f, -1)
+    bias = to_adu(data.bias.mean(0)).astype(np.float32)
+    dark = to_adu(data.dark.mean(0)).astype(np.float32)
+    flat = np.maximum(data.flat.mean(0) / np.median(data.flat.mean(0)), 1e-6).astype(np.float32)
+    weights = np.ones(f, np.float32)
+    weights[1:1 + args.zero_weight] = 0.0
+    invs = [np.linalg.inv(m) for m in data.matrices]
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cpu_ref.load()
+    t0 = time.perf_counter()
+    stack = cpu_ref.calibrate_u16be(raw, h * w, bias, dark, flat, float(np.float32(light_exposure / dark_exposure)))
+    stack = stack.reshape(f, h, w)
+    for i in range(1, f):
+        stack[i], _ = cpu_ref.warp(stack[i], invs[i], args.interpolation, np.nan)
+    cpu_ref.integrate_winsorized(stack, weights)
+    elapsed = time.perf_counter() - t0
+    frame_mpix = f * h * w / 1e6
+    return {
+        "value": round(frame_mpix / elapsed, 3),
+        "unit": "frame-Mpix/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{f} frames x {h} x {w} (first {h} rows; sub-pixel drift), calibrate+{args.interpolation} "
+                  f"warp+winsorized integrate in oracle/cpu_ref.c, {elapsed:.2f} s",
+    }
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs the resident stacking path on one GPU; "
+                                   "--gpus > 1 is not supported (see DESIGN.md for the sharding design)"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import glass_amd
+    from glass_amd import pipeline as P
+    from glass_amd.synthetic_gpu import make_calibration_raw, make_light_raw
+
+    if not glass_amd.cuda_available():
+        raise SystemExit("no HIP device / libglass_hip.so not loadable")
+    dev = torch.device("cuda:0")
+    f, h, w = args.frames, args.height, args.width
+    npix = h * w
+    light_exp, dark_exp = 60.0, 60.0
+    log(f"generating {args.calib_frames}x3 calibration frames and {f} lights at {w}x{h} on the GPU")
+    t = time.perf_counter()
+    bias_raw = make_calibration_raw("bias", args.calib_frames, h, w, args.seed + 10)
+    dark_raw = make_calibration_raw("dark", args.calib_frames, h, w, args.seed + 20, dark_exposure_s=dark_exp)
+    flat_raw = make_calibration_raw("flat", args.calib_frames, h, w, args.seed + 30)
+    blurred = list(range(1, 1 + args.zero_weight))
+    light_raw, mats = make_light_raw(f, h, w, args.seed, blurred=blurred, light_exposure_s=light_exp)
+    torch.cuda.synchronize()
+    log(f"generated in {time.perf_counter() - t:.1f} s; building masters")
+    t = time.perf_counter()
+    masters = P.build_masters_device(bias_raw, dark_raw, flat_raw, h, w)
+    torch.cuda.synchronize()
+    master_s = time.perf_counter() - t
+    del bias_raw, dark_raw, flat_raw
+    torch.cuda.empty_cache()
+    log(f"masters built in {master_s:.2f} s")
+
+    weights = np.ones(f, np.float32)
+    weights[blurred] = 0.0
+    n_active = int(np.count_nonzero(weights))
+    moving = [i for i in range(f) if not np.array_equal(mats[i], np.eye(3))]
+    stack = glass_amd.ResidentCalibratedStack(f, h, w)
+    stack.set_calibration_masters(masters["bias"], masters["dark"], masters["flat"])
+    outs = {
+        "master": torch.empty((h, w), dtype=torch.float32, device=dev),
+        "weight": torch.empty((h, w), dtype=torch.float32, device=dev),
+        "coverage": torch.empty((h, w), dtype=torch.int16, device=dev),
+        "low": torch.empty((h, w), dtype=torch.int16, device=dev),
+        "high": torch.empty((h, w), dtype=torch.int16, device=dev),
+        "dq": torch.empty((h, w), dtype=torch.int32, device=dev),
+    }
+    stream = torch.cuda.current_stream(dev)
+    sh = stream.cuda_stream
+    policy = {"master_dark_includes_bias": True}
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    stage_ms = {"calibrate": [], "warp": [], "integrate": []}
+
+    def step(timed):
+        if timed:
+            ev[0].record(stream)
+        stack.calibrate_frames_device(list(range(f)), light_raw, "fits_u16be_bzero", [light_exp] * f, dark_exp,
+                                      policy, stream=sh)
+        if timed:
+            ev[1].record(stream)
+        stack.reset_warp_coverage()
+        for _ in range(f - len(moving)):
+            stack.accumulate_full_warp_coverage_frame()
+        stack.warp_frames(moving, mats[moving], args.interpolation, "cpu", stream=sh)
+        if timed:
+            ev[2].record(stream)
+        stack.integrate_to_device(outs, weights, "winsorized_sigma", 3.0, 3.0, 3, 0.5, "uint16",
+                                  dq_geometric=True, dq_active_frames=n_active, stream=sh)
+        if timed:
+            ev[3].record(stream)
+            torch.cuda.synchronize()
+            stage_ms["calibrate"].append(ev[0].elapsed_time(ev[1]))
+            stage_ms["warp"].append(ev[1].elapsed_time(ev[2]))
+            stage_ms["integrate"].append(ev[2].elapsed_time(ev[3]))
+
+    for i in range(args.warmup):
+        step(False)
+        torch.cuda.synchronize()
+        log(f"warmup step {i} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(True)
+        log(f"step {i}: calibrate {stage_ms['calibrate'][-1]:.1f} ms, warp {stage_ms['warp'][-1]:.1f} ms, "
+            f"integrate {stage_ms['integrate'][-1]:.1f} ms")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed * 1e3 / args.steps
+    frame_mpix = f * npix / 1e6
+    value = frame_mpix / (ms_per_step / 1e3)
+
+    # Algorithmic bytes (staged resident model): calibrate 2 B raw + 4 B f32 per
+    # frame-pixel (+12 B/px masters once), warp 8 B per moved frame-pixel,
+    # integrate 4 B per active frame-pixel + 18 B/px of output maps.
+    med = {k: float(np.median(v)) for k, v in stage_ms.items()}
+    bytes_cal = 6.0 * f * npix + 12.0 * npix
+    bytes_warp = 8.0 * len(moving) * npix
+    bytes_int = 4.0 * n_active * npix + 18.0 * npix
+    gbps = {
+        "calibrate": bytes_cal / (med["calibrate"] * 1e-3) / 1e9,
+        "warp": bytes_warp / (med["warp"] * 1e-3) / 1e9,
+        "integrate": bytes_int / (med["integrate"] * 1e-3) / 1e9,
+    }
+    dominant = max(med, key=med.get)
+    launches = {"calibrate": -(-f // 16), "warp": -(-len(moving) // 8), "integrate": 1}[dominant]
+    per_launch_bytes = {"calibrate": bytes_cal, "warp": bytes_warp, "integrate": bytes_int}[dominant] / launches
+    avg_launch_ms = med[dominant] / launches
+    total_bytes = bytes_cal + bytes_warp + bytes_int
+    line = {
+        "metric": "calib+warp+reject+integrate frame-Mpix/s & HBM %peak, 200x9600x6422",
+        "value": round(value, 2),
+        "unit": "frame-Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (u16 raw, f64 statistics)",
+        "data": "synthetic (GPU-generated, seeded)",
+        "config": {
+            "workload": f"{f} lights x {w}x{h} u16 FITS raw resident -> calibrate -> {args.interpolation} warp -> "
+                        f"winsorized sigma 3/3 integrate + maps",
+            "frames": f, "width": w, "height": h, "active_frames": n_active, "warped_frames": len(moving),
+            "interpolation": args.interpolation, "calibration_frames": args.calib_frames,
+        },
+        "stages_ms": {k: round(v, 3) for k, v in med.items()},
+        "stages_gbps": {k: round(v, 1) for k, v in gbps.items()},
+        "path_hbm_frac": round(total_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+        "master_build_s": round(master_s, 3),
+        "roofline": {
+            "kernel": dominant,
+            "bound": "hbm",
+            "achieved": round(per_launch_bytes / (avg_launch_ms * 1e-3) / 1e9, 1),
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(per_launch_bytes / (avg_launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+            "traffic": None,
+            "launches_per_step": launches,
+            "avg_launch_ms": round(avg_launch_ms, 4),
+        },
+    }
+    stack.close()
+    del light_raw
+    torch.cuda.empty_cache()
+    if not args.no_cpu_baseline:
+        log("timing the CPU baseline sample")
+        line["cpu_baseline"] = cpu_baseline(args, light_exp, dark_exp)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

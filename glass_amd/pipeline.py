@@ -119,6 +119,55 @@ def build_master_flat(frames, master_bias=None, policy: dict | None = None):
     return np.maximum(master, np.float32(floor)).astype(np.float32), scalars
 
 
+def build_masters_device(bias_raw, dark_raw, flat_raw, height: int, width: int, policy: dict | None = None,
+                         raw_format: str = "fits_u16be_bzero") -> dict:
+    """Masters from raw frames already in HBM (torch tensors [n, H*W]), same
+    semantics as build_master_* but without a host round trip; the masters
+    stay on the device as torch float32 tensors."""
+    import torch
+
+    pol = _policy(policy)
+    device = bias_raw.device
+    out: dict = {}
+
+    def stack_of(raw):
+        n = int(raw.shape[0])
+        s = ResidentCalibratedStack(n, height, width)
+        s.calibrate_frames_device(list(range(n)), raw, raw_format, [0.0] * n, None, pol)
+        return s
+
+    def integrate(s):
+        master = torch.empty((height, width), dtype=torch.float32, device=device)
+        s.integrate_stack_device(master, low_sigma=float(pol["master_rejection_low_sigma"]),
+                                 high_sigma=float(pol["master_rejection_high_sigma"]),
+                                 min_samples=int(pol["master_rejection_min_samples"]),
+                                 max_reject_fraction=float(pol["master_rejection_max_fraction"]),
+                                 tile_size=int(pol["tile_size"]))
+        return master
+
+    with stack_of(bias_raw) as s:
+        out["bias"] = integrate(s)
+    with stack_of(dark_raw) as s:
+        dark = integrate(s)
+    if not pol["master_dark_includes_bias"]:
+        with ResidentCalibratedStack(1, height, width) as tmp:
+            tmp.upload_calibrated_frame(0, dark)
+            tmp.set_calibration_masters(bias=out["bias"])
+            tmp.calibrate_frames_device([0], dark, "float32", [0.0], None, {"master_dark_includes_bias": False})
+            dark = torch.from_numpy(tmp.download_frame(0)).to(device)
+    out["dark"] = dark
+    n = int(flat_raw.shape[0])
+    with ResidentCalibratedStack(n, height, width) as s:
+        s.set_calibration_masters(bias=out["bias"])
+        s.calibrate_frames_device(list(range(n)), flat_raw, raw_format, [0.0] * n, None, pol)
+        scalars = [s.frame_global_stats(i, exact_median=True)["median"] for i in range(n)]
+        s.normalize_frames(list(range(n)), np.zeros(n), np.asarray(scalars, np.float32))
+        flat = integrate(s)
+    out["flat"] = torch.clamp_min(flat, float(np.float32(pol["flat_floor"])))
+    out["flat_scalars"] = scalars
+    return out
+
+
 def variance_aware_weights(rms: Sequence[float], active: Sequence[bool] | None = None) -> np.ndarray:
     """engine/integration.py:80-106 ``variance_aware`` from per-frame rms
     (``background_rms``, quality.py:294): 1/max(rms^2, 1e-12), floored at 1e-6,

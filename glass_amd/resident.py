@@ -512,6 +512,22 @@ class ResidentCalibratedStack:
                                    statistics_variant=-1, tile_size=int(tile_size), require_positive_scale=1)
         return maps
 
+    def integrate_stack_device(self, master_out, low_sigma=3.0, high_sigma=3.0, min_samples=3,
+                               max_reject_fraction=0.5, tile_size=512, stream=None) -> None:
+        """CPUStackEngine semantics (see integrate_stack_engine) into a GPU tensor."""
+        params = N.IntegrateParams(N.GH_REJECT_WINSORIZED_SIGMA, float(low_sigma), float(high_sigma),
+                                   int(min_samples), float(max_reject_fraction), N.GH_COUNT_F32, -1, 1,
+                                   int(tile_size))
+        self._integrate(None, params, set(), N.GH_COUNT_F32, device_outputs={"master": master_out}, stream=stream)
+
+    def warp_frames(self, indices, matrices, interpolation="lanczos3", semantics="cpu", fill=math.nan,
+                    clamping_threshold=-1.0, track_coverage=True, stream=None) -> dict:
+        """Batched matrix warp on a caller stream; semantics "cpu" (engine/warp.py)
+        or "resident" (non-finite taps skipped, optional clamp)."""
+        interp = N.GH_INTERP_LANCZOS3 if interpolation == "lanczos3" else N.GH_INTERP_BILINEAR
+        return self._warp(indices, matrices, interp, fill, clamping_threshold, track_coverage,
+                          semantics == "resident", stream)
+
     def integrate_to_device(self, outputs: dict, weights=None, rejection="winsorized_sigma", low_sigma=3.0,
                             high_sigma=3.0, min_samples=3, max_reject_fraction=0.5, count_map_dtype="uint16",
                             dq_geometric=True, dq_active_frames=0, stream=None) -> None:
