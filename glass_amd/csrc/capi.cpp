@@ -103,6 +103,8 @@ struct gh_stack {
   DeviceBuffer<unsigned> out_dq;
   DeviceBuffer<gh::FrameMoments> moments;
   DeviceBuffer<unsigned> radix;
+  DeviceBuffer<unsigned> warp_flags;        // [kWarpBatch] source-flag snapshot
+  DeviceBuffer<float> lanczos_lut;          // normalised Lanczos3 weight cubics
 
   hipStream_t pick(void* s) const { return s ? static_cast<hipStream_t>(s) : stream; }
 
@@ -180,6 +182,8 @@ struct gh_stack {
     out_dq.release();
     moments.release();
     radix.release();
+    warp_flags.release();
+    lanczos_lut.release();
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -436,6 +440,18 @@ int gh_warp_frames(gh_stack* s, const int64_t* indices, int64_t count, const dou
       s->ensure_geometric();
       p.coverage = s->geometric;
     }
+    if (interpolation == GH_INTERP_LANCZOS3 && p.affine) {
+      constexpr size_t kLutFloats = (gh::kLanczosLutIntervals + 1) * gh::kLanczosLutStride;
+      if (s->lanczos_lut.ptr == nullptr) {
+        std::vector<float> lut(kLutFloats);
+        gh::make_lanczos_lut(lut.data());
+        s->lanczos_lut.ensure(kLutFloats, "hipMalloc(lanczos table)");
+        check(hipMemcpy(s->lanczos_lut.ptr, lut.data(), kLutFloats * sizeof(float), hipMemcpyHostToDevice),
+              "hipMemcpy(lanczos table)");
+      }
+      p.lanczos_lut = reinterpret_cast<const float4*>(s->lanczos_lut.ptr);
+    }
+    s->warp_flags.ensure(gh::kWarpBatch, "hipMalloc(warp flags)");
     s->ensure_spares(static_cast<size_t>(std::min<int64_t>(count, gh::kWarpBatch)));
     for (int64_t start = 0; start < count; start += gh::kWarpBatch) {
       const int n = static_cast<int>(std::min<int64_t>(gh::kWarpBatch, count - start));
@@ -447,9 +463,8 @@ int gh_warp_frames(gh_stack* s, const int64_t* indices, int64_t count, const dou
         std::memcpy(b.inv[j], inverse + (start + j) * 9, 9 * sizeof(double));
         b.nonfinite[j] = s->nonfinite + f;
       }
-      gh::CalibBatch clear{};
-      for (int j = 0; j < n; ++j) clear.nonfinite[j] = b.nonfinite[j];
-      gh::launch_clear_flags(s->nonfinite, clear, n, st);
+      b.src_flags = s->warp_flags.ptr;
+      gh::launch_snapshot_warp_flags(b, n, s->warp_flags.ptr, st);
       gh::launch_warp(b, n, p, st);
       for (int j = 0; j < n; ++j) {
         const int64_t f = indices[start + j];

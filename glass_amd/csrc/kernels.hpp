@@ -78,6 +78,9 @@ struct WarpBatch {
   float* dst[kWarpBatch];
   double inv[kWarpBatch][9];
   unsigned* nonfinite[kWarpBatch];  // per-frame "holds a non-finite sample" flag (device)
+  // [count] device snapshot of the source frames' flags taken before the
+  // destination flags were cleared (launch_snapshot_warp_flags).
+  const unsigned* src_flags;
 };
 
 struct WarpParams {
@@ -88,7 +91,26 @@ struct WarpParams {
   int skip_nonfinite;
   int affine;         // 1: no projective divide
   float* coverage;    // geometric coverage accumulator (nullable)
+  // Normalised Lanczos3 weight table (see make_lanczos_lut); non-null selects
+  // the LDS-tiled kernel, valid for affine maps of all-finite frames without clamp.
+  const float4* lanczos_lut;
 };
+
+// Per-interval quadratic fits of the normalised Lanczos3 tap weights.
+// Interval i holds kLanczosLutStride floats; (a, b, c) of tap k sit at
+// [3k, 3k + 3), so that for f = (i + u) / intervals,
+// w_k(f) ~= a + u (b + u c) with w_k(f) = L(f - k + 2) / sum_j L(f - j + 2),
+// L(x) = sinc(x) sinc(x/3) (engine/warp.py:73-84), k = 0..5 for taps
+// floor(s) - 2 .. floor(s) + 3.  Each quadratic interpolates w_k at u = 0,
+// 1/2, 1 (exact delta weights at integer offsets); fitted in FP64 on the
+// host, max error after f32 evaluation ~1e-7.
+constexpr int kLanczosLutIntervals = 128;
+constexpr int kLanczosLutStride = 20;  // 18 used, padded to whole float4s
+// Row kLanczosLutIntervals (one past the last interval) holds the f == 1 weights.
+void make_lanczos_lut(float* table /* (kLanczosLutIntervals + 1) * kLanczosLutStride floats */);
+
+// snap[j] = *b.nonfinite[j]; *b.nonfinite[j] = 0 for j < count.
+void launch_snapshot_warp_flags(const WarpBatch& b, int count, unsigned* snap, hipStream_t stream);
 
 void launch_warp(const WarpBatch& batch, int count, const WarpParams& p, hipStream_t stream);
 

@@ -5,16 +5,21 @@
 //    divided by the projective denominator unless the inverse is affine;
 //  * bilinear valid 0 <= s <= dim-1, f32 lerps as warp.py:99-118;
 //  * Lanczos3 valid 2 <= s < dim-3, taps floor(s)-2 .. floor(s)+3
-//    (warp.py:121-152); weights sinc(d)*sinc(d/3) evaluated in f32 from one
-//    sin/cos pair per axis (shifted-sinc identity, see lanczos_axis);
+//    (warp.py:121-152); weights sinc(d)*sinc(d/3), normalised by their sum;
 //  * GH_WARP_SKIP_NONFINITE_TAPS / clamp: resident semantics of
 //    cpp/cuda/warp_kernels.cu:643-777 as pinned by
 //    tests/test_gpu_warp_vs_cpu.py:80-128.
 //
-// One thread owns one output pixel and walks the frames of the batch, so the
-// geometric coverage accumulator is updated with one plain read-modify-write
-// per pixel per launch (no atomics).  Frames are written out of place into a
-// spare slot; the host swaps slots afterwards.
+// Affine Lanczos3 warps run warp_lanczos3_tiled_kernel (LDS-staged source
+// boxes, tabulated normalised weights); bilinear and projective warps run the
+// direct kernels, where one thread owns one output pixel and evaluates the
+// weights from one sin/cos pair per axis (shifted-sinc identity, see
+// lanczos_axis).  Every kernel walks the frames of the batch per output
+// pixel, so the geometric coverage accumulator is updated with one plain
+// read-modify-write per pixel per launch (no atomics).  Frames are written out
+// of place into a spare slot; the host swaps slots afterwards.
+
+#include <cmath>
 
 #include "common.hpp"
 #include "kernels.hpp"
@@ -103,6 +108,64 @@ __global__ __launch_bounds__(kTileX* kTileY) void warp_bilinear_kernel(WarpBatch
   if (p.coverage) p.coverage[o] = p.coverage[o] + cov;
 }
 
+
+// Lanczos3 sample of one pixel read straight from the frame, covering every
+// semantic variant (CPU: no tap skipping, threshold 1e-8; resident: non-finite
+// taps skipped, threshold 1e-6, optional clamp).  `covered` reports whether
+// the pixel counts toward geometric coverage.
+__device__ __forceinline__ float lanczos_direct(const float* frame, int width, double sx, double sy,
+                                                const WarpParams& p, float thr, bool& covered) {
+  const double fx0 = floor(sx), fy0 = floor(sy);
+  const int x0 = static_cast<int>(fx0), y0 = static_cast<int>(fy0);
+  float wx[6], wy[6];
+  lanczos_axis(static_cast<float>(sx - fx0), thr, wx);
+  lanczos_axis(static_cast<float>(sy - fy0), thr, wy);
+  const float* src = frame + static_cast<long long>(y0 - 2) * width + (x0 - 2);
+  float acc = 0.0f, wsum = 0.0f;
+  float vmin = INFINITY, vmax = -INFINITY;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    const float* row = src + static_cast<long long>(j) * width;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const float v = row[i];
+      const float w = wy[j] * wx[i];
+      if (p.skip_nonfinite) {
+        if (isfinite(v)) {
+          acc = fmaf(v, w, acc);
+          wsum += w;
+          vmin = fminf(vmin, v);
+          vmax = fmaxf(vmax, v);
+        }
+      } else if (wx[i] != 0.0f && wy[j] != 0.0f) {
+        acc = fmaf(v, w, acc);
+        wsum += w;
+      }
+    }
+  }
+  covered = false;
+  float value = p.fill;
+  if (p.skip_nonfinite) {
+    if (fabsf(wsum) > 1.0e-12f) {
+      value = acc / wsum;
+      if (p.clamp >= 0.0f && vmin <= vmax) {
+        const float r = vmax - vmin;
+        value = fminf(vmax + p.clamp * r, fmaxf(vmin - p.clamp * r, value));
+      }
+      covered = true;
+    }
+  } else {
+    value = (fabsf(wsum) > 1.0e-12f) ? acc / wsum : 0.0f;
+    covered = true;
+  }
+  return value;
+}
+
+__device__ __forceinline__ bool lanczos_footprint(double sx, double sy, int width, int height) {
+  return isfinite(sx) && isfinite(sy) && sx >= 2.0 && sx < static_cast<double>(width - 3) && sy >= 2.0 &&
+         sy < static_cast<double>(height - 3);
+}
+
 template <bool AFFINE>
 __global__ __launch_bounds__(kTileX* kTileY) void warp_lanczos3_kernel(WarpBatch b, int count, WarpParams p) {
   const int x = blockIdx.x * kTileX + (threadIdx.x % kTileX);
@@ -113,67 +176,416 @@ __global__ __launch_bounds__(kTileX* kTileY) void warp_lanczos3_kernel(WarpBatch
   float cov = 0.0f;
   for (int f = 0; f < count; ++f) {
     double sx, sy;
-    bool ok = source_coords(b.inv[f], AFFINE, x, y, sx, sy);
-    ok = ok && isfinite(sx) && isfinite(sy) && sx >= 2.0 && sx < static_cast<double>(p.width - 3) && sy >= 2.0 &&
-         sy < static_cast<double>(p.height - 3);
+    const bool ok = source_coords(b.inv[f], AFFINE, x, y, sx, sy) && lanczos_footprint(sx, sy, p.width, p.height);
     float value = p.fill;
-    bool covered = false;
     if (ok) {
-      const double fx0 = floor(sx), fy0 = floor(sy);
-      const int x0 = static_cast<int>(fx0), y0 = static_cast<int>(fy0);
-      float wx[6], wy[6];
-      lanczos_axis(static_cast<float>(sx - fx0), thr, wx);
-      lanczos_axis(static_cast<float>(sy - fy0), thr, wy);
-      const float* src = b.src[f] + static_cast<long long>(y0 - 2) * p.width + (x0 - 2);
-      float acc = 0.0f, wsum = 0.0f;
-      float vmin = INFINITY, vmax = -INFINITY;
-#pragma unroll
-      for (int j = 0; j < 6; ++j) {
-        const float* row = src + static_cast<long long>(j) * p.width;
-#pragma unroll
-        for (int i = 0; i < 6; ++i) {
-          const float v = row[i];
-          const float w = wy[j] * wx[i];
-          if (p.skip_nonfinite) {
-            if (isfinite(v)) {
-              acc = fmaf(v, w, acc);
-              wsum += w;
-              vmin = fminf(vmin, v);
-              vmax = fmaxf(vmax, v);
-            }
-          } else if (wx[i] != 0.0f && wy[j] != 0.0f) {
-            acc = fmaf(v, w, acc);
-            wsum += w;
-          }
-        }
-      }
-      if (p.skip_nonfinite) {
-        if (fabsf(wsum) > 1.0e-12f) {
-          value = acc / wsum;
-          if (p.clamp >= 0.0f && vmin <= vmax) {
-            const float r = vmax - vmin;
-            value = fminf(vmax + p.clamp * r, fmaxf(vmin - p.clamp * r, value));
-          }
-          covered = true;
-        }
-      } else {
-        value = (fabsf(wsum) > 1.0e-12f) ? acc / wsum : 0.0f;
-        covered = true;
-      }
+      bool covered;
+      value = lanczos_direct(b.src[f], p.width, sx, sy, p, thr, covered);
+      if (covered) cov += 1.0f;
     }
-    if (covered) cov += 1.0f;
     b.dst[f][o] = value;
     if (!isfinite(value) && __ldg(b.nonfinite[f]) == 0u) atomicOr(b.nonfinite[f], 1u);
   }
   if (p.coverage) p.coverage[o] = p.coverage[o] + cov;
 }
 
+// ------------------------------------------------------------------------
+// Tiled Lanczos3 for affine maps.  A block of kWaves waves owns a kOutW x
+// kOutH output tile and walks the frames of the batch.  For a frame whose
+// samples are all finite (and no clamp is requested) the tile's source
+// bounding box, with the +-3 tap halo, is copied into LDS by pipelined
+// coalesced loads; every output pixel then reads its 36 taps from LDS and its
+// 12 tap weights from an LDS table of per-interval quadratics of the
+// *normalised* Lanczos3 weights, so there is no per-pixel transcendental,
+// division or weight-sum normalisation.  Source coordinates, the footprint
+// test and floor() stay FP64 and identical to the CPU path.  Frames holding
+// non-finite samples (or clamped resident warps) take lanczos_direct per pixel
+// inside the same launch; boxes too large for LDS read taps from the frame.
+//
+// Thread layout: lane tx of wave ty owns columns ox0 + tx and ox0 + tx + 64
+// of rows oy0 + ty + kWaves rr, rr = 0 .. kRowsPerThread - 1.  Its per-launch
+// coverage counts (<= kWarpBatch = 8 each) are packed 4 bits per row in one
+// u32 per column.
+//
+// Shape parameters (swept on MI355X: 4 waves, 128 x 32 outputs and a 29 KB
+// box buffer beat 2/8 waves, 16/24-row tiles and double-buffered asynchronous
+// global->LDS staging; the macros exist for such sweeps):
+//   GH_WARP_WAVES     waves per block
+//   GH_WARP_OUT_H     output tile rows (a multiple of the wave count)
+//   GH_WARP_MAX_TILE  floats of the LDS box buffer
+#ifndef GH_WARP_WAVES
+#define GH_WARP_WAVES 4
+#endif
+#ifndef GH_WARP_OUT_H
+#define GH_WARP_OUT_H 32
+#endif
+#ifndef GH_WARP_MAX_TILE
+#define GH_WARP_MAX_TILE 7424
+#endif
+constexpr int kWaves = GH_WARP_WAVES;
+constexpr int kTiledThreads = 64 * kWaves;
+constexpr int kOutW = 128;
+constexpr int kOutH = GH_WARP_OUT_H;
+constexpr int kRowsPerThread = kOutH / kWaves;
+constexpr int kMaxTile = GH_WARP_MAX_TILE;
+static_assert(kOutH % kWaves == 0, "output tile rows must split evenly over the waves");
+constexpr int kLutVec = kLanczosLutStride / 4;
+constexpr int kLutRows = kLanczosLutIntervals + 1;
+static_assert(kLanczosLutStride % 4 == 0, "table intervals are whole float4s");
+static_assert(kRowsPerThread * 4 <= 32 && kWarpBatch <= 15, "coverage counts are packed 4 bits per row");
+
+// Six normalised tap weights for fractional offset f in [0, 1].  The LDS table
+// is stored as kLutVec planes of kLutRows float4s (plane v holds floats
+// 4v .. 4v+3 of every interval), so lanes reading distinct intervals touch
+// consecutive 16-byte slots; row kLanczosLutIntervals serves f == 1 exactly.
+__device__ __forceinline__ void lut_weights(const float4* lut, float f, float (&w)[6]) {
+  const float t = f * static_cast<float>(kLanczosLutIntervals);
+  const int i = static_cast<int>(t);
+  const float u = __builtin_amdgcn_fractf(t);
+  const float4* e = lut + i;
+  float c[kLanczosLutStride];
+#pragma unroll
+  for (int v = 0; v < kLutVec; ++v) {
+    const float4 q = e[v * kLutRows];
+    c[4 * v + 0] = q.x;
+    c[4 * v + 1] = q.y;
+    c[4 * v + 2] = q.z;
+    c[4 * v + 3] = q.w;
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) w[k] = fmaf(u, fmaf(u, c[3 * k + 2], c[3 * k + 1]), c[3 * k]);
+}
+
+__device__ __forceinline__ float lanczos_taps(const float* t, int stride, const float (&wx)[6], const float (&wy)[6]) {
+  float acc = 0.0f;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    const float* r = t + j * stride;
+    float h = r[0] * wx[0];
+#pragma unroll
+    for (int i = 1; i < 6; ++i) h = fmaf(r[i], wx[i], h);
+    acc = j == 0 ? h * wy[0] : fmaf(h, wy[j], acc);
+  }
+  return acc;
+}
+
+struct TileFrame {
+  const float* src;
+  float* dst;
+  double m0, m1, m2, m3, m4, m5;
+  int bx0, by0, bw, bh;  // staged box origin, row stride and rows
+  bool fast;             // table weights (finite source, no clamp)
+  bool use_tile;         // fast and the box fits one LDS buffer
+};
+
+// Frame f's matrix, path choice and source bounding box for the output tile
+// [ox0, ox1] x [oy0, oy1].  An affine map takes its extremes at the corners;
+// interior points computed in FP64 stay within ~1e-11 px of the corners' hull,
+// so one pixel beyond the +-3 tap reach covers every tap.
+__device__ __forceinline__ TileFrame tile_frame(const WarpBatch& b, int f, bool clamp_on, int ox0, int oy0, int ox1,
+                                                int oy1, int W, int H) {
+  TileFrame fr;
+  fr.src = b.src[f];
+  fr.dst = b.dst[f];
+  const double* m = b.inv[f];
+  fr.m0 = m[0]; fr.m1 = m[1]; fr.m2 = m[2];
+  fr.m3 = m[3]; fr.m4 = m[4]; fr.m5 = m[5];
+  fr.fast = !clamp_on && b.src_flags[f] == 0u;
+  double mnx = INFINITY, mxx = -INFINITY, mny = INFINITY, mxy = -INFINITY;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    double sx, sy;
+    source_coords(m, 1, (c & 1) ? ox1 : ox0, (c & 2) ? oy1 : oy0, sx, sy);
+    mnx = fmin(mnx, sx);
+    mxx = fmax(mxx, sx);
+    mny = fmin(mny, sy);
+    mxy = fmax(mxy, sy);
+  }
+  const bool box_finite = fabs(mnx) < 1.0e9 && fabs(mxx) < 1.0e9 && fabs(mny) < 1.0e9 && fabs(mxy) < 1.0e9;
+  fr.bx0 = fr.by0 = fr.bw = fr.bh = 0;
+  if (box_finite) {
+    fr.bx0 = max(static_cast<int>(floor(mnx)) - 3, 0);
+    fr.by0 = max(static_cast<int>(floor(mny)) - 3, 0);
+    fr.bw = min(static_cast<int>(floor(mxx)) + 4, W - 1) - fr.bx0 + 1;
+    fr.bh = min(static_cast<int>(floor(mxy)) + 4, H - 1) - fr.by0 + 1;
+  }
+  // (boxes narrower than the 6-tap window only arise when no pixel of the
+  // tile is inside the footprint; they take the global-tap form)
+  fr.use_tile = fr.fast && box_finite && fr.bw >= 6 && fr.bh >= 6 && fr.bw * fr.bh <= kMaxTile;
+  return fr;
+}
+
+// Copies the frame's box [by0, by0 + bh) x [bx0, bx0 + bw) into tile[] (row
+// stride bw) with plain loads: element e = row * bw + col goes to tile[e];
+// thread t takes e = t, t + kTiledThreads, ... with 8 loads in flight per
+// round.  Loads past the end read the box origin instead (in bounds) and are
+// not stored.  The caller synchronises before and after.
+__device__ __forceinline__ void stage_box(const TileFrame& fr, int W, float* tile) {
+  constexpr int kUnroll = 8;
+  const int total = fr.bw * fr.bh;
+  const int dq = kTiledThreads / fr.bw, dr = kTiledThreads - dq * fr.bw;
+  const float* origin = fr.src + static_cast<long long>(fr.by0) * W + fr.bx0;
+  int e = threadIdx.x;
+  int row = e / fr.bw, col = e - row * fr.bw;
+  for (; e < total; e += kTiledThreads * kUnroll) {
+    float v[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const bool in = e + kTiledThreads * u < total;
+      v[u] = *(in ? origin + static_cast<long long>(row) * W + col : origin);
+      col += dr;
+      row += dq;
+      if (col >= fr.bw) {
+        col -= fr.bw;
+        row += 1;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      if (e + kTiledThreads * u < total) tile[e + kTiledThreads * u] = v[u];
+    }
+  }
+}
+
+// Pixels of one frame with table weights and taps from the LDS box.  Pixels
+// outside the footprint evaluate a safe in-box position and write `fill`, so
+// the body is branch-free and the two pixels of a row interleave.  FULL: the
+// whole output tile lies inside the frame (no per-lane edge tests).  Returns
+// whether any written value is non-finite.
+template <bool FULL>
+__device__ __forceinline__ bool tiled_lds_rows(const TileFrame& fr, const float* tile, const float4* lut, int ox0,
+                                               int oy0, int tx, int ty, int W, int H, float fill,
+                                               unsigned (&cov)[2]) {
+  bool bad = false;
+  const int xa = ox0 + tx;
+  double px[2], py[2];
+#pragma unroll
+  for (int xx = 0; xx < 2; ++xx) {
+    px[xx] = fr.m0 * static_cast<double>(xa + 64 * xx);
+    py[xx] = fr.m3 * static_cast<double>(xa + 64 * xx);
+  }
+  const double xmax = static_cast<double>(W - 3), ymax = static_cast<double>(H - 3);
+  const double safe_x = static_cast<double>(fr.bx0 + 2), safe_y = static_cast<double>(fr.by0 + 2);
+  // tile index of source (x0 - 2, y0 - 2) is y0 * bw + x0 + base
+  const int base = -(fr.by0 + 2) * fr.bw - (fr.bx0 + 2);
+#pragma unroll 1
+  for (int rr = 0; rr < kRowsPerThread; ++rr) {
+    const int y = oy0 + ty + kWaves * rr;
+    if (!FULL && y >= H) break;
+    const double qx = fr.m1 * static_cast<double>(y);
+    const double qy = fr.m4 * static_cast<double>(y);
+    float* drow = fr.dst + static_cast<long long>(y) * W;
+#pragma unroll
+    for (int xx = 0; xx < 2; ++xx) {
+      const int x = xa + 64 * xx;
+      // ((m0 x + m1 y) + m2) exactly as the CPU path rounds it
+      const double sx = (px[xx] + qx) + fr.m2;
+      const double sy = (py[xx] + qy) + fr.m5;
+      const bool ok = sx >= 2.0 && sx < xmax && sy >= 2.0 && sy < ymax;
+      const double sxc = ok ? sx : safe_x, syc = ok ? sy : safe_y;
+      const double fx0 = floor(sxc), fy0 = floor(syc);
+      float wx[6], wy[6];
+      lut_weights(lut, static_cast<float>(sxc - fx0), wx);
+      lut_weights(lut, static_cast<float>(syc - fy0), wy);
+      const int idx = static_cast<int>(fy0) * fr.bw + static_cast<int>(fx0) + base;
+      float value = lanczos_taps(tile + idx, fr.bw, wx, wy);
+      value = ok ? value : fill;
+      if (FULL || x < W) {
+        cov[xx] += ok ? (1u << (4 * rr)) : 0u;
+        drow[x] = value;
+        bad |= !isfinite(value);
+      }
+    }
+  }
+  return bad;
+}
+
+// Pixels of one frame with table weights and taps read straight from the
+// frame (source box too large for LDS).
+__device__ __forceinline__ bool tiled_global_rows(const TileFrame& fr, const float4* lut, int ox0, int oy0, int tx,
+                                                  int ty, int W, int H, float fill, unsigned (&cov)[2]) {
+  bool bad = false;
+  const double xmax = static_cast<double>(W - 3), ymax = static_cast<double>(H - 3);
+#pragma unroll 1
+  for (int rr = 0; rr < kRowsPerThread; ++rr) {
+#pragma unroll
+    for (int xx = 0; xx < 2; ++xx) {
+      const int x = ox0 + tx + 64 * xx;
+      const int y = oy0 + ty + kWaves * rr;
+      if (x < W && y < H) {
+        const double sx = (fr.m0 * static_cast<double>(x) + fr.m1 * static_cast<double>(y)) + fr.m2;
+        const double sy = (fr.m3 * static_cast<double>(x) + fr.m4 * static_cast<double>(y)) + fr.m5;
+        float value = fill;
+        if (sx >= 2.0 && sx < xmax && sy >= 2.0 && sy < ymax) {
+          const double fx0 = floor(sx), fy0 = floor(sy);
+          const int x0 = static_cast<int>(fx0), y0 = static_cast<int>(fy0);
+          float wx[6], wy[6];
+          lut_weights(lut, static_cast<float>(sx - fx0), wx);
+          lut_weights(lut, static_cast<float>(sy - fy0), wy);
+          value = lanczos_taps(fr.src + static_cast<long long>(y0 - 2) * W + (x0 - 2), W, wx, wy);
+          cov[xx] += 1u << (4 * rr);
+        }
+        fr.dst[static_cast<long long>(y) * W + x] = value;
+        bad |= !isfinite(value);
+      }
+    }
+  }
+  return bad;
+}
+
+// Pixels of one frame through lanczos_direct (non-finite sources, clamp).
+__device__ __forceinline__ bool tiled_direct_rows(const TileFrame& fr, const WarpParams& p, int ox0, int oy0, int tx,
+                                                  int ty, unsigned (&cov)[2]) {
+  bool bad = false;
+  const float thr = p.skip_nonfinite ? 1.0e-6f : 1.0e-8f;
+  const double m[6] = {fr.m0, fr.m1, fr.m2, fr.m3, fr.m4, fr.m5};
+#pragma unroll 1
+  for (int rr = 0; rr < kRowsPerThread; ++rr) {
+#pragma unroll 1
+    for (int xx = 0; xx < 2; ++xx) {
+      const int x = ox0 + tx + 64 * xx;
+      const int y = oy0 + ty + kWaves * rr;
+      if (x >= p.width || y >= p.height) continue;
+      double sx, sy;
+      source_coords(m, 1, x, y, sx, sy);
+      float value = p.fill;
+      if (lanczos_footprint(sx, sy, p.width, p.height)) {
+        bool covered;
+        value = lanczos_direct(fr.src, p.width, sx, sy, p, thr, covered);
+        if (covered) cov[xx] += 1u << (4 * rr);
+      }
+      fr.dst[static_cast<long long>(y) * p.width + x] = value;
+      bad |= !isfinite(value);
+    }
+  }
+  return bad;
+}
+
+__global__ __launch_bounds__(kTiledThreads) void warp_lanczos3_tiled_kernel(WarpBatch b, int count, WarpParams p,
+                                                                             int tiles_x) {
+  __shared__ float4 lut[kLutRows * kLutVec];
+  __shared__ float tile[kMaxTile];
+  // host layout [interval][vec] -> LDS planes [vec][interval]
+  for (int i = threadIdx.x; i < kLutRows * kLutVec; i += kTiledThreads) {
+    lut[(i % kLutVec) * kLutRows + i / kLutVec] = p.lanczos_lut[i];
+  }
+
+  // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+  // contiguous run of tiles so vertically adjacent tiles (which share source
+  // halo rows) meet in the same L2.
+  const unsigned nb = gridDim.x, bid = blockIdx.x;
+  const unsigned xcd = bid & 7u, q = nb >> 3, r = nb & 7u;
+  const unsigned logical = xcd * q + min(xcd, r) + (bid >> 3);
+  const int ox0 = static_cast<int>(logical % tiles_x) * kOutW;
+  const int oy0 = static_cast<int>(logical / tiles_x) * kOutH;
+
+  const int tx = threadIdx.x & 63;
+  const int ty = threadIdx.x >> 6;
+  const int W = p.width, H = p.height;
+  const bool clamp_on = p.skip_nonfinite && p.clamp >= 0.0f;
+  const int ox1 = min(ox0 + kOutW, W) - 1, oy1 = min(oy0 + kOutH, H) - 1;
+  const bool full = ox0 + kOutW <= W && oy0 + kOutH <= H;
+  unsigned cov[2] = {0u, 0u};
+
+  for (int f = 0; f < count; ++f) {
+    const TileFrame cur = tile_frame(b, f, clamp_on, ox0, oy0, ox1, oy1, W, H);
+    __syncthreads();  // previous frame's taps consumed (first pass: table stored)
+    if (cur.use_tile) stage_box(cur, W, tile);
+    __syncthreads();
+    bool bad;
+    if (cur.use_tile) {
+      bad = full ? tiled_lds_rows<true>(cur, tile, lut, ox0, oy0, tx, ty, W, H, p.fill, cov)
+                 : tiled_lds_rows<false>(cur, tile, lut, ox0, oy0, tx, ty, W, H, p.fill, cov);
+    } else if (cur.fast) {
+      bad = tiled_global_rows(cur, lut, ox0, oy0, tx, ty, W, H, p.fill, cov);
+    } else {
+      bad = tiled_direct_rows(cur, p, ox0, oy0, tx, ty, cov);
+    }
+    if (__any(bad) && tx == 0) atomicOr(b.nonfinite[f], 1u);
+  }
+  if (p.coverage) {
+#pragma unroll 1
+    for (int rr = 0; rr < kRowsPerThread; ++rr) {
+#pragma unroll
+      for (int xx = 0; xx < 2; ++xx) {
+        const int x = ox0 + tx + 64 * xx;
+        const int y = oy0 + ty + kWaves * rr;
+        if (x < W && y < H) {
+          const long long o = static_cast<long long>(y) * W + x;
+          p.coverage[o] = p.coverage[o] + static_cast<float>((cov[xx] >> (4 * rr)) & 15u);
+        }
+      }
+    }
+  }
+}
+
+__global__ void snapshot_warp_flags_kernel(WarpBatch b, int count, unsigned* snap) {
+  const int j = threadIdx.x;
+  if (j < count) {
+    snap[j] = *b.nonfinite[j];
+    *b.nonfinite[j] = 0u;
+  }
+}
+
 }  // namespace
+
+void make_lanczos_lut(float* table) {
+  // Normalised weights w_k(f), FP64 with the CPU sinc definition.
+  auto weights = [](double f, double (&w)[6]) {
+    constexpr double kPi = 3.14159265358979323846;
+    double sum = 0.0;
+    for (int k = 0; k < 6; ++k) {
+      const double d = std::fabs(f - static_cast<double>(k - 2));
+      double v = 0.0;
+      if (d < 3.0) {
+        const double a = d < 1.0e-8 ? 1.0 : std::sin(kPi * d) / (kPi * d);
+        const double b3 = d / 3.0 < 1.0e-8 ? 1.0 : std::sin(kPi * d / 3.0) / (kPi * d / 3.0);
+        v = a * b3;
+      }
+      // Integer offsets: the mathematical kernel is exactly a delta.
+      if (d != 0.0 && d == std::floor(d)) v = 0.0;
+      w[k] = v;
+      sum += v;
+    }
+    for (int k = 0; k < 6; ++k) w[k] /= sum;
+  };
+  for (int i = 0; i <= kLanczosLutIntervals; ++i) {
+    double y0[6], y1[6], y2[6];
+    weights((i + 0.0) / kLanczosLutIntervals, y0);
+    weights((i + 0.5) / kLanczosLutIntervals, y1);
+    weights((i + 1.0) / kLanczosLutIntervals, y2);
+    float* e = table + i * kLanczosLutStride;
+    for (int k = 0; k < 6; ++k) {
+      // p(u) = y0 + b u + c u^2 through (0, y0), (1/2, y1), (1, y2)
+      const double c = 2.0 * y2[k] - 4.0 * y1[k] + 2.0 * y0[k];
+      const double b = y2[k] - y0[k] - c;
+      e[3 * k + 0] = static_cast<float>(y0[k]);
+      e[3 * k + 1] = static_cast<float>(b);
+      e[3 * k + 2] = static_cast<float>(c);
+    }
+    for (int k = 18; k < kLanczosLutStride; ++k) e[k] = 0.0f;
+    if (i == kLanczosLutIntervals) {  // f == 1 exactly: constant delta on tap 3
+      for (int k = 0; k < 6; ++k) {
+        e[3 * k + 0] = k == 3 ? 1.0f : 0.0f;
+        e[3 * k + 1] = e[3 * k + 2] = 0.0f;
+      }
+    }
+  }
+}
+
+void launch_snapshot_warp_flags(const WarpBatch& b, int count, unsigned* snap, hipStream_t stream) {
+  snapshot_warp_flags_kernel<<<1, 64, 0, stream>>>(b, count, snap);
+  check(hipGetLastError(), "snapshot_warp_flags launch");
+}
 
 void launch_warp(const WarpBatch& batch, int count, const WarpParams& p, hipStream_t stream) {
   const dim3 grid((p.width + kTileX - 1) / kTileX, (p.height + kTileY - 1) / kTileY);
   const dim3 block(kTileX * kTileY);
-  if (p.interpolation == 1) {
+  if (p.interpolation == 3 && p.affine && p.lanczos_lut != nullptr) {
+    const int tiles_x = (p.width + kOutW - 1) / kOutW;
+    const int tiles_y = (p.height + kOutH - 1) / kOutH;
+    warp_lanczos3_tiled_kernel<<<tiles_x * tiles_y, kTiledThreads, 0, stream>>>(batch, count, p, tiles_x);
+  } else if (p.interpolation == 1) {
     if (p.affine) warp_bilinear_kernel<true><<<grid, block, 0, stream>>>(batch, count, p);
     else warp_bilinear_kernel<false><<<grid, block, 0, stream>>>(batch, count, p);
   } else {

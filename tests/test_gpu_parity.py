@@ -146,6 +146,55 @@ def test_warp_cpu_semantics_matches_oracle(gpu, interp):
     assert np.array_equal(geo, cov_sum)
 
 
+def test_warp_lanczos3_tiled_paths_match_oracle(gpu):
+    # 150 x 300 spans 5 x 3 output tiles; each frame drives one branch of the
+    # tiled kernel: LDS-staged taps (small rotation, zoom), global taps (40 deg
+    # rotation: the source box exceeds the LDS tile), per-pixel direct taps
+    # (a frame holding NaN), and an integer shift (delta weights: exact copy).
+    rng = np.random.default_rng(11)
+    h, w = 150, 300
+    frames = rng.normal(100, 10, size=(5, h, w)).astype(np.float32)
+    frames[2, 70, 150] = np.nan
+    mats = [_rot(0.4, 2.7, -1.9), _rot(40.0, 60.0, -80.0), _rot(-0.6, 1.1, 0.3), _rot(0.0, 3.0, -2.0),
+            _rot(0.2, -5.0, 4.0, 1.3)]
+    with _stack_of(frames) as s:
+        s.reset_warp_coverage()
+        s.apply_matrix_frames_cpu_semantics(list(range(5)), np.stack(mats), "lanczos3", fill=0.0)
+        got = s.download_frames_tile(list(range(5)), 0, 0, w, h)
+        geo = s.warp_coverage_map()
+    cov_sum = np.zeros((h, w), np.float32)
+    for i in range(5):
+        exp, cov = oracle.warp_matrix(frames[i], mats[i], fill=0.0, interpolation="lanczos3")
+        cov_sum += cov
+        assert np.array_equal(np.isnan(got[i]), np.isnan(exp)), i
+        ok = ~np.isnan(exp)
+        assert np.allclose(got[i][ok], exp[ok], rtol=0, atol=2e-4), (i, np.max(np.abs(got[i][ok] - exp[ok])))
+    assert np.array_equal(geo, cov_sum)
+    # integer shift: output (y, x) takes source (y + 2, x - 3) exactly
+    assert np.array_equal(got[3][2:h - 5, 5:w - 3], frames[3][4:h - 3, 2:w - 6])
+
+
+def test_warp_lanczos3_resident_tiled_matches_oracle(gpu):
+    rng = np.random.default_rng(12)
+    h, w = 100, 260
+    frames = rng.normal(500, 20, size=(3, h, w)).astype(np.float32)
+    frames[1, 40:43, 100] = np.nan
+    mats = [_rot(0.9, -1.4, 2.2), _rot(-0.3, 0.6, -0.8), _rot(25.0, 30.0, -40.0)]
+    with _stack_of(frames) as s:
+        s.reset_warp_coverage()
+        s.apply_matrix_lanczos3_frames([0, 1, 2], np.stack(mats), np.nan)
+        got = s.download_frames_tile([0, 1, 2], 0, 0, w, h)
+        geo = s.warp_coverage_map()
+    cov_sum = np.zeros((h, w), np.float32)
+    for i in range(3):
+        exp, cov = oracle.warp_matrix_resident(frames[i], mats[i], np.nan, "lanczos3")
+        cov_sum += cov
+        assert np.array_equal(np.isfinite(got[i]), cov > 0), i
+        ok = cov > 0
+        assert np.allclose(got[i][ok], exp[ok], rtol=0, atol=1e-3), (i, np.max(np.abs(got[i][ok] - exp[ok])))
+    assert np.array_equal(geo, cov_sum)
+
+
 def test_warp_resident_semantics_nan_taps_and_clamp(gpu):
     yy, xx = np.indices((12, 13), dtype=np.float32)
     data = 0.2 * xx + 0.4 * yy
