@@ -5,11 +5,11 @@
 // -ffp-contract=off):  x = raw; x -= dark*s; [x -= bias]; x /= max(flat, floor);
 // [x += pedestal].  Raw FITS u16 decode as cpp/cuda/calibration_kernels.cu:133-137.
 //
-// HBM bytes per frame-pixel: 2 (u16 raw) or 4 (f32 raw) read + 4 written; the
-// master tiles (12 B/px) are shared by every frame of a batch and stay in L2/MALL
-// for the batch.  Each thread handles 4 consecutive pixels (8 B / 16 B loads,
-// 16 B stores); the batch index runs on gridDim.y so one launch covers up to 16
-// frames.  Optional fused FP64 moments replace a separate stats pass over the stack.
+// HBM bytes per frame-pixel: 2 (u16 raw) or 4 (f32 raw) read + 4 written, plus
+// the masters (12 B/px) once per batch of up to 16 frames: each thread holds 4
+// consecutive pixels' masters in registers and walks every frame of the batch
+// (8 B / 16 B raw loads, 16 B stores).  Optional fused FP64 moments replace a
+// separate stats pass over the stack.
 
 #include "common.hpp"
 #include "kernels.hpp"
@@ -92,6 +92,149 @@ __device__ __forceinline__ float load_raw1(const void* src, long long i) {
   }
 }
 
+// Batched calibration: each thread owns kCalPx consecutive pixels, loads their
+// bias/dark/flat once and calibrates them in every frame of the batch, so the
+// masters cost 12 B per pixel per batch rather than per frame.  Raw loads are
+// issued kCalAhead frames at a time ahead of the arithmetic; calibrated frames
+// are written with non-temporal stores (they are read again only by the next
+// pass over the stack).  STATS adds the fused per-frame FP64 moments
+// (partials[k * gridDim.x + blockIdx.x]); MB is the frames per launch (the
+// statistics variant keeps 4 frames of FP64 sums in registers; the host splits
+// larger batches).
+#ifndef GH_CAL_PX
+#define GH_CAL_PX 4
+#endif
+#ifndef GH_CAL_AHEAD
+#define GH_CAL_AHEAD 4
+#endif
+#ifndef GH_CAL_NT
+#define GH_CAL_NT 1
+#endif
+constexpr int kCalPx = GH_CAL_PX;
+constexpr int kCalAhead = GH_CAL_AHEAD;
+static_assert(kCalPx == 4 || kCalPx == 8, "4 or 8 pixels per thread");
+
+template <int FMT>
+__device__ __forceinline__ void load_raw_px(const void* src, long long g, float (&v)[kCalPx]) {
+  if constexpr (kCalPx == 4) {
+    load_raw4<FMT>(src, g, v);
+  } else {
+    float a[4], b[4];
+    load_raw4<FMT>(src, 2 * g, a);
+    load_raw4<FMT>(src, 2 * g + 1, b);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      v[j] = a[j];
+      v[4 + j] = b[j];
+    }
+  }
+}
+
+__device__ __forceinline__ void load_master_px(const float* m, long long g, float (&v)[kCalPx]) {
+#pragma unroll
+  for (int h = 0; h < kCalPx / 4; ++h) {
+    const float4 q = reinterpret_cast<const float4*>(m)[g * (kCalPx / 4) + h];
+    v[4 * h + 0] = q.x; v[4 * h + 1] = q.y; v[4 * h + 2] = q.z; v[4 * h + 3] = q.w;
+  }
+}
+
+__device__ __forceinline__ void store_px(float* dst, long long g, const float (&v)[kCalPx]) {
+#pragma unroll
+  for (int h = 0; h < kCalPx / 4; ++h) {
+    typedef float f32x4 __attribute__((ext_vector_type(4)));
+    f32x4* o = reinterpret_cast<f32x4*>(dst) + g * (kCalPx / 4) + h;
+    const f32x4 q = {v[4 * h + 0], v[4 * h + 1], v[4 * h + 2], v[4 * h + 3]};
+    if constexpr (GH_CAL_NT) {
+      __builtin_nontemporal_store(q, o);
+    } else {
+      *o = q;
+    }
+  }
+}
+
+template <int FMT, bool STATS, int MB = STATS ? 4 : kCalibBatch>
+__global__ __launch_bounds__(kBlock) void calibrate_batch_kernel(CalibBatch batch, int count, CalibParams p,
+                                                                 long long groups, StatPartial* partials) {
+  constexpr int kAhead = MB < kCalAhead ? MB : kCalAhead;
+  static_assert(MB % kAhead == 0 && MB <= kCalibBatch, "batch splits into load groups");
+  double sum[MB], sum_sq[MB];
+  unsigned fin[MB], nonfin[MB];
+#pragma unroll
+  for (int k = 0; k < MB; ++k) {
+    sum[k] = sum_sq[k] = 0.0;
+    fin[k] = nonfin[k] = 0u;
+  }
+  unsigned bad = 0;  // bit k: frame k produced a non-finite value
+  const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+  for (long long g = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; g < groups; g += stride) {
+    float b[kCalPx] = {}, d[kCalPx] = {}, fl[kCalPx] = {};
+    if (p.has_bias) load_master_px(p.bias, g, b);
+    if (p.has_dark) load_master_px(p.dark, g, d);
+    if (p.has_flat) load_master_px(p.flat, g, fl);
+#pragma unroll
+    for (int k0 = 0; k0 < MB; k0 += kAhead) {
+      if (k0 >= count) break;
+      float v[kAhead][kCalPx];
+#pragma unroll
+      for (int a = 0; a < kAhead; ++a) {
+        if (k0 + a < count) load_raw_px<FMT>(batch.src[k0 + a], g, v[a]);
+      }
+#pragma unroll
+      for (int a = 0; a < kAhead; ++a) {
+        const int k = k0 + a;
+        if (k >= count) break;
+#pragma unroll
+        for (int j = 0; j < kCalPx; ++j) {
+          v[a][j] = calibrate_one(v[a][j], b[j], d[j], fl[j], p, batch.dark_scale[k]);
+          if (isfinite(v[a][j])) {
+            if constexpr (STATS) {
+              const double x = static_cast<double>(v[a][j]);
+              sum[k] += x;
+              sum_sq[k] += x * x;
+              ++fin[k];
+            }
+          } else {
+            bad |= 1u << k;
+            if constexpr (STATS) ++nonfin[k];
+          }
+        }
+        store_px(batch.dst[k], g, v[a]);
+      }
+    }
+  }
+  // Any non-finite output marks its frame (one atomic per wave that saw one).
+#pragma unroll
+  for (int k = 0; k < MB; ++k) {
+    if (k < count && __any((bad >> k) & 1u) && (threadIdx.x & 63) == 0) atomicOr(batch.nonfinite[k], 1u);
+  }
+  if constexpr (STATS) {
+    __shared__ StatPartial s_part[kBlock / 64][MB];
+    const int wave = threadIdx.x / 64;
+#pragma unroll
+    for (int k = 0; k < MB; ++k) {
+      if (k >= count) break;
+      const double ws = wave_sum(sum[k]), wq = wave_sum(sum_sq[k]);
+      const unsigned long long wf = wave_sum(static_cast<unsigned long long>(fin[k]));
+      const unsigned long long wn = wave_sum(static_cast<unsigned long long>(nonfin[k]));
+      if ((threadIdx.x & 63) == 0) s_part[wave][k] = StatPartial{ws, wq, wf, wn};
+    }
+    __syncthreads();
+    if (threadIdx.x < count) {
+      const int k = threadIdx.x;
+      StatPartial out{0.0, 0.0, 0, 0};
+      for (int w = 0; w < kBlock / 64; ++w) {
+        out.sum += s_part[w][k].sum;
+        out.sum_sq += s_part[w][k].sum_sq;
+        out.finite += s_part[w][k].finite;
+        out.nonfinite += s_part[w][k].nonfinite;
+      }
+      partials[static_cast<long long>(k) * gridDim.x + blockIdx.x] = out;
+    }
+  }
+}
+
+// One frame per blockIdx.y; serves frames whose pixel count is not a multiple
+// of the batched kernel's pixels per thread (VEC = 4 or 1).
 template <int FMT, int VEC>
 __global__ __launch_bounds__(kBlock) void calibrate_kernel(CalibBatch batch, CalibParams p, long long npix,
                                                            StatPartial* partials) {
@@ -370,7 +513,33 @@ void launch_calibrate(const CalibBatch& batch, int count, const CalibParams& p, 
   const bool vec4 = (npix % 4 == 0);
   const dim3 grid(calibrate_blocks_per_frame(npix), count);
 #define GH_CAL(FMT, VEC) calibrate_kernel<FMT, VEC><<<grid, kBlock, 0, stream>>>(batch, p, npix, partials)
-  if (vec4) {
+#define GH_CALB(FMT, STATS) \
+  calibrate_batch_kernel<FMT, STATS><<<grid.x, kBlock, 0, stream>>>(batch, count, p, npix / kCalPx, partials)
+#define GH_CALS(FMT)                                                                                      \
+  calibrate_batch_kernel<FMT, true><<<grid.x, kBlock, 0, stream>>>(sub, n, p, npix / kCalPx,                   \
+                                                                   partials + static_cast<long long>(k0) * grid.x)
+  if (npix % kCalPx == 0) {
+    if (p.accumulate_stats) {
+      for (int k0 = 0; k0 < count; k0 += 4) {
+        const int n = count - k0 < 4 ? count - k0 : 4;
+        CalibBatch sub{};
+        for (int j = 0; j < n; ++j) {
+          sub.src[j] = batch.src[k0 + j];
+          sub.dst[j] = batch.dst[k0 + j];
+          sub.dark_scale[j] = batch.dark_scale[k0 + j];
+          sub.nonfinite[j] = batch.nonfinite[k0 + j];
+          sub.frame[j] = batch.frame[k0 + j];
+        }
+        if (p.format == kFmtF32) GH_CALS(kFmtF32);
+        else if (p.format == kFmtU16) GH_CALS(kFmtU16);
+        else GH_CALS(kFmtBE);
+      }
+    } else {
+      if (p.format == kFmtF32) GH_CALB(kFmtF32, false);
+      else if (p.format == kFmtU16) GH_CALB(kFmtU16, false);
+      else GH_CALB(kFmtBE, false);
+    }
+  } else if (vec4) {
     if (p.format == kFmtF32) GH_CAL(kFmtF32, 4);
     else if (p.format == kFmtU16) GH_CAL(kFmtU16, 4);
     else GH_CAL(kFmtBE, 4);
@@ -380,6 +549,8 @@ void launch_calibrate(const CalibBatch& batch, int count, const CalibParams& p, 
     else GH_CAL(kFmtBE, 1);
   }
 #undef GH_CAL
+#undef GH_CALB
+#undef GH_CALS
   check(hipGetLastError(), "calibrate kernel launch");
 }
 

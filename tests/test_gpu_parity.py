@@ -86,6 +86,50 @@ def test_calibrate_device_raw_with_fused_stats(gpu):
         assert stats[i]["std"] == pytest.approx(ref["std"], rel=1e-9)
 
 
+@pytest.mark.parametrize("stats", [False, True])
+def test_calibrate_device_batches_flags_and_stats(gpu, stats):
+    # 37 frames in shuffled order span three 16-frame batches (and 4-frame
+    # statistics launches); a NaN flat pixel then makes every frame non-finite.
+    import torch
+
+    rng = np.random.default_rng(8)
+    h, w, f = 40, 52, 37
+    adu = rng.integers(500, 60000, size=(f, h, w), dtype=np.uint16)
+    raw = torch.from_numpy(encode_fits_u16be_bzero(adu).reshape(f, -1).copy()).cuda()
+    bias = rng.uniform(90, 110, (h, w)).astype(np.float32)
+    dark = rng.uniform(100, 140, (h, w)).astype(np.float32)
+    flat = rng.uniform(0.5, 1.5, (h, w)).astype(np.float32)
+    nan_flat = flat.copy()
+    nan_flat[7, 9] = np.nan
+    exposures = list(rng.uniform(10, 90, f))
+    policy = {"master_dark_includes_bias": False, "flat_floor": 1e-6}
+    order = list(rng.permutation(f))
+    with ResidentCalibratedStack(f, h, w) as s:
+        s.set_calibration_masters(bias, dark, flat)
+        s.calibrate_frames_device(order, raw[order], "fits_u16be_bzero", [exposures[i] for i in order], 60.0,
+                                  policy, accumulate_stats=stats)
+        got = s.download_frames_tile(list(range(f)), 0, 0, w, h)
+        flags = [s.frame_has_nonfinite(i) for i in range(f)]
+        st = [s.calibration_stats(i) for i in range(f)] if stats else None
+        s.set_calibration_masters(bias, dark, nan_flat)
+        s.calibrate_frames_device(list(range(f)), raw, "fits_u16be_bzero", exposures, 60.0, policy,
+                                  accumulate_stats=stats)
+        got_nan = s.download_frames_tile(list(range(f)), 0, 0, w, h)
+        flags_nan = [s.frame_has_nonfinite(i) for i in range(f)]
+    assert not any(flags) and all(flags_nan)
+    for i in range(f):
+        exp = oracle.calibrate_light(adu[i], bias, dark, flat, exposures[i], 60.0, master_dark_includes_bias=False)
+        assert np.array_equal(got[i], exp), i
+        exp_nan = oracle.calibrate_light(adu[i], bias, dark, nan_flat, exposures[i], 60.0,
+                                         master_dark_includes_bias=False)
+        assert np.array_equal(got_nan[i], exp_nan, equal_nan=True), i
+        if stats:
+            ref = oracle.frame_global_stats(exp)
+            assert st[i]["valid_pixels"] == ref["valid_pixels"]
+            assert st[i]["mean"] == pytest.approx(ref["mean"], rel=1e-13)
+            assert st[i]["std"] == pytest.approx(ref["std"], rel=1e-9)
+
+
 # ----------------------------------------------------------------- masters
 def test_masters_golden_314_bit_exact(gpu, golden_314):
     g = golden_314
