@@ -180,6 +180,7 @@ __device__ __forceinline__ void sort_registers(Samples<CAP>& v) {
 struct Quartiles {
   float a25, b25, a50, b50, a75, b75;
   float t25, t50, t75;
+  float mx;  // largest valid sample
 };
 
 // Linear-percentile ranks for `count` samples: position (count-1)*fraction is
@@ -200,6 +201,7 @@ __device__ __forceinline__ void pick_ranks(int c, const Samples<CAP>& v, Quartil
   q.a75 = v.at_uniform(l75);
   q.b75 = v.at_uniform(min(l75 + 1, last));
   q.t75 = static_cast<float>(p75 & 3) * 0.25f;
+  q.mx = v.at_uniform(max(last, 0));
 }
 
 __device__ __forceinline__ float lerp_percentile(float a, float b, float t, bool all_valid) {
@@ -282,7 +284,7 @@ __device__ __attribute__((noinline)) float frame_axis_std(const IntegrateArgs& a
 
 // ------------------------------------------------------------------ kernels
 template <int CAP, bool UNIT>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void integrate_winsorized_kernel(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CAP > 208 ? 1 : 2, 2))) void integrate_winsorized_kernel(
     IntegrateArgs a) {
   const long long p = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
   const bool in_range = p < a.npix;
@@ -291,15 +293,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
   Samples<CAP> v;
   int cnt = 0;
   double swx = 0.0, sw = 0.0;
-  float mn = INFINITY, mx = -INFINITY;
   // Load pass: every capacity slot issues its load before any use, so all
   // loads of the pixel are in flight together.  The frame table is padded to
   // the capacity with the last active frame (re-reads hit L2), which keeps the
-  // table index a compile-time constant (merged scalar loads).
+  // table index a compile-time constant (merged scalar loads).  Each group of
+  // 16 table entries is addressed through a fresh opaque zero so the scalar
+  // loads of the pointers stay next to their use: hoisted to the kernel entry,
+  // CAP 64-bit pointers overflow the SGPRs and spill.
   const unsigned byte_offset = static_cast<unsigned>(pp) * 4u;  // host guarantees npix < 2^30
+  int zero = 0;
   static_for<CAP>([&](auto K) GH_ALWAYS_INLINE {
     constexpr int k = decltype(K)::value;
-    v.template set<k>(load_global_off(a.frame_ptr[k], byte_offset));
+    if constexpr (k % 16 == 0) asm volatile("" : "+s"(zero));
+    v.template set<k>(load_global_off(a.frame_ptr[k + zero], byte_offset));
   });
   // Invalid samples add +0.0, as the oracle's np.where(valid, x, 0.0) * 0.0 does.
   static_for<CAP>([&](auto K) GH_ALWAYS_INLINE {
@@ -308,8 +314,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     const float x = v.template get<k>();
     const bool ok = (k < a.n_active) && isfinite(x);
     cnt += ok ? 1 : 0;
-    mn = ok ? fminf(mn, x) : mn;
-    mx = ok ? fmaxf(mx, x) : mx;
     if constexpr (UNIT) {
       swx += static_cast<double>(ok ? x : 0.0f);
     } else {
@@ -323,7 +327,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
   // Force the load-pass reductions to complete here.  Otherwise the compiler
   // defers them (they are only needed at the end) and keeps every raw sample
   // alive next to its sort key: twice the registers.
-  asm volatile("" ::"v"(swx), "v"(sw), "v"(cnt), "v"(mn), "v"(mx));
+  asm volatile("" ::"v"(swx), "v"(sw), "v"(cnt));
 
   // No branch between the load pass and the network: a branch here lets the
   // compiler sink the key selects past it and keep raw values and masks alive
@@ -376,11 +380,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
 
   // Winsorized mean and population variance in FP64.  np.clip(x, lo, hi) is
   // the median of (x, lo, hi) because lo <= hi.
+  // Each pass below masks slots k >= cnt through its own opaque copy of cnt:
+  // with one shared cnt the compiler keeps the CAP lane masks (k < cnt) alive
+  // across the passes, which overflows the SGPRs.
   double ws = 0.0;
+  int cnt_ws = cnt;
+  asm volatile("" : "+v"(cnt_ws));
   static_for<CAP>([&](auto K) GH_ALWAYS_INLINE {
     constexpr int k = decltype(K)::value;
     const float w = __builtin_amdgcn_fmed3f(v.template get<k>(), lo_clip, hi_clip);
-    ws += static_cast<double>((k < cnt) ? w : 0.0f);
+    ws += static_cast<double>((k < cnt_ws) ? w : 0.0f);
   });
   double wmean = ws / n;
   float center = static_cast<float>(wmean);
@@ -389,12 +398,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
   // (one med3 each) instead of being kept live from the previous pass, which
   // would hold CAP doubles in registers and spill.
   float lo_clip2 = lo_clip, hi_clip2 = hi_clip;
-  asm volatile("" : "+v"(lo_clip2), "+v"(hi_clip2));
+  int cnt_wss = cnt;
+  asm volatile("" : "+v"(lo_clip2), "+v"(hi_clip2), "+v"(cnt_wss));
   double wss = 0.0;
   static_for<CAP>([&](auto K) GH_ALWAYS_INLINE {
     constexpr int k = decltype(K)::value;
     const float w = __builtin_amdgcn_fmed3f(v.template get<k>(), lo_clip2, hi_clip2);
-    const double d = (k < cnt) ? static_cast<double>(w) - wmean : 0.0;
+    const double d = (k < cnt_wss) ? static_cast<double>(w) - wmean : 0.0;
     wss += d * d;
   });
   float scale = static_cast<float>(sqrt(wss / n));
@@ -410,14 +420,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
 
   int n_low = 0, n_high = 0;
   const bool usable = !a.require_positive_scale || (isfinite(scale) && scale > 0.0f);
+  // Sorted keys: the smallest valid sample is slot 0, the largest rank cnt-1.
+  const float mn = v.template get<0>(), mx = q.mx;
   if (usable && !(mn >= lo_t && mx <= hi_t)) {
+    int cnt_count = cnt;
+    asm volatile("" : "+v"(cnt_count));
+    // Branch-free: invalid slots hold +inf, which is never below lo_t.
     static_for<CAP>([&](auto K) GH_ALWAYS_INLINE {
       constexpr int k = decltype(K)::value;
-      if (k < cnt) {
-        const float x = v.template get<k>();
-        n_low += (x < lo_t) ? 1 : 0;
-        n_high += (x > hi_t) ? 1 : 0;
-      }
+      const float x = v.template get<k>();
+      n_low += (x < lo_t) ? 1 : 0;
+      n_high += (x > hi_t && k < cnt_count) ? 1 : 0;
     });
     const int rejected = n_low + n_high;
     if (rejected > 0) {
@@ -439,12 +452,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     // Opaque threshold copies keep the compares of the counting pass from
     // being kept alive (CAP lane masks) for this pass.
     float lo_t2 = lo_t, hi_t2 = hi_t;
-    asm volatile("" : "+v"(lo_t2), "+v"(hi_t2));
+    int cnt_sum = cnt;
+    asm volatile("" : "+v"(lo_t2), "+v"(hi_t2), "+v"(cnt_sum));
     double s = 0.0;
     static_for<CAP>([&](auto K) GH_ALWAYS_INLINE {
       constexpr int k = decltype(K)::value;
       const float x = v.template get<k>();
-      s += static_cast<double>((k < cnt && x >= lo_t2 && x <= hi_t2) ? x : 0.0f);
+      s += static_cast<double>((k < cnt_sum && x >= lo_t2 && x <= hi_t2) ? x : 0.0f);
     });
     const int kept = cnt - n_low - n_high;
     master = static_cast<float>(s / static_cast<double>(kept));
