@@ -20,6 +20,7 @@
 // of place into a spare slot; the host swaps slots afterwards.
 
 #include <cmath>
+#include <utility>
 
 #include "common.hpp"
 #include "kernels.hpp"
@@ -194,18 +195,20 @@ __global__ __launch_bounds__(kTileX* kTileY) void warp_lanczos3_kernel(WarpBatch
 // kOutH output tile and walks the frames of the batch.  For a frame whose
 // samples are all finite (and no clamp is requested) the tile's source
 // bounding box, with the +-3 tap halo, is copied into LDS by pipelined
-// coalesced loads; every output pixel then reads its 36 taps from LDS and its
-// 12 tap weights from an LDS table of per-interval quadratics of the
-// *normalised* Lanczos3 weights, so there is no per-pixel transcendental,
-// division or weight-sum normalisation.  Source coordinates, the footprint
+// coalesced loads; every output pixel then takes its 36 taps from LDS (through
+// a per-lane register window, see strip_window_rows) and its 12 tap weights
+// from an LDS table of per-interval quadratics of the *normalised* Lanczos3
+// weights, so there is no per-pixel transcendental, division or weight-sum
+// normalisation.  Source coordinates, the footprint
 // test and floor() stay FP64 and identical to the CPU path.  Frames holding
 // non-finite samples (or clamped resident warps) take lanczos_direct per pixel
 // inside the same launch; boxes too large for LDS read taps from the frame.
 //
-// Thread layout: lane tx of wave ty owns columns ox0 + tx and ox0 + tx + 64
-// of rows oy0 + ty + kWaves rr, rr = 0 .. kRowsPerThread - 1.  Its per-launch
-// coverage counts (<= kWarpBatch = 8 each) are packed 4 bits per row in one
-// u32 per column.
+// Thread layout: the block's waves split the tile into kColGroups column
+// groups of 64 and row bands of kLaneRows; lane tx of wave ty owns column
+// ox0 + 64 (ty % kColGroups) + tx of the kLaneRows rows starting at
+// oy0 + kLaneRows (ty / kColGroups).  Its per-launch coverage counts
+// (<= kWarpBatch = 8 each) are packed 4 bits per row in one u64.
 //
 // Shape parameters (swept on MI355X: 4 waves, 128 x 32 outputs and a 29 KB
 // box buffer beat 2/8 waves, 16/24-row tiles and double-buffered asynchronous
@@ -226,13 +229,14 @@ constexpr int kWaves = GH_WARP_WAVES;
 constexpr int kTiledThreads = 64 * kWaves;
 constexpr int kOutW = 128;
 constexpr int kOutH = GH_WARP_OUT_H;
-constexpr int kRowsPerThread = kOutH / kWaves;
+constexpr int kColGroups = kOutW / 64;
+constexpr int kLaneRows = kOutH * kColGroups / kWaves;
 constexpr int kMaxTile = GH_WARP_MAX_TILE;
-static_assert(kOutH % kWaves == 0, "output tile rows must split evenly over the waves");
+static_assert(kWaves % kColGroups == 0 && kOutH % (kWaves / kColGroups) == 0, "waves tile the output evenly");
 constexpr int kLutVec = kLanczosLutStride / 4;
 constexpr int kLutRows = kLanczosLutIntervals + 1;
 static_assert(kLanczosLutStride % 4 == 0, "table intervals are whole float4s");
-static_assert(kRowsPerThread * 4 <= 32 && kWarpBatch <= 15, "coverage counts are packed 4 bits per row");
+static_assert(kLaneRows * 4 <= 64 && kWarpBatch <= 15, "coverage counts are packed 4 bits per row");
 
 // Six normalised tap weights for fractional offset f in [0, 1].  The LDS table
 // is stored as kLutVec planes of kLutRows float4s (plane v holds floats
@@ -347,116 +351,162 @@ __device__ __forceinline__ void stage_box(const TileFrame& fr, int W, float* til
   }
 }
 
-// Pixels of one frame with table weights and taps from the LDS box.  Pixels
-// outside the footprint evaluate a safe in-box position and write `fill`, so
-// the body is branch-free and the two pixels of a row interleave.  FULL: the
-// whole output tile lies inside the frame (no per-lane edge tests).  Returns
-// whether any written value is non-finite.
-template <bool FULL>
-__device__ __forceinline__ bool tiled_lds_rows(const TileFrame& fr, const float* tile, const float4* lut, int ox0,
-                                               int oy0, int tx, int ty, int W, int H, float fill,
-                                               unsigned (&cov)[2]) {
-  bool bad = false;
-  const int xa = ox0 + tx;
-  double px[2], py[2];
-#pragma unroll
-  for (int xx = 0; xx < 2; ++xx) {
-    px[xx] = fr.m0 * static_cast<double>(xa + 64 * xx);
-    py[xx] = fr.m3 * static_cast<double>(xa + 64 * xx);
-  }
+// Compile-time loop: f(std::integral_constant<int, I>) for I in [0, N).
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// Lanczos3 sum over a 6 x 6 tap block held in registers: rows R0 .. R0+5 and
+// columns C0 .. C0+5 of the window.
+// (Compile-time loops throughout: the window must stay in registers, and
+// SROA runs before loop unrolling.)
+template <int R0, int C0, int NR, int NC>
+__device__ __forceinline__ float window_taps(const float (&win)[NR][NC], const float (&wx)[6], const float (&wy)[6]) {
+  float acc = 0.0f;
+  static_for<6>([&](auto J) __attribute__((always_inline)) {
+    constexpr int j = decltype(J)::value;
+    float h = win[R0 + j][C0] * wx[0];
+    static_for<5>([&](auto I) __attribute__((always_inline)) {
+      constexpr int i = decltype(I)::value + 1;
+      h = fmaf(win[R0 + j][C0 + i], wx[i], h);
+    });
+    acc = j == 0 ? h * wy[0] : fmaf(h, wy[j], acc);
+  });
+  return acc;
+}
+
+// One vertical strip of kStripRows output pixels of column x, rows y_first ..,
+// taps from the LDS box through a register window.  Consecutive output rows of
+// a near-identity map read source windows one row apart (and at most one
+// column apart), so the lane keeps a kWinRows x kWinCols window of the box in
+// registers and loads one new row per pixel instead of 36 taps.  At each row a
+// wave-wide vote checks that every lane inside the footprint finds its taps at
+// window row r and the same column offset (0 or 1); otherwise that row reads
+// its taps from LDS directly.  Pixels outside the footprint evaluate a safe
+// in-box position and write `fill`.  Coverage counts go to bits 4 k of cov,
+// k = kbase + r.  Returns whether any written value is non-finite.
+constexpr int kStripRows = 8;
+constexpr int kWinRows = kStripRows + 5;
+constexpr int kWinCols = 7;
+static_assert(kLaneRows % kStripRows == 0, "a lane's rows split into whole strips");
+
+__device__ __forceinline__ bool strip_window_rows(const TileFrame& fr, const float* tile, const float4* lut, int x,
+                                                  int y_first, int W, int H, float fill, int kbase,
+                                                  unsigned long long& cov) {
+  const double xd = static_cast<double>(x);
+  const double px = fr.m0 * xd, py = fr.m3 * xd;
   const double xmax = static_cast<double>(W - 3), ymax = static_cast<double>(H - 3);
   const double safe_x = static_cast<double>(fr.bx0 + 2), safe_y = static_cast<double>(fr.by0 + 2);
-  // tile index of source (x0 - 2, y0 - 2) is y0 * bw + x0 + base
   const int base = -(fr.by0 + 2) * fr.bw - (fr.bx0 + 2);
-#pragma unroll 1
-  for (int rr = 0; rr < kRowsPerThread; ++rr) {
-    const int y = oy0 + ty + kWaves * rr;
-    if (!FULL && y >= H) break;
-    const double qx = fr.m1 * static_cast<double>(y);
-    const double qy = fr.m4 * static_cast<double>(y);
-    float* drow = fr.dst + static_cast<long long>(y) * W;
-#pragma unroll
-    for (int xx = 0; xx < 2; ++xx) {
-      const int x = xa + 64 * xx;
-      // ((m0 x + m1 y) + m2) exactly as the CPU path rounds it
-      const double sx = (px[xx] + qx) + fr.m2;
-      const double sy = (py[xx] + qy) + fr.m5;
-      const bool ok = sx >= 2.0 && sx < xmax && sy >= 2.0 && sy < ymax;
-      const double sxc = ok ? sx : safe_x, syc = ok ? sy : safe_y;
-      const double fx0 = floor(sxc), fy0 = floor(syc);
-      float wx[6], wy[6];
-      lut_weights(lut, static_cast<float>(sxc - fx0), wx);
-      lut_weights(lut, static_cast<float>(syc - fy0), wy);
-      const int idx = static_cast<int>(fy0) * fr.bw + static_cast<int>(fx0) + base;
-      float value = lanczos_taps(tile + idx, fr.bw, wx, wy);
-      value = ok ? value : fill;
-      if (FULL || x < W) {
-        cov[xx] += ok ? (1u << (4 * rr)) : 0u;
-        drow[x] = value;
-        bad |= !isfinite(value);
-      }
+  // Window origin (box-relative) from the strip's first pixel; the spare
+  // column goes on the side the columns drift to (sx moves by m1 per row).
+  const double sx0 = fmin(fmax((px + fr.m1 * static_cast<double>(y_first)) + fr.m2, -1.0e6), 1.0e6);
+  const double sy0 = fmin(fmax((py + fr.m4 * static_cast<double>(y_first)) + fr.m5, -1.0e6), 1.0e6);
+  const int lead = fr.m1 < 0.0 ? 3 : 2;
+  const int xb = min(max(static_cast<int>(floor(sx0)) - lead - fr.bx0, 0), fr.bw - kWinCols);
+  const int yb = min(max(static_cast<int>(floor(sy0)) - 2 - fr.by0, 0), fr.bh - kWinRows);
+  int woff = yb * fr.bw + xb;
+  float win[kWinRows][kWinCols];
+  static_for<5 * kWinCols>([&](auto E) __attribute__((always_inline)) {
+    constexpr int j = decltype(E)::value / kWinCols, c = decltype(E)::value % kWinCols;
+    win[j][c] = tile[woff + j * fr.bw + c];
+  });
+  bool bad = false;
+  static_for<kStripRows>([&](auto R) __attribute__((always_inline)) {
+    constexpr int r = decltype(R)::value;
+    // The row's offset passes through an opaque move so its loads stay in this
+    // step: hoisted, all kWinRows rows would be live at once (occupancy 3).
+    asm volatile("" : "+v"(woff));
+    static_for<kWinCols>([&](auto C) __attribute__((always_inline)) {
+      constexpr int c = decltype(C)::value;
+      win[r + 5][c] = tile[woff + (r + 5) * fr.bw + c];
+    });
+    const int y = y_first + r;
+    if (y >= H) return;
+    const double sx = (px + fr.m1 * static_cast<double>(y)) + fr.m2;
+    const double sy = (py + fr.m4 * static_cast<double>(y)) + fr.m5;
+    const bool ok = x < W && sx >= 2.0 && sx < xmax && sy >= 2.0 && sy < ymax;
+    const double sxc = ok ? sx : safe_x, syc = ok ? sy : safe_y;
+    const double fx0 = floor(sxc), fy0 = floor(syc);
+    float wx[6], wy[6];
+    lut_weights(lut, static_cast<float>(sxc - fx0), wx);
+    lut_weights(lut, static_cast<float>(syc - fy0), wy);
+    const int ix = static_cast<int>(fx0), iy = static_cast<int>(fy0);
+    const int o = ix - 2 - fr.bx0 - xb;
+    const bool row_ok = iy - 2 - fr.by0 - yb == r;
+    float value;
+    if (__all(!ok || (row_ok && o == 0))) {
+      value = window_taps<r, 0>(win, wx, wy);
+    } else if (__all(!ok || (row_ok && o == 1))) {
+      value = window_taps<r, 1>(win, wx, wy);
+    } else {
+      value = lanczos_taps(tile + (iy * fr.bw + ix + base), fr.bw, wx, wy);
     }
-  }
+    value = ok ? value : fill;
+    if (x < W) {
+      cov += ok ? (1ull << (4 * (kbase + r))) : 0ull;
+      fr.dst[static_cast<long long>(y) * W + x] = value;
+      bad |= !isfinite(value);
+    }
+  });
   return bad;
 }
 
 // Pixels of one frame with table weights and taps read straight from the
 // frame (source box too large for LDS).
-__device__ __forceinline__ bool tiled_global_rows(const TileFrame& fr, const float4* lut, int ox0, int oy0, int tx,
-                                                  int ty, int W, int H, float fill, unsigned (&cov)[2]) {
+__device__ __forceinline__ bool column_global_rows(const TileFrame& fr, const float4* lut, int x, int y_first, int W,
+                                                   int H, float fill, unsigned long long& cov) {
   bool bad = false;
   const double xmax = static_cast<double>(W - 3), ymax = static_cast<double>(H - 3);
+  if (x >= W) return false;
 #pragma unroll 1
-  for (int rr = 0; rr < kRowsPerThread; ++rr) {
-#pragma unroll
-    for (int xx = 0; xx < 2; ++xx) {
-      const int x = ox0 + tx + 64 * xx;
-      const int y = oy0 + ty + kWaves * rr;
-      if (x < W && y < H) {
-        const double sx = (fr.m0 * static_cast<double>(x) + fr.m1 * static_cast<double>(y)) + fr.m2;
-        const double sy = (fr.m3 * static_cast<double>(x) + fr.m4 * static_cast<double>(y)) + fr.m5;
-        float value = fill;
-        if (sx >= 2.0 && sx < xmax && sy >= 2.0 && sy < ymax) {
-          const double fx0 = floor(sx), fy0 = floor(sy);
-          const int x0 = static_cast<int>(fx0), y0 = static_cast<int>(fy0);
-          float wx[6], wy[6];
-          lut_weights(lut, static_cast<float>(sx - fx0), wx);
-          lut_weights(lut, static_cast<float>(sy - fy0), wy);
-          value = lanczos_taps(fr.src + static_cast<long long>(y0 - 2) * W + (x0 - 2), W, wx, wy);
-          cov[xx] += 1u << (4 * rr);
-        }
-        fr.dst[static_cast<long long>(y) * W + x] = value;
-        bad |= !isfinite(value);
-      }
+  for (int k = 0; k < kLaneRows; ++k) {
+    const int y = y_first + k;
+    if (y >= H) break;
+    const double sx = (fr.m0 * static_cast<double>(x) + fr.m1 * static_cast<double>(y)) + fr.m2;
+    const double sy = (fr.m3 * static_cast<double>(x) + fr.m4 * static_cast<double>(y)) + fr.m5;
+    float value = fill;
+    if (sx >= 2.0 && sx < xmax && sy >= 2.0 && sy < ymax) {
+      const double fx0 = floor(sx), fy0 = floor(sy);
+      const int x0 = static_cast<int>(fx0), y0 = static_cast<int>(fy0);
+      float wx[6], wy[6];
+      lut_weights(lut, static_cast<float>(sx - fx0), wx);
+      lut_weights(lut, static_cast<float>(sy - fy0), wy);
+      value = lanczos_taps(fr.src + static_cast<long long>(y0 - 2) * W + (x0 - 2), W, wx, wy);
+      cov += 1ull << (4 * k);
     }
+    fr.dst[static_cast<long long>(y) * W + x] = value;
+    bad |= !isfinite(value);
   }
   return bad;
 }
 
 // Pixels of one frame through lanczos_direct (non-finite sources, clamp).
-__device__ __forceinline__ bool tiled_direct_rows(const TileFrame& fr, const WarpParams& p, int ox0, int oy0, int tx,
-                                                  int ty, unsigned (&cov)[2]) {
+__device__ __forceinline__ bool column_direct_rows(const TileFrame& fr, const WarpParams& p, int x, int y_first,
+                                                   unsigned long long& cov) {
   bool bad = false;
   const float thr = p.skip_nonfinite ? 1.0e-6f : 1.0e-8f;
   const double m[6] = {fr.m0, fr.m1, fr.m2, fr.m3, fr.m4, fr.m5};
+  if (x >= p.width) return false;
 #pragma unroll 1
-  for (int rr = 0; rr < kRowsPerThread; ++rr) {
-#pragma unroll 1
-    for (int xx = 0; xx < 2; ++xx) {
-      const int x = ox0 + tx + 64 * xx;
-      const int y = oy0 + ty + kWaves * rr;
-      if (x >= p.width || y >= p.height) continue;
-      double sx, sy;
-      source_coords(m, 1, x, y, sx, sy);
-      float value = p.fill;
-      if (lanczos_footprint(sx, sy, p.width, p.height)) {
-        bool covered;
-        value = lanczos_direct(fr.src, p.width, sx, sy, p, thr, covered);
-        if (covered) cov[xx] += 1u << (4 * rr);
-      }
-      fr.dst[static_cast<long long>(y) * p.width + x] = value;
-      bad |= !isfinite(value);
+  for (int k = 0; k < kLaneRows; ++k) {
+    const int y = y_first + k;
+    if (y >= p.height) break;
+    double sx, sy;
+    source_coords(m, 1, x, y, sx, sy);
+    float value = p.fill;
+    if (lanczos_footprint(sx, sy, p.width, p.height)) {
+      bool covered;
+      value = lanczos_direct(fr.src, p.width, sx, sy, p, thr, covered);
+      if (covered) cov += 1ull << (4 * k);
     }
+    fr.dst[static_cast<long long>(y) * p.width + x] = value;
+    bad |= !isfinite(value);
   }
   return bad;
 }
@@ -481,40 +531,38 @@ __global__ __launch_bounds__(kTiledThreads) void warp_lanczos3_tiled_kernel(Warp
 
   const int tx = threadIdx.x & 63;
   const int ty = threadIdx.x >> 6;
+  const int x = ox0 + 64 * (ty % kColGroups) + tx;
+  const int y_first = oy0 + kLaneRows * (ty / kColGroups);
   const int W = p.width, H = p.height;
   const bool clamp_on = p.skip_nonfinite && p.clamp >= 0.0f;
   const int ox1 = min(ox0 + kOutW, W) - 1, oy1 = min(oy0 + kOutH, H) - 1;
-  const bool full = ox0 + kOutW <= W && oy0 + kOutH <= H;
-  unsigned cov[2] = {0u, 0u};
+  unsigned long long cov = 0;
 
   for (int f = 0; f < count; ++f) {
     const TileFrame cur = tile_frame(b, f, clamp_on, ox0, oy0, ox1, oy1, W, H);
     __syncthreads();  // previous frame's taps consumed (first pass: table stored)
     if (cur.use_tile) stage_box(cur, W, tile);
     __syncthreads();
-    bool bad;
+    bool bad = false;
     if (cur.use_tile) {
-      bad = full ? tiled_lds_rows<true>(cur, tile, lut, ox0, oy0, tx, ty, W, H, p.fill, cov)
-                 : tiled_lds_rows<false>(cur, tile, lut, ox0, oy0, tx, ty, W, H, p.fill, cov);
+#pragma unroll 1
+      for (int s = 0; s < kLaneRows / kStripRows; ++s) {
+        bad |= strip_window_rows(cur, tile, lut, x, y_first + s * kStripRows, W, H, p.fill, s * kStripRows, cov);
+      }
     } else if (cur.fast) {
-      bad = tiled_global_rows(cur, lut, ox0, oy0, tx, ty, W, H, p.fill, cov);
+      bad = column_global_rows(cur, lut, x, y_first, W, H, p.fill, cov);
     } else {
-      bad = tiled_direct_rows(cur, p, ox0, oy0, tx, ty, cov);
+      bad = column_direct_rows(cur, p, x, y_first, cov);
     }
     if (__any(bad) && tx == 0) atomicOr(b.nonfinite[f], 1u);
   }
-  if (p.coverage) {
+  if (p.coverage && x < W) {
 #pragma unroll 1
-    for (int rr = 0; rr < kRowsPerThread; ++rr) {
-#pragma unroll
-      for (int xx = 0; xx < 2; ++xx) {
-        const int x = ox0 + tx + 64 * xx;
-        const int y = oy0 + ty + kWaves * rr;
-        if (x < W && y < H) {
-          const long long o = static_cast<long long>(y) * W + x;
-          p.coverage[o] = p.coverage[o] + static_cast<float>((cov[xx] >> (4 * rr)) & 15u);
-        }
-      }
+    for (int k = 0; k < kLaneRows; ++k) {
+      const int y = y_first + k;
+      if (y >= H) break;
+      const long long o = static_cast<long long>(y) * W + x;
+      p.coverage[o] = p.coverage[o] + static_cast<float>((cov >> (4 * k)) & 15ull);
     }
   }
 }
