@@ -35,6 +35,10 @@ GH_REJECT_WINSORIZED_SIGMA = 1
 GH_COUNT_F32 = 0
 GH_COUNT_U16 = 1
 
+GH_MODEL_TRANSLATION = 0
+GH_MODEL_SIMILARITY = 1
+GH_MODEL_AFFINE = 2
+
 
 class CalibPolicy(ctypes.Structure):
     _fields_ = [
@@ -88,6 +92,22 @@ class FrameStatistics(ctypes.Structure):
     ]
 
 
+class StarRecord(ctypes.Structure):
+    _fields_ = [
+        ("x", c_double),
+        ("y", c_double),
+        ("flux", c_double),
+        ("peak", c_double),
+        ("fwhm_px", c_double),
+        ("eccentricity", c_double),
+        ("snr", c_double),
+        ("sharpness", c_double),
+        ("npix", c_int64),
+        ("has_fwhm", c_int32),
+        ("has_eccentricity", c_int32),
+    ]
+
+
 # name -> (argtypes); every function returns int
 _SIGNATURES = {
     "gh_last_error": [c_char_p, c_size_t],
@@ -118,6 +138,11 @@ _SIGNATURES = {
     "gh_frame_stats": [c_void_p, c_int64, c_int32, POINTER(FrameStatistics)],
     "gh_calibration_stats": [c_void_p, c_int64, POINTER(c_double), POINTER(c_double), POINTER(c_int64)],
     "gh_synchronize": [c_void_p],
+    "gh_detect_stars": [c_void_p, c_int64, c_double, c_int64, c_double, c_double, c_int32, c_double, c_int64,
+                        c_int64, POINTER(StarRecord), c_int64, POINTER(c_int64), POINTER(c_double)],
+    "gh_estimate_star_transform": [POINTER(c_double), c_int64, POINTER(c_double), c_int64, c_int32, c_int64,
+                                   c_double, c_int64, POINTER(c_double), POINTER(c_int64), POINTER(c_int64),
+                                   POINTER(c_double), POINTER(c_int32)],
 }
 
 EXPORTED_SYMBOLS = tuple(_SIGNATURES)

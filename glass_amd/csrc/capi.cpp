@@ -18,6 +18,7 @@
 #include "../../include/glass_hip.h"
 #include "common.hpp"
 #include "kernels.hpp"
+#include "stars.hpp"
 
 using gh::check;
 
@@ -103,6 +104,8 @@ struct gh_stack {
   DeviceBuffer<unsigned> out_dq;
   DeviceBuffer<gh::FrameMoments> moments;
   DeviceBuffer<unsigned> radix;
+  DeviceBuffer<unsigned> star_cand;         // [1 + capacity] count, candidate pixels
+  DeviceBuffer<float> star_windows;         // gathered candidate windows
   DeviceBuffer<unsigned> warp_flags;        // [kWarpBatch] source-flag snapshot
   DeviceBuffer<float> lanczos_lut;          // normalised Lanczos3 weight cubics
 
@@ -184,6 +187,8 @@ struct gh_stack {
     radix.release();
     warp_flags.release();
     lanczos_lut.release();
+    star_cand.release();
+    star_windows.release();
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -713,6 +718,145 @@ int gh_calibration_stats(gh_stack* s, int64_t index, double* sum, double* sum_sq
 
 int gh_synchronize(gh_stack* s) {
   return guarded([&] { check(hipStreamSynchronize(s->stream), "synchronize"); });
+}
+
+int gh_detect_stars(gh_stack* s, int64_t index, double threshold_sigma, int64_t max_stars, double background,
+                    double noise, int32_t window_radius, double min_separation_px, int64_t origin_x,
+                    int64_t origin_y, gh_star* out, int64_t capacity, int64_t* count, double* background_out) {
+  return guarded([&] {
+    s->require_loaded(index, "star detection");
+    if (!count || (capacity > 0 && !out)) throw gh::ValueError("output pointer is NULL");
+    if (window_radius > gh::kMaxStarRadius) throw gh::ValueError("window_radius must be at most 8");
+    const int radius = std::max(1, static_cast<int>(window_radius));
+    const float* frame = s->slot[index];
+    const int h = static_cast<int>(s->height), w = static_cast<int>(s->width);
+    *count = 0;
+    // robust_background (star_detect.py:28-42): numpy float32 medians.
+    const unsigned blocks = gh::frame_moments_blocks(s->npix);
+    s->moments.ensure(blocks + 1, "hipMalloc(frame moments)");
+    gh::launch_frame_moments(frame, s->npix, s->moments.ptr + 1, blocks, s->moments.ptr, s->stream);
+    gh::FrameMoments m{};
+    check(hipMemcpyAsync(&m, s->moments.ptr, sizeof(m), hipMemcpyDeviceToHost, s->stream), "hipMemcpy(moments)");
+    check(hipStreamSynchronize(s->stream), "star background");
+    const uint64_t n = m.finite;
+    double median = 0.0, mad = 0.0, sigma_bg = 1.0;
+    if (n > 0) {
+      s->radix.ensure(256 + 4, "hipMalloc(radix select)");
+      auto select = [&](int abs_mode, float center, uint64_t rank) -> float {
+        gh::launch_radix_select(frame, s->npix, abs_mode, center, rank, s->radix.ptr, s->radix.ptr + 256, s->stream);
+        unsigned key = 0;
+        check(hipMemcpyAsync(&key, s->radix.ptr + 256, sizeof(unsigned), hipMemcpyDeviceToHost, s->stream),
+              "hipMemcpy(radix key)");
+        check(hipStreamSynchronize(s->stream), "radix select");
+        const unsigned bits = (key & 0x80000000u) ? (key & 0x7fffffffu) : ~key;
+        float v;
+        std::memcpy(&v, &bits, sizeof(v));
+        return v;
+      };
+      // np.median of float32: the middle pair is averaged in float32.
+      auto median32 = [&](int abs_mode, float center) {
+        const float hi = select(abs_mode, center, n / 2);
+        if (n % 2) return hi;
+        const float lo = select(abs_mode, center, n / 2 - 1);
+        return (lo + hi) / 2.0f;
+      };
+      const float med = median32(0, 0.0f);
+      median = med;
+      mad = median32(1, med);
+      sigma_bg = 1.4826 * mad;
+      if (sigma_bg <= 0.0) {
+        // np.std of the finite float32 samples, in numpy's float32 pairwise order.
+        std::vector<float> host(static_cast<size_t>(s->npix));
+        check(hipMemcpy(host.data(), frame, s->frame_bytes(), hipMemcpyDeviceToHost), "hipMemcpy(frame)");
+        size_t k = 0;
+        for (float v : host)
+          if (std::isfinite(v)) host[k++] = v;
+        sigma_bg = gh::numpy_std_f32(host.data(), k);
+      }
+      if (sigma_bg <= 0.0) sigma_bg = 1.0;
+    }
+    if (background_out) {
+      background_out[0] = median;
+      background_out[1] = mad;
+      background_out[2] = sigma_bg;
+    }
+    const double bg = std::isnan(background) ? median : background;
+    const double sigma = std::isnan(noise) ? sigma_bg : noise;
+    const double threshold = bg + threshold_sigma * std::max(sigma, 1.0e-6);
+    if (h < 3 || w < 3) return;
+    // Candidates (star_detect.py:148-160); the FP64 threshold meets the float32
+    // image as a float32 (numpy weak-scalar comparison).
+    unsigned cap = 1u << 16;
+    unsigned found = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+      s->star_cand.ensure(static_cast<size_t>(cap) + 1, "hipMalloc(star candidates)");
+      gh::launch_fill(reinterpret_cast<float*>(s->star_cand.ptr), 1, 0.0f, s->stream);
+      gh::launch_star_candidates(frame, h, w, static_cast<float>(threshold), s->star_cand.ptr + 1, cap,
+                                 s->star_cand.ptr, s->stream);
+      check(hipMemcpyAsync(&found, s->star_cand.ptr, sizeof(unsigned), hipMemcpyDeviceToHost, s->stream),
+            "hipMemcpy(candidate count)");
+      check(hipStreamSynchronize(s->stream), "star candidates");
+      if (found <= cap) break;
+      cap = found;
+    }
+    std::vector<unsigned> cand(found);
+    if (found) {
+      check(hipMemcpy(cand.data(), s->star_cand.ptr + 1, found * sizeof(unsigned), hipMemcpyDeviceToHost),
+            "hipMemcpy(candidates)");
+    }
+    std::sort(cand.begin(), cand.end());  // np.nonzero order: row-major
+    const int side = 2 * radius + 1, cells = side * side;
+    std::vector<float> windows(static_cast<size_t>(found) * cells);
+    if (found) {
+      check(hipMemcpy(s->star_cand.ptr + 1, cand.data(), found * sizeof(unsigned), hipMemcpyHostToDevice),
+            "hipMemcpy(sorted candidates)");
+      s->star_windows.ensure(windows.size(), "hipMalloc(star windows)");
+      gh::launch_gather_windows(frame, h, w, s->star_cand.ptr + 1, found, radius, s->star_windows.ptr, s->stream);
+      check(hipMemcpyAsync(windows.data(), s->star_windows.ptr, windows.size() * sizeof(float),
+                           hipMemcpyDeviceToHost, s->stream),
+            "hipMemcpy(star windows)");
+      check(hipStreamSynchronize(s->stream), "star windows");
+    }
+    std::vector<gh::Star> stars;
+    stars.reserve(found);
+    for (unsigned k = 0; k < found; ++k) {
+      gh::CandidatePatch c;
+      c.x = static_cast<int>(cand[k] % static_cast<unsigned>(w));
+      c.y = static_cast<int>(cand[k] / static_cast<unsigned>(w));
+      c.values = windows.data() + static_cast<size_t>(k) * cells;
+      gh::Star st;
+      if (gh::measure_candidate(c, h, w, radius, bg, sigma, origin_x, origin_y, st)) stars.push_back(st);
+    }
+    stars = gh::finish_star_list(std::move(stars), min_separation_px, max_stars);
+    const int64_t n_out = std::min<int64_t>(static_cast<int64_t>(stars.size()), std::max<int64_t>(capacity, 0));
+    for (int64_t k = 0; k < n_out; ++k) {
+      const gh::Star& st = stars[k];
+      out[k] = gh_star{st.x, st.y, st.flux, st.peak, st.fwhm_px, st.eccentricity, st.snr, st.sharpness, st.npix,
+                       st.has_fwhm ? 1 : 0, st.has_eccentricity ? 1 : 0};
+    }
+    *count = static_cast<int64_t>(stars.size());
+  });
+}
+
+int gh_estimate_star_transform(const double* ref_xy, int64_t n_ref, const double* mov_xy, int64_t n_mov,
+                               int32_t model, int64_t min_inliers, double tolerance_px, int64_t max_stars,
+                               double* matrix, int64_t* pairs, int64_t* n_pairs, double* rms, int32_t* ok) {
+  return guarded([&] {
+    if ((n_ref > 0 && !ref_xy) || (n_mov > 0 && !mov_xy) || !matrix || !n_pairs || !rms || !ok)
+      throw gh::ValueError("argument pointer is NULL");
+    const gh::TransformResult r =
+        gh::estimate_star_transform(ref_xy, n_ref, mov_xy, n_mov, model, min_inliers, tolerance_px, max_stars);
+    std::copy(r.matrix, r.matrix + 9, matrix);
+    *n_pairs = static_cast<int64_t>(r.pairs.size());
+    if (pairs) {
+      for (size_t k = 0; k < r.pairs.size(); ++k) {
+        pairs[2 * k] = r.pairs[k].first;
+        pairs[2 * k + 1] = r.pairs[k].second;
+      }
+    }
+    *rms = std::isfinite(r.rms) ? r.rms : NAN;
+    *ok = r.ok ? 1 : 0;
+  });
 }
 
 }  // extern "C"

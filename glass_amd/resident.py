@@ -378,6 +378,18 @@ class ResidentCalibratedStack:
         N.call("gh_warp_coverage_map", self._h, ctypes.c_void_p(out.ctypes.data))
         return out
 
+    # ---------------------------------------------------------------- stars
+    def detect_stars(self, index: int, threshold_sigma: float = 5.0, max_stars: int = 500, *, background=None,
+                     noise=None, window_radius: int = 4, min_separation_px: float = 2.0, origin_x: int = 0,
+                     origin_y: int = 0) -> list:
+        """detect_stars (src/glass/cpu/star_detect.py:121-168) on resident frame
+        `index`, no host copy of the frame."""
+        from .registration import detect_resident
+
+        stars, _bg = detect_resident(self._h, index, threshold_sigma, max_stars, background, noise, window_radius,
+                                     min_separation_px, origin_x, origin_y)
+        return stars
+
     # ---------------------------------------------------------------- stats
     def frame_global_stats(self, index: int, exact_median: bool = False) -> dict:
         st = N.FrameStatistics()

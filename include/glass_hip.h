@@ -194,6 +194,40 @@ int gh_calibration_stats(gh_stack* stack, int64_t index, double* sum, double* su
 /* Wait for all work queued on the stack's stream. */
 int gh_synchronize(gh_stack* stack);
 
+/* ------------------------------------------------ star detection / matching */
+#define GH_MODEL_TRANSLATION 0
+#define GH_MODEL_SIMILARITY 1
+#define GH_MODEL_AFFINE 2
+
+/* One detected star (src/glass/cpu/star_detect.py:8-18 Star). */
+typedef struct gh_star {
+  double x, y, flux, peak, fwhm_px, eccentricity, snr, sharpness;
+  int64_t npix;
+  int32_t has_fwhm, has_eccentricity; /* 0: the reference field is None */
+} gh_star;
+
+/* detect_stars(data, threshold_sigma, max_stars, background, noise,
+ * window_radius, min_separation_px, origin_x, origin_y)
+ * (src/glass/cpu/star_detect.py:121-168, robust_background :28-42) on resident
+ * frame `index`: float32 median / MAD background (NaN `background` / `noise`
+ * selects the robust estimate), GPU 3x3 local-maximum candidates, FP64 window
+ * moments.  Up to `capacity` stars go to `out` (brightest first); *count gets
+ * the list length.  background_out (nullable) receives {median, mad, noise}.
+ * window_radius must be in [1, 8]. */
+int gh_detect_stars(gh_stack* stack, int64_t index, double threshold_sigma, int64_t max_stars, double background,
+                    double noise, int32_t window_radius, double min_separation_px, int64_t origin_x,
+                    int64_t origin_y, gh_star* out, int64_t capacity, int64_t* count, double* background_out);
+
+/* estimate_star_transform(reference, moving, transform_model, min_inliers,
+ * tolerance_px, max_stars) (src/glass/cpu/registration.py:509-569) on star
+ * positions ref_xy[2*n_ref], mov_xy[2*n_mov] (brightest first), FP64, host
+ * only.  model: GH_MODEL_*.  matrix[9] (row-major, moving -> reference);
+ * pairs[2 * max_stars] receives (moving, reference) index pairs, *n_pairs the
+ * inlier count; *rms the pair RMS (NaN when none); *ok = inliers >= min_inliers. */
+int gh_estimate_star_transform(const double* ref_xy, int64_t n_ref, const double* mov_xy, int64_t n_mov,
+                               int32_t model, int64_t min_inliers, double tolerance_px, int64_t max_stars,
+                               double* matrix, int64_t* pairs, int64_t* n_pairs, double* rms, int32_t* ok);
+
 #ifdef __cplusplus
 }
 #endif

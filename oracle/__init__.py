@@ -26,6 +26,13 @@ Pinning status (what checks each function; see tests/test_oracle_*.py):
                                 (integer shifts, constant images, linear ramps)
   scan_quality_stats         -- closed forms (np.median / exact sort)
   resident_dq_map            -- explicit rule table cases
+  registration.detect_stars  -- reference literal cases (elongated star,
+                                star fields), closed forms (centroid of a
+                                centred star, float32 median pair average)
+  registration.estimate_star_transform
+                             -- reference literal cases (translation with
+                                outlier, similarity), exact affine recovery;
+                                the lstsq / svd last bits are parity unpinned
 """
 
 from .calibration import calibrate_light, decode_fits_u16be_bzero, decode_u16le

@@ -1,0 +1,324 @@
+"""Star detection and star-list transform estimation oracle (test
+infrastructure; see oracle/__init__.py).
+
+Restates the reference CPU path that produces the warp matrices:
+
+* ``detect_stars`` -- src/glass/cpu/star_detect.py:121-168 with
+  ``robust_background`` (:28-42), ``_measure_candidate`` (:45-102) and
+  ``_dedupe_stars`` (:105-118).  numpy semantics the product must follow:
+  float32 medians average the middle pair in float32, the FP64 threshold is
+  compared against the float32 image after rounding to float32 (weak Python
+  scalar), candidate sums are FP64 over the contiguous patch in numpy's
+  pairwise order, sorts are stable.
+* ``estimate_star_transform`` -- src/glass/cpu/registration.py:509-569 with
+  ``_translation_candidates`` (:219-235), ``_asterism_candidates`` (:346-371),
+  ``_triangle_descriptors`` (:316-343), ``_greedy_pairs`` (:177-205) and the
+  model fits (:101-171) for the translation, similarity and affine models.
+
+The affine fit is ``np.linalg.lstsq`` (LAPACK gelsd) and the similarity fit a
+2x2 ``np.linalg.svd``: their last bits (and numpy's BLAS matrix products in
+``_apply_matrix``) are not pinned by any reference test, so matrices of those
+models are compared with a tolerance ("parity unpinned" at the last bit).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from itertools import combinations, permutations
+
+import numpy as np
+
+
+@dataclass
+class Star:
+    x: float
+    y: float
+    flux: float
+    peak: float = 0.0
+    fwhm_px: float | None = None
+    eccentricity: float | None = None
+    snr: float | None = None
+    sharpness: float | None = None
+    npix: int = 0
+
+
+@dataclass
+class Registration:
+    transform_model: str
+    matrix: list
+    matched_stars: int
+    inliers: int
+    rms_px: float
+    status: str
+    pairs: list = field(default_factory=list)
+
+
+def robust_background(image):
+    """star_detect.py:28-42: float32 median / MAD of the finite samples,
+    noise = 1.4826 MAD, falling back to the float32 std, then 1."""
+    finite = image[np.isfinite(image)]
+    if finite.size == 0:
+        return 0.0, 0.0, 0.0, 1.0
+    median = float(np.median(finite))
+    rms = float(np.std(finite))
+    mad = float(np.median(np.abs(finite - median)))
+    noise = float(1.4826 * mad)
+    if noise <= 0.0:
+        noise = rms
+    if noise <= 0.0:
+        noise = 1.0
+    return median, rms, mad, noise
+
+
+def measure_candidate(image, x, y, background, noise, radius, origin_x=0, origin_y=0):
+    """star_detect.py:45-102: FP64 flux, centroid and second moments of the
+    background-subtracted, zero-clipped window around (x, y)."""
+    h, w = image.shape
+    y0, y1 = max(0, y - radius), min(h, y + radius + 1)
+    x0, x1 = max(0, x - radius), min(w, x + radius + 1)
+    patch = image[y0:y1, x0:x1]
+    ok = np.isfinite(patch)
+    if not ok.any():
+        return None
+    wts = np.where(ok, np.maximum(patch - np.float32(background), 0.0), 0.0)
+    flux = float(np.sum(wts, dtype=np.float64))
+    if flux <= 0.0:
+        return None
+    yy, xx = np.mgrid[y0:y1, x0:x1]
+    cx = float(np.sum(xx * wts, dtype=np.float64) / flux)
+    cy = float(np.sum(yy * wts, dtype=np.float64) / flux)
+    dx = xx - cx
+    dy = yy - cy
+    mxx = float(np.sum(dx * dx * wts, dtype=np.float64) / flux)
+    myy = float(np.sum(dy * dy * wts, dtype=np.float64) / flux)
+    mxy = float(np.sum(dx * dy * wts, dtype=np.float64) / flux)
+    trace = max(mxx + myy, 0.0)
+    det_term = max((mxx - myy) * (mxx - myy) + 4.0 * mxy * mxy, 0.0)
+    root = float(np.sqrt(det_term))
+    major = max((trace + root) / 2.0, 0.0)
+    minor = max((trace - root) / 2.0, 0.0)
+    sigma = float(np.sqrt(max((major + minor) / 2.0, 0.0)))
+    fwhm = 2.354820045 * sigma if sigma > 0.0 else None
+    ecc = float(np.sqrt(max(0.0, 1.0 - minor / major))) if major > 0.0 else None
+    peak = float(max(float(image[y, x]) - background, 0.0))
+    npix = int(np.count_nonzero(wts > 0.0))
+    snr = flux / max(float(noise) * np.sqrt(max(npix, 1)), 1.0e-6)
+    return Star(x=cx + float(origin_x), y=cy + float(origin_y), flux=flux, peak=peak,
+                fwhm_px=None if fwhm is None else float(fwhm), eccentricity=ecc, snr=float(snr),
+                sharpness=float(peak / max(flux, 1.0e-6)), npix=npix)
+
+
+def dedupe_stars(stars, min_separation_px):
+    """star_detect.py:105-118: brightest first (stable), drop stars closer
+    than the separation to a kept one."""
+    if min_separation_px <= 0.0 or len(stars) <= 1:
+        return stars
+    kept = []
+    limit = float(min_separation_px * min_separation_px)
+    for s in sorted(stars, key=lambda item: item.flux, reverse=True):
+        if all((s.x - k.x) * (s.x - k.x) + (s.y - k.y) * (s.y - k.y) >= limit for k in kept):
+            kept.append(s)
+    return kept
+
+
+def detect_stars(data, threshold_sigma=5.0, max_stars=500, *, background=None, noise=None, window_radius=4,
+                 min_separation_px=2.0, origin_x=0, origin_y=0):
+    """star_detect.py:121-168."""
+    image = np.asarray(data, dtype=np.float32)
+    median, _rms, _mad, sigma_bg = robust_background(image)
+    median = median if background is None else float(background)
+    sigma = sigma_bg if noise is None else float(noise)
+    threshold = median + threshold_sigma * max(sigma, 1.0e-6)
+    h, w = image.shape
+    if h < 3 or w < 3:
+        return []
+    core = image[1:-1, 1:-1]
+    mask = np.isfinite(core) & (core > threshold)
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            if dx or dy:
+                mask &= core >= image[1 + dy:h - 1 + dy, 1 + dx:w - 1 + dx]
+    ys, xs = np.nonzero(mask)
+    radius = max(1, int(window_radius))
+    stars = []
+    for ly, lx in zip(ys, xs):
+        s = measure_candidate(image, int(lx + 1), int(ly + 1), median, sigma, radius, int(origin_x), int(origin_y))
+        if s is not None:
+            stars.append(s)
+    stars = dedupe_stars(stars, float(min_separation_px))
+    stars.sort(key=lambda s: s.flux, reverse=True)
+    return stars[:max_stars]
+
+
+# ------------------------------------------------------------ registration
+def _points(stars, max_stars):
+    return np.asarray([(s.x, s.y) for s in stars[: max(0, int(max_stars))]], dtype=np.float64)
+
+
+def _translation(dx, dy):
+    return np.asarray([[1.0, 0.0, dx], [0.0, 1.0, dy], [0.0, 0.0, 1.0]], dtype=np.float64)
+
+
+def apply_matrix(points, matrix):
+    """registration.py:49-54."""
+    hom = np.hstack([points, np.ones((points.shape[0], 1), dtype=np.float64)])
+    out = hom @ matrix.T
+    wcol = out[:, 2:3]
+    return np.divide(out[:, :2], wcol, out=out[:, :2].copy(), where=np.abs(wcol) > 1.0e-12)
+
+
+def fit_translation(moving, reference):
+    """registration.py:96-98."""
+    d = np.median(reference - moving, axis=0)
+    return _translation(d[0], d[1])
+
+
+def fit_affine(moving, reference):
+    """registration.py:101-115 (FP64 least squares)."""
+    if len(moving) < 3:
+        raise ValueError("affine registration requires at least three matched stars")
+    a = np.zeros((2 * len(moving), 6), dtype=np.float64)
+    a[0::2, 0:2] = moving
+    a[0::2, 2] = 1.0
+    a[1::2, 3:5] = moving
+    a[1::2, 5] = 1.0
+    rhs = np.asarray(reference, dtype=np.float64).reshape(-1)
+    p = np.linalg.lstsq(a, rhs, rcond=None)[0]
+    return np.asarray([[p[0], p[1], p[2]], [p[3], p[4], p[5]], [0.0, 0.0, 1.0]], dtype=np.float64)
+
+
+def fit_similarity(moving, reference):
+    """registration.py:136-158 (orthogonal Procrustes with scale)."""
+    if len(moving) < 2:
+        raise ValueError("similarity registration requires at least two matched stars")
+    mm, rm = np.mean(moving, axis=0), np.mean(reference, axis=0)
+    mc, rc = moving - mm, reference - rm
+    var = float(np.sum(mc * mc))
+    if var <= 1.0e-12:
+        raise ValueError("degenerate moving stars for similarity registration")
+    u, sv, vt = np.linalg.svd(mc.T @ rc)
+    rot = vt.T @ u.T
+    if np.linalg.det(rot) < 0:
+        vt[-1, :] *= -1
+        rot = vt.T @ u.T
+    lin = float(np.sum(sv) / var) * rot
+    t = rm - mm @ lin.T
+    return np.asarray([[lin[0, 0], lin[0, 1], t[0]], [lin[1, 0], lin[1, 1], t[1]], [0.0, 0.0, 1.0]], dtype=np.float64)
+
+
+def fit_model(model, moving, reference):
+    if model == "translation":
+        return fit_translation(moving, reference)
+    if model == "similarity":
+        return fit_similarity(moving, reference)
+    if model == "affine":
+        return fit_affine(moving, reference)
+    raise ValueError(f"unsupported star registration model: {model}")
+
+
+def greedy_pairs(reference, transformed, tolerance_px):
+    """registration.py:177-205: candidate pairs within the tolerance, nearest
+    first (stable over the row-major candidate order), each star used once."""
+    if len(reference) == 0 or len(transformed) == 0:
+        return [], float("inf")
+    d = transformed[:, None, :] - reference[None, :, :]
+    dist = np.sqrt(np.sum(d * d, axis=2))
+    cand = sorted(((int(i), int(j)) for i, j in np.argwhere(dist <= tolerance_px)), key=lambda p: float(dist[p]))
+    used_m, used_r, pairs, res = set(), set(), [], []
+    for i, j in cand:
+        if i in used_m or j in used_r:
+            continue
+        used_m.add(i)
+        used_r.add(j)
+        pairs.append((i, j))
+        res.append(float(dist[i, j]))
+    rms = float(np.sqrt(np.mean(np.square(res)))) if res else float("inf")
+    return pairs, rms
+
+
+def translation_candidates(reference, moving, brightest):
+    """registration.py:219-235: one candidate per distinct rounded offset."""
+    out, seen = [], set()
+    for rp in reference[: min(len(reference), brightest)]:
+        for mp in moving[: min(len(moving), brightest)]:
+            d = rp - mp
+            key = (int(round(d[0])), int(round(d[1])))
+            if key not in seen:
+                seen.add(key)
+                out.append(_translation(d[0], d[1]))
+    return out
+
+
+def triangle_descriptors(points, max_points, max_triangles):
+    """registration.py:316-343: (short/longest, middle/longest) of every
+    non-degenerate triangle of the first max_points, largest area first."""
+    pts = points[: min(len(points), max_points)]
+    tris = []
+    for i, j, k in combinations(range(len(pts)), 3):
+        t = pts[[i, j, k]]
+        ea, eb = t[1] - t[0], t[2] - t[0]
+        area = abs(float(ea[0] * eb[1] - ea[1] * eb[0])) * 0.5
+        if area < 2.0:
+            continue
+        sides = np.asarray([np.linalg.norm(t[0] - t[1]), np.linalg.norm(t[1] - t[2]), np.linalg.norm(t[2] - t[0])])
+        longest = float(np.max(sides))
+        if longest <= 1.0e-6:
+            continue
+        o = np.sort(sides)
+        tris.append((np.asarray([o[0] / longest, o[1] / longest]), (i, j, k), area))
+    tris.sort(key=lambda item: item[2], reverse=True)
+    return [(d, idx) for d, idx, _a in tris[:max_triangles]]
+
+
+def asterism_candidates(reference, moving, model, tolerance, max_triangles):
+    """registration.py:346-371."""
+    ref_t = triangle_descriptors(reference, 30, max_triangles)
+    mov_t = triangle_descriptors(moving, 30, max_triangles)
+    out = []
+    for md, mi in mov_t:
+        for rd, ri in ref_t:
+            if float(np.linalg.norm(md - rd)) > tolerance:
+                continue
+            mt = moving[list(mi)]
+            rt_raw = reference[list(ri)]
+            for order in permutations(range(3)):
+                try:
+                    out.append(fit_model(model, mt, rt_raw[list(order)]))
+                except ValueError:
+                    continue
+            if len(out) >= max_triangles:
+                return out
+    return out
+
+
+def estimate_star_transform(reference, moving, transform_model="translation", min_inliers=6, tolerance_px=3.0,
+                            max_stars=80):
+    """registration.py:509-569 (translation / similarity / affine)."""
+    ref, mov = _points(reference, max_stars), _points(moving, max_stars)
+    minimum = 3 if transform_model == "affine" else 2 if transform_model == "similarity" else 1
+    if len(ref) < minimum or len(mov) < minimum:
+        return Registration(transform_model, _translation(0.0, 0.0).tolist(), 0, 0, float("nan"), "failed", [])
+    cands = translation_candidates(ref, mov, 35)
+    if transform_model in ("similarity", "affine"):
+        cands = asterism_candidates(ref, mov, transform_model, 0.025, 600) + cands
+    if not cands:
+        cands = [np.eye(3, dtype=np.float64)]
+    best, best_pairs, best_rms = cands[0], [], float("inf")
+    for m in cands:
+        pairs, rms = greedy_pairs(ref, apply_matrix(mov, m), tolerance_px)
+        if len(pairs) > len(best_pairs) or (len(pairs) == len(best_pairs) and rms < best_rms):
+            best, best_pairs, best_rms = m, pairs, rms
+    for _ in range(2):
+        if len(best_pairs) < minimum:
+            break
+        try:
+            refined = fit_model(transform_model, mov[[p[0] for p in best_pairs]], ref[[p[1] for p in best_pairs]])
+        except ValueError:
+            break
+        pairs, rms = greedy_pairs(ref, apply_matrix(mov, refined), tolerance_px)
+        if len(pairs) > len(best_pairs) or (len(pairs) == len(best_pairs) and rms <= best_rms):
+            best, best_pairs, best_rms = refined, pairs, rms
+    status = "ok" if len(best_pairs) >= min_inliers else "failed"
+    return Registration(transform_model, [[float(v) for v in row] for row in best], len(best_pairs),
+                        len(best_pairs), float(best_rms) if np.isfinite(best_rms) else float("nan"), status,
+                        best_pairs)

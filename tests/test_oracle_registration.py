@@ -1,0 +1,129 @@
+"""Pins the star detection / registration oracle (CPU) against the reference
+tests' literal cases (tests/test_cpu_registration.py:64-112,
+tests/test_cpu_star_detect.py:30-40) and closed forms, and checks the
+library's host-side transform search against it (no GPU needed)."""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+import oracle.registration as O
+
+POINTS = np.array([(10, 10), (20, 10), (10, 25), (30, 30), (5, 35), (35, 5), (25, 18), (14, 32)], dtype=np.float64)
+
+
+def _gaussian_field(h, w, stars, sigma=1.4, background=10.0):
+    yy, xx = np.indices((h, w), dtype=np.float32)
+    img = np.full((h, w), background, dtype=np.float32)
+    for x, y, f in stars:
+        img += f * np.exp(-(((xx - x) ** 2 + (yy - y) ** 2) / (2.0 * sigma ** 2)))
+    return img.astype(np.float32)
+
+
+def test_translation_with_outlier_reference_case():
+    # tests/test_cpu_registration.py:64-84
+    ref = [O.Star(float(x), float(y), 100.0 - 10 * i) for i, (x, y) in enumerate(POINTS)]
+    mov = [O.Star(s.x - 3, s.y + 2, s.flux) for s in ref] + [O.Star(100.0, 100.0, 10.0)]
+    r = O.estimate_star_transform(ref, mov, "translation", min_inliers=6, tolerance_px=0.5)
+    assert r.status == "ok" and r.inliers == 8 and r.rms_px == 0.0
+    assert abs(r.matrix[0][2] - 3.0) < 1e-6 and abs(r.matrix[1][2] + 2.0) < 1e-6
+
+
+def _similarity_case(angle_deg, scale, t):
+    a = math.radians(angle_deg)
+    lin = scale * np.array([[math.cos(a), -math.sin(a)], [math.sin(a), math.cos(a)]])
+    mov = (POINTS - np.asarray(t)) @ np.linalg.inv(lin).T
+    ref = [O.Star(float(x), float(y), 1000.0 - i) for i, (x, y) in enumerate(POINTS)]
+    movs = [O.Star(float(x), float(y), 1000.0 - i) for i, (x, y) in enumerate(mov)]
+    return ref, movs, lin
+
+
+def test_similarity_reference_case():
+    # tests/test_cpu_registration.py:87-111
+    ref, mov, lin = _similarity_case(5.0, 1.02, (3.0, -2.0))
+    r = O.estimate_star_transform(ref, mov, "similarity", min_inliers=6, tolerance_px=0.5)
+    assert r.status == "ok" and r.inliers == 8 and r.rms_px < 1e-5
+    m = np.asarray(r.matrix)
+    assert np.allclose(m[:2, :2], lin, atol=1e-6) and np.allclose(m[:2, 2], [3.0, -2.0], atol=1e-6)
+
+
+def test_affine_recovers_exact_map():
+    lin = np.array([[0.99, -0.05], [0.04, 1.01]])
+    t = np.array([4.5, 1.25])
+    mov = (POINTS - t) @ np.linalg.inv(lin).T
+    ref = [O.Star(float(x), float(y), 1000.0 - i) for i, (x, y) in enumerate(POINTS)]
+    movs = [O.Star(float(x), float(y), 1000.0 - i) for i, (x, y) in enumerate(mov)]
+    r = O.estimate_star_transform(ref, movs, "affine", min_inliers=6, tolerance_px=0.5)
+    assert r.status == "ok" and r.inliers == 8
+    m = np.asarray(r.matrix)
+    assert np.allclose(m[:2, :2], lin, atol=1e-9) and np.allclose(m[:2, 2], t, atol=1e-9)
+
+
+def test_detect_symmetric_star_centroid_and_order():
+    img = _gaussian_field(48, 64, [(20, 16, 300.0), (44, 30, 500.0)])
+    stars = O.detect_stars(img, threshold_sigma=3.0)
+    assert len(stars) == 2
+    # brightest first; a star centred on a pixel has its centroid there by symmetry
+    assert abs(stars[0].x - 44.0) < 1e-9 and abs(stars[0].y - 30.0) < 1e-9
+    assert abs(stars[1].x - 20.0) < 1e-9 and abs(stars[1].y - 16.0) < 1e-9
+    # second moments of a sampled Gaussian: fwhm ~ 2.3548 sigma, round star
+    assert 0.9 * 2.3548 * 1.4 < stars[0].fwhm_px < 1.05 * 2.3548 * 1.4
+    assert stars[0].eccentricity < 1e-6
+
+
+def test_detect_elongated_reference_case():
+    # tests/test_cpu_star_detect.py:30-40
+    yy, xx = np.mgrid[0:64, 0:64]
+    img = (1000.0 * np.exp(-(((xx - 32.0) / 2.3) ** 2 + ((yy - 32.0) / 1.0) ** 2) / 2.0)).astype(np.float32)
+    stars = O.detect_stars(img, threshold_sigma=3.0)
+    assert len(stars) == 1 and stars[0].eccentricity > 0.7
+
+
+def test_background_uses_float32_median_pair_average():
+    # np.median of float32 averages the middle pair in float32: f32(a + b) / 2
+    a, b = np.float32(1000.1), np.float32(1000.2)
+    img = np.array([[a, b, 5.0, 2000.0]], dtype=np.float32)
+    med, _rms, _mad, _noise = O.robust_background(img)
+    assert med == float(np.float32(a + b) / np.float32(2.0))
+    assert med != (float(a) + float(b)) / 2.0
+
+
+def test_dedupe_keeps_brighter_of_close_pair():
+    s = [O.Star(10.0, 10.0, 5.0), O.Star(11.0, 10.5, 9.0), O.Star(30.0, 30.0, 1.0)]
+    kept = O.dedupe_stars(s, 2.0)
+    assert [k.flux for k in kept] == [9.0, 1.0]
+
+
+# -------------------------------------------- library host-side transforms
+@pytest.mark.parametrize("model", ["translation", "similarity", "affine"])
+def test_library_transform_matches_oracle(model):
+    from glass_amd import registration as R
+
+    rng = np.random.default_rng(3)
+    pts = rng.uniform(0, 400, size=(40, 2))
+    a = math.radians(0.7)
+    lin = np.array([[math.cos(a), -math.sin(a)], [math.sin(a), math.cos(a)]]) * (1.0 if model != "affine" else 1.001)
+    if model == "translation":
+        lin = np.eye(2)
+    t = np.array([12.3, -7.6])
+    mov = (pts - t) @ np.linalg.inv(lin).T + rng.normal(0, 0.05, size=pts.shape)
+    ref = [O.Star(float(x), float(y), 1000.0 - i) for i, (x, y) in enumerate(pts)]
+    movs = [O.Star(float(x), float(y), 1000.0 - i) for i, (x, y) in enumerate(mov)]
+    movs += [O.Star(1000.0, 1000.0, 1.0)]
+    o = O.estimate_star_transform(ref, movs, model, min_inliers=6, tolerance_px=1.0)
+    g = R.estimate_star_transform(ref, movs, model, min_inliers=6, tolerance_px=1.0)
+    assert g.status == o.status == "ok"
+    assert sorted(g.pairs) == sorted(o.pairs)
+    tol = 0.0 if model == "translation" else 1e-9  # lstsq / svd last bits are not pinned
+    assert np.allclose(np.asarray(g.matrix), np.asarray(o.matrix), rtol=0, atol=tol)
+    assert g.rms_px == pytest.approx(o.rms_px, rel=1e-9)
+
+
+def test_library_transform_rejects_unknown_model():
+    from glass_amd import registration as R
+
+    with pytest.raises(ValueError):
+        R.estimate_star_transform([O.Star(1.0, 1.0, 1.0)], [O.Star(1.0, 1.0, 1.0)], "homography")
