@@ -140,6 +140,12 @@ _SIGNATURES = {
     "gh_synchronize": [c_void_p],
     "gh_detect_stars": [c_void_p, c_int64, c_double, c_int64, c_double, c_double, c_int32, c_double, c_int64,
                         c_int64, POINTER(StarRecord), c_int64, POINTER(c_int64), POINTER(c_double)],
+    "gh_affine_workspace_size": [c_int64, c_int64, POINTER(c_int64)],
+    "gh_affine_lanczos3_forward": [c_void_p, c_int64, c_int64, POINTER(c_double), c_float, c_void_p, c_void_p],
+    "gh_affine_lanczos3_backward": [c_void_p, c_int64, c_int64, POINTER(c_double), c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_int64, c_void_p],
+    "gh_refine_affine": [c_void_p, c_int64, c_int64, POINTER(c_double), c_int32, c_double, POINTER(c_double),
+                         POINTER(c_double)],
     "gh_estimate_star_transform": [POINTER(c_double), c_int64, POINTER(c_double), c_int64, c_int32, c_int64,
                                    c_double, c_int64, POINTER(c_double), POINTER(c_int64), POINTER(c_int64),
                                    POINTER(c_double), POINTER(c_int32)],

@@ -19,6 +19,7 @@
 #include "common.hpp"
 #include "kernels.hpp"
 #include "stars.hpp"
+#include "adjoint.hpp"
 
 using gh::check;
 
@@ -104,6 +105,7 @@ struct gh_stack {
   DeviceBuffer<unsigned> out_dq;
   DeviceBuffer<gh::FrameMoments> moments;
   DeviceBuffer<unsigned> radix;
+  DeviceBuffer<double> adj_ws;              // refinement partials + sums
   DeviceBuffer<unsigned> star_cand;         // [1 + capacity] count, candidate pixels
   DeviceBuffer<float> star_windows;         // gathered candidate windows
   DeviceBuffer<unsigned> warp_flags;        // [kWarpBatch] source-flag snapshot
@@ -189,6 +191,7 @@ struct gh_stack {
     lanczos_lut.release();
     star_cand.release();
     star_windows.release();
+    adj_ws.release();
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -835,6 +838,190 @@ int gh_detect_stars(gh_stack* s, int64_t index, double threshold_sigma, int64_t 
                        st.has_fwhm ? 1 : 0, st.has_eccentricity ? 1 : 0};
     }
     *count = static_cast<int64_t>(stars.size());
+  });
+}
+
+namespace {
+
+gh::AffineMap affine_map_of(const double* map6) {
+  if (!map6) throw gh::ValueError("map pointer is NULL");
+  gh::AffineMap m{};
+  for (int k = 0; k < 6; ++k) {
+    if (!std::isfinite(map6[k])) throw gh::ValueError("affine map must be finite");
+    m.b[k] = map6[k];
+  }
+  return m;
+}
+
+void check_image_dims(int64_t h, int64_t w) {
+  if (h < 1 || w < 1 || h > (1 << 24) || w > (1 << 24) || h * w >= (int64_t{1} << 31))
+    throw gh::ValueError("image dimensions must be positive and below 2^31 pixels");
+}
+
+// Inverse of a 3x3 affine matrix (last row 0 0 1), FP64.
+void invert_affine3(const double* m, double* inv) {
+  if (!(m[6] == 0.0 && m[7] == 0.0 && m[8] == 1.0)) throw gh::ValueError("matrix must be affine (last row 0 0 1)");
+  for (int k = 0; k < 9; ++k)
+    if (!std::isfinite(m[k])) throw gh::ValueError("matrix must be finite");
+  const double det = m[0] * m[4] - m[1] * m[3];
+  if (!(std::fabs(det) > 1.0e-20)) throw gh::ValueError("matrix is singular");
+  inv[0] = m[4] / det;
+  inv[1] = -m[1] / det;
+  inv[3] = -m[3] / det;
+  inv[4] = m[0] / det;
+  inv[2] = -(inv[0] * m[2] + inv[1] * m[5]);
+  inv[5] = -(inv[3] * m[2] + inv[4] * m[5]);
+  inv[6] = 0.0;
+  inv[7] = 0.0;
+  inv[8] = 1.0;
+}
+
+// Solves the symmetric positive definite 6x6 system a x = b (Cholesky, FP64,
+// Jacobi-scaled).  Returns false when the matrix is not positive definite.
+bool solve_spd6(const double (&a_in)[6][6], const double (&b_in)[6], double (&x)[6]) {
+  double d[6], a[6][6], b[6];
+  for (int i = 0; i < 6; ++i) d[i] = a_in[i][i] > 0.0 ? std::sqrt(a_in[i][i]) : 1.0;
+  for (int i = 0; i < 6; ++i) {
+    b[i] = b_in[i] / d[i];
+    for (int j = 0; j < 6; ++j) a[i][j] = a_in[i][j] / (d[i] * d[j]);
+  }
+  double l[6][6] = {};
+  for (int i = 0; i < 6; ++i) {
+    for (int j = 0; j <= i; ++j) {
+      double s = a[i][j];
+      for (int k = 0; k < j; ++k) s -= l[i][k] * l[j][k];
+      if (i == j) {
+        if (!(s > 1.0e-14)) return false;
+        l[i][i] = std::sqrt(s);
+      } else {
+        l[i][j] = s / l[j][j];
+      }
+    }
+  }
+  double y[6];
+  for (int i = 0; i < 6; ++i) {
+    double s = b[i];
+    for (int k = 0; k < i; ++k) s -= l[i][k] * y[k];
+    y[i] = s / l[i][i];
+  }
+  for (int i = 5; i >= 0; --i) {
+    double s = y[i];
+    for (int k = i + 1; k < 6; ++k) s -= l[k][i] * x[k];
+    x[i] = s / l[i][i];
+  }
+  for (int i = 0; i < 6; ++i) x[i] /= d[i];
+  return true;
+}
+
+}  // namespace
+
+int gh_affine_workspace_size(int64_t height, int64_t width, int64_t* doubles) {
+  return guarded([&] {
+    check_image_dims(height, width);
+    if (!doubles) throw gh::ValueError("output pointer is NULL");
+    *doubles = static_cast<int64_t>(gh::adjoint_workspace_doubles(height * width));
+  });
+}
+
+int gh_affine_lanczos3_forward(const float* src, int64_t height, int64_t width, const double* map6, float fill,
+                               float* out, void* stream) {
+  return guarded([&] {
+    check_image_dims(height, width);
+    if (!src || !out) throw gh::ValueError("image pointer is NULL");
+    if (!is_device_pointer(src) || !is_device_pointer(out)) throw gh::ValueError("images must be device memory");
+    gh::launch_affine_forward(src, static_cast<int>(height), static_cast<int>(width), affine_map_of(map6), fill, out,
+                              static_cast<hipStream_t>(stream));
+  });
+}
+
+int gh_affine_lanczos3_backward(const float* src, int64_t height, int64_t width, const double* map6,
+                                const float* grad_out, float* grad_src, double* grad_map6, double* workspace,
+                                int64_t workspace_doubles, void* stream) {
+  return guarded([&] {
+    check_image_dims(height, width);
+    if (!src || !grad_out) throw gh::ValueError("image pointer is NULL");
+    if (!is_device_pointer(src) || !is_device_pointer(grad_out) || (grad_src && !is_device_pointer(grad_src)))
+      throw gh::ValueError("images must be device memory");
+    if (grad_map6) {
+      if (!is_device_pointer(grad_map6) || !workspace || !is_device_pointer(workspace))
+        throw gh::ValueError("grad_map6 and workspace must be device memory");
+      if (workspace_doubles < static_cast<int64_t>(gh::adjoint_workspace_doubles(height * width)))
+        throw gh::ValueError("workspace is smaller than gh_affine_workspace_size");
+    }
+    gh::launch_affine_backward(src, static_cast<int>(height), static_cast<int>(width), affine_map_of(map6), grad_out,
+                               grad_src, workspace, grad_map6, static_cast<hipStream_t>(stream));
+  });
+}
+
+int gh_refine_affine(gh_stack* s, int64_t reference_index, int64_t moving_index, const double* matrix_in,
+                     int32_t max_iterations, double tolerance_px, double* matrix_out, double* report) {
+  return guarded([&] {
+    s->require_loaded(reference_index, "affine refinement");
+    s->require_loaded(moving_index, "affine refinement");
+    if (!matrix_in || !matrix_out) throw gh::ValueError("matrix pointer is NULL");
+    if (max_iterations < 0) throw gh::ValueError("max_iterations must be >= 0");
+    const int h = static_cast<int>(s->height), w = static_cast<int>(s->width);
+    double inv[9];
+    invert_affine3(matrix_in, inv);  // reference pixel -> moving pixel
+    const double cx = 0.5 * (w - 1), cy = 0.5 * (h - 1);
+    // Centred parameters: s = Bc (x - cx, y - cy, 1).
+    double bc[6] = {inv[0], inv[1], inv[0] * cx + inv[1] * cy + inv[2],
+                    inv[3], inv[4], inv[3] * cx + inv[4] * cy + inv[5]};
+    auto to_map = [&](const double (&c)[6]) {
+      gh::AffineMap m{};
+      m.b[0] = c[0];
+      m.b[1] = c[1];
+      m.b[2] = c[2] - c[0] * cx - c[1] * cy;
+      m.b[3] = c[3];
+      m.b[4] = c[4];
+      m.b[5] = c[5] - c[3] * cx - c[4] * cy;
+      return m;
+    };
+    const size_t ws = gh::adjoint_workspace_doubles(s->npix);
+    s->adj_ws.ensure(ws, "hipMalloc(refinement workspace)");
+    double* sums_dev = s->adj_ws.ptr + (ws - gh::kNormalSums);
+    double sums[gh::kNormalSums];
+    auto normal_sums = [&](const gh::AffineMap& m) {
+      gh::launch_normal_equations(s->slot[reference_index], s->slot[moving_index], h, w, m, cx, cy, s->adj_ws.ptr,
+                                  sums_dev, s->stream);
+      check(hipMemcpyAsync(sums, sums_dev, sizeof(sums), hipMemcpyDeviceToHost, s->stream), "hipMemcpy(sums)");
+      check(hipStreamSynchronize(s->stream), "normal equations");
+    };
+    int it = 0;
+    double last_step = 0.0;
+    for (; it < max_iterations; ++it) {
+      normal_sums(to_map(bc));
+      if (sums[28] < 6.0) throw std::runtime_error("affine refinement: too few overlapping pixels");
+      double a[6][6], g[6], delta[6];
+      int t = 0;
+      for (int i = 0; i < 6; ++i)
+        for (int j = i; j < 6; ++j) a[i][j] = a[j][i] = sums[t++];
+      for (int i = 0; i < 6; ++i) g[i] = -sums[21 + i];
+      if (!solve_spd6(a, g, delta)) throw std::runtime_error("affine refinement: singular normal equations");
+      for (int i = 0; i < 6; ++i) bc[i] += delta[i];
+      // largest displacement of an image corner under the update
+      last_step = 0.0;
+      for (int c = 0; c < 4; ++c) {
+        const double xc = ((c & 1) ? w - 1 : 0) - cx, yc = ((c & 2) ? h - 1 : 0) - cy;
+        const double ux = delta[0] * xc + delta[1] * yc + delta[2];
+        const double uy = delta[3] * xc + delta[4] * yc + delta[5];
+        last_step = std::max(last_step, std::sqrt(ux * ux + uy * uy));
+      }
+      if (last_step < tolerance_px) {
+        ++it;
+        break;
+      }
+    }
+    const gh::AffineMap fin = to_map(bc);
+    double out_inv[9] = {fin.b[0], fin.b[1], fin.b[2], fin.b[3], fin.b[4], fin.b[5], 0.0, 0.0, 1.0};
+    invert_affine3(out_inv, matrix_out);
+    if (report) {
+      normal_sums(fin);
+      report[0] = static_cast<double>(it);
+      report[1] = sums[28] > 0.0 ? std::sqrt(sums[27] / sums[28]) : NAN;
+      report[2] = sums[28];
+      report[3] = last_step;
+    }
   });
 }
 

@@ -390,6 +390,14 @@ class ResidentCalibratedStack:
                                      min_separation_px, origin_x, origin_y)
         return stars
 
+    def refine_affine(self, reference_index: int, moving_index: int, matrix, max_iterations: int = 20,
+                      tolerance_px: float = 1.0e-4):
+        """Gauss-Newton photometric refinement of the moving -> reference affine
+        matrix of two resident frames (see glass_amd.refine.refine_affine)."""
+        from .refine import refine_affine
+
+        return refine_affine(self, reference_index, moving_index, matrix, max_iterations, tolerance_px)
+
     # ---------------------------------------------------------------- stats
     def frame_global_stats(self, index: int, exact_median: bool = False) -> dict:
         st = N.FrameStatistics()

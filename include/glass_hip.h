@@ -228,6 +228,39 @@ int gh_estimate_star_transform(const double* ref_xy, int64_t n_ref, const double
                                int32_t model, int64_t min_inliers, double tolerance_px, int64_t max_stars,
                                double* matrix, int64_t* pairs, int64_t* n_pairs, double* rms, int32_t* ok);
 
+/* --------------------------------------- affine Lanczos3 adjoint / refinement
+ * No counterpart in the reference: the warp of src/glass/engine/warp.py:121-228
+ * (CPU-path Lanczos3, normalised weights) for an affine source map
+ * s = B (x, y, 1), B = map6 row-major 2x3 (reference pixel -> source pixel),
+ * and its adjoint with respect to the pixels and the 6 map parameters, for
+ * photometric registration refinement (the analogue of the reference's
+ * refine_matrix_translation_candidates_*, native_bindings.cpp:10079-10416).
+ * Images are device f32 [h*w]; nothing is allocated inside these calls. */
+
+/* Device doubles of workspace gh_affine_lanczos3_backward needs for h x w. */
+int gh_affine_workspace_size(int64_t height, int64_t width, int64_t* doubles);
+
+/* out = warp(src; map6), `fill` outside the footprint 2 <= s < dim-3. */
+int gh_affine_lanczos3_forward(const float* src, int64_t height, int64_t width, const double* map6, float fill,
+                               float* out, void* stream);
+
+/* grad_src (device f32, nullable) += adjoint of grad_out; grad_map6 (device f64
+ * [6], nullable) = d <grad_out, warp(src; map6)> / d map6.  workspace: device
+ * f64 of gh_affine_workspace_size doubles. */
+int gh_affine_lanczos3_backward(const float* src, int64_t height, int64_t width, const double* map6,
+                                const float* grad_out, float* grad_src, double* grad_map6, double* workspace,
+                                int64_t workspace_doubles, void* stream);
+
+/* Gauss-Newton refinement of the moving->reference affine matrix of resident
+ * frame `moving_index` against frame `reference_index`: minimises
+ * sum (warp(moving) - reference)^2 over the pixels in the footprint.
+ * matrix_in / matrix_out: 3x3 row-major (host); stops when the update moves
+ * every image corner by less than tolerance_px or after max_iterations.
+ * report (host, nullable) [4]: iterations, final RMS residual, pixels used,
+ * last update (px). */
+int gh_refine_affine(gh_stack* stack, int64_t reference_index, int64_t moving_index, const double* matrix_in,
+                     int32_t max_iterations, double tolerance_px, double* matrix_out, double* report);
+
 #ifdef __cplusplus
 }
 #endif

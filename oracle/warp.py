@@ -184,3 +184,31 @@ def warp_matrix_resident(image, matrix, fill=np.nan, interpolation="lanczos3", c
     out.ravel()[idx] = value[ok]
     cov.ravel()[idx] = 1.0
     return out, cov
+
+
+def warp_lanczos3_source_map(image, source_map, fill=0.0, dtype=np.float64):
+    """CPU-path Lanczos3 (engine/warp.py:121-152, no tap skipping) for an
+    affine source map s = B (x, y, 1), B = source_map (2x3, reference pixel
+    -> source pixel), returned in `dtype` (FP64 for finite differences).
+    Footprint 2 <= s < dim-3 as engine/warp.py:207-212; `fill` elsewhere."""
+    img = np.asarray(image, dtype=np.float32)
+    h, w = img.shape
+    b = np.asarray(source_map, dtype=np.float64).reshape(2, 3)
+    ys, xs = np.mgrid[0:h, 0:w].astype(np.float64)
+    sx = (b[0, 0] * xs + b[0, 1] * ys) + b[0, 2]
+    sy = (b[1, 0] * xs + b[1, 1] * ys) + b[1, 2]
+    ok = (sx >= 2.0) & (sx < w - 3) & (sy >= 2.0) & (sy < h - 3)
+    out = np.full((h, w), fill, dtype=np.float64)
+    if np.any(ok):
+        px, py = sx[ok], sy[ok]
+        bx, by = np.floor(px).astype(np.int64), np.floor(py).astype(np.int64)
+        acc = np.zeros(px.shape)
+        wsum = np.zeros(px.shape)
+        for dy in range(-2, 4):
+            wy = lanczos3_weight(py - (by + dy), 1.0e-8)
+            for dx in range(-2, 4):
+                wgt = wy * lanczos3_weight(px - (bx + dx), 1.0e-8)
+                acc += img[by + dy, bx + dx].astype(np.float64) * wgt
+                wsum += wgt
+        out[ok] = acc / wsum
+    return out.astype(dtype)
