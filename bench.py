@@ -45,6 +45,9 @@ def parse_args():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-rows", type=int, default=32, help="rows per frame of the CPU baseline sample")
     p.add_argument("--seed", type=int, default=2024)
+    p.add_argument("--no-refine", action="store_true", help="skip the refined-registration (config 4) leg")
+    p.add_argument("--refine-frames", type=int, default=64)
+    p.add_argument("--refine-size", type=int, default=4096)
     return p.parse_args()
 
 
@@ -89,6 +92,87 @@ def cpu_baseline(args, light_exposure, dark_exposure):
         "kind": "port",
         "sample": f"{f} frames x {h} x {w} (first {h} rows; sub-pixel drift), calibrate+{args.interpolation} "
                   f"warp+winsorized integrate in oracle/cpu_ref.c, {elapsed:.2f} s",
+    }
+
+
+def refine_leg(args):
+    """Refined-registration form (config 4): 64 frames of 4096x4096 rendered
+    from one analytic scene through known affine maps (rotation U(+-0.5 deg),
+    shift U(+-20 px), scale 1 +- 5e-4); each refinement starts 0.3 px / 0.02
+    deg away and runs Gauss-Newton on the photometric residual through the
+    Lanczos3 adjoint.  Reports throughput (frame-Mpix of refined frames per s)
+    and the corner error against the known maps."""
+    import math
+
+    import numpy as np
+    import torch
+
+    import glass_amd
+    from glass_amd.synthetic_gpu import render_stars_
+
+    n, size = args.refine_frames, args.refine_size
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(args.seed + 400)
+    rng = np.random.default_rng(args.seed + 401)
+    n_stars = 1500
+    sx = torch.rand(n_stars, generator=g, device=dev) * size
+    sy = torch.rand(n_stars, generator=g, device=dev) * size
+    peaks = 10 ** (torch.rand(n_stars, generator=g, device=dev) * 2.0 - 0.5)
+    sig = (torch.rand(n_stars, generator=g, device=dev) * 2.0 + 2.0) / 2.3548  # FWHM 2-4 px
+    yy, xx = torch.meshgrid(torch.arange(size, device=dev, dtype=torch.float64),
+                            torch.arange(size, device=dev, dtype=torch.float64), indexing="ij")
+
+    def render(m):
+        # frame pixel p shows the scene at m @ p (m: frame -> scene/reference)
+        u = m[0, 0] * xx + m[0, 1] * yy + m[0, 2]
+        v = m[1, 0] * xx + m[1, 1] * yy + m[1, 2]
+        img = (0.05 + 0.01 * torch.sin(u / 300.0) * torch.cos(v / 410.0)).float()
+        inv = np.linalg.inv(m)
+        px = (inv[0, 0] * sx + inv[0, 1] * sy + inv[0, 2]).float()
+        py = (inv[1, 0] * sx + inv[1, 1] * sy + inv[1, 2]).float()
+        render_stars_(img, px, py, peaks.float(), sig.float(), 8)
+        return img
+
+    stack = glass_amd.ResidentCalibratedStack(n + 1, size, size)
+    stack.upload_calibrated_frame(0, render(np.eye(3)))
+    truths, starts = [], []
+    for k in range(n):
+        a = math.radians(rng.uniform(-0.5, 0.5))
+        sc = 1.0 + rng.uniform(-5e-4, 5e-4)
+        m = np.array([[sc * math.cos(a), -sc * math.sin(a), rng.uniform(-20, 20)],
+                      [sc * math.sin(a), sc * math.cos(a), rng.uniform(-20, 20)], [0.0, 0.0, 1.0]])
+        stack.upload_calibrated_frame(k + 1, render(m))
+        da = math.radians(rng.choice([-0.02, 0.02]))
+        rot = np.array([[math.cos(da), -math.sin(da), 0.0], [math.sin(da), math.cos(da), 0.0], [0.0, 0.0, 1.0]])
+        start = rot @ m
+        start[0, 2] += rng.choice([-0.3, 0.3])
+        start[1, 2] += rng.choice([-0.3, 0.3])
+        truths.append(m)
+        starts.append(start)
+    torch.cuda.synchronize()
+    del xx, yy
+    corners = np.array([[0, 0, 1], [size - 1, 0, 1], [0, size - 1, 1], [size - 1, size - 1, 1]], dtype=np.float64).T
+    # one untimed refinement (workspace allocation)
+    stack.refine_affine(0, 1, starts[0], max_iterations=2, tolerance_px=1e-3)
+    t0 = time.perf_counter()
+    errs, iters = [], []
+    for k in range(n):
+        refined, rep = stack.refine_affine(0, k + 1, starts[k], max_iterations=20, tolerance_px=1e-3)
+        iters.append(rep["iterations"])
+        errs.append(float(np.max(np.abs((refined @ corners)[:2] - (truths[k] @ corners)[:2]))))
+    elapsed = time.perf_counter() - t0
+    start_err = max(float(np.max(np.abs((starts[k] @ corners)[:2] - (truths[k] @ corners)[:2]))) for k in range(n))
+    stack.close()
+    torch.cuda.empty_cache()
+    return {
+        "workload": f"{n} frames x {size}x{size}, Gauss-Newton affine refinement (Lanczos3 adjoint)",
+        "value": round(n * size * size / 1e6 / elapsed, 1),
+        "unit": "frame-Mpix/s",
+        "ms_per_frame": round(elapsed * 1e3 / n, 3),
+        "mean_iterations": round(float(np.mean(iters)), 2),
+        "start_corner_err_px_max": round(start_err, 4),
+        "corner_err_px_max": round(max(errs), 5),
+        "corner_err_px_median": round(float(np.median(errs)), 5),
     }
 
 
@@ -241,6 +325,9 @@ def main():
     stack.close()
     del light_raw
     torch.cuda.empty_cache()
+    if not args.no_refine:
+        log("refined-registration leg (config 4)")
+        line["refined_registration"] = refine_leg(args)
     if not args.no_cpu_baseline:
         log("timing the CPU baseline sample")
         line["cpu_baseline"] = cpu_baseline(args, light_exp, dark_exp)

@@ -108,6 +108,25 @@ class StarRecord(ctypes.Structure):
     ]
 
 
+class BatchJob(ctypes.Structure):
+    _fields_ = [
+        ("stack", c_void_p),
+        ("warp_indices", POINTER(c_int64)),
+        ("inverse", POINTER(c_double)),
+        ("n_warp", c_int64),
+        ("interpolation", c_int32),
+        ("warp_flags", c_uint32),
+        ("fill", c_float),
+        ("clamping_threshold", c_float),
+        ("weights", POINTER(c_float)),
+        ("params", IntegrateParams),
+        ("outputs", IntegrateOutputs),
+    ]
+
+
+GH_BATCH_USE_GRAPH = 1
+GH_BATCH_AUTOTUNE = 2
+
 # name -> (argtypes); every function returns int
 _SIGNATURES = {
     "gh_last_error": [c_char_p, c_size_t],
@@ -146,6 +165,10 @@ _SIGNATURES = {
                                     c_void_p, c_int64, c_void_p],
     "gh_refine_affine": [c_void_p, c_int64, c_int64, POINTER(c_double), c_int32, c_double, POINTER(c_double),
                          POINTER(c_double)],
+    "gh_batch_create": [POINTER(BatchJob), c_int64, c_uint32, POINTER(c_void_p)],
+    "gh_batch_run": [c_void_p, c_void_p],
+    "gh_batch_info": [c_void_p, POINTER(c_int64)],
+    "gh_batch_destroy": [c_void_p],
     "gh_estimate_star_transform": [POINTER(c_double), c_int64, POINTER(c_double), c_int64, c_int32, c_int64,
                                    c_double, c_int64, POINTER(c_double), POINTER(c_int64), POINTER(c_int64),
                                    POINTER(c_double), POINTER(c_int32)],

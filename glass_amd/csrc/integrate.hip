@@ -22,6 +22,8 @@
 
 #include <utility>
 
+#include <algorithm>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -283,10 +285,9 @@ __device__ __attribute__((noinline)) float frame_axis_std(const IntegrateArgs& a
 }
 
 // ------------------------------------------------------------------ kernels
+// Winsorized clip + weighted mean of pixel p (p >= npix: no output).
 template <int CAP, bool UNIT>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CAP > 208 ? 1 : 2, 2))) void integrate_winsorized_kernel(
-    IntegrateArgs a) {
-  const long long p = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+__device__ __forceinline__ void winsorized_pixel(const IntegrateArgs& a, long long p) {
   const bool in_range = p < a.npix;
   const long long pp = in_range ? p : 0;
 
@@ -487,6 +488,39 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CAP > 20
   write_outputs(a, p, master, weight, coverage, static_cast<float>(n_low), static_cast<float>(n_high));
 }
 
+template <int CAP, bool UNIT>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CAP > 208 ? 1 : 2, 2))) void integrate_winsorized_kernel(
+    IntegrateArgs a) {
+  winsorized_pixel<CAP, UNIT>(a, static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x);
+}
+
+// Persistent kernel over the pixel blocks of many jobs of one capacity class:
+// virtual block vb belongs to the job whose [first_block, next first_block)
+// range holds it.  When a job carries active_flags, its statistics arithmetic
+// (all-valid vs valid, rejection.py:39-44) is decided here from the frames'
+// non-finite flags, so the launch needs no host decision between the warp and
+// the integration.
+template <int CAP, bool UNIT>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CAP > 208 ? 1 : 2, 2))) void integrate_batched_kernel(
+    const IntegrateArgs* __restrict__ jobs, const long long* __restrict__ first_block, int n_jobs,
+    long long total_blocks) {
+  for (long long vb = blockIdx.x; vb < total_blocks; vb += gridDim.x) {
+    int lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (first_block[mid] <= vb) lo = mid;
+      else hi = mid - 1;
+    }
+    IntegrateArgs a = jobs[lo];
+    if (a.active_flags) {
+      unsigned any = 0u;
+      for (int k = 0; k < a.n_active; ++k) any |= *a.active_flags[k];
+      a.variant_all_valid = (a.all_weights_positive && any == 0u) ? 1 : 0;
+    }
+    winsorized_pixel<CAP, UNIT>(a, (vb - first_block[lo]) * kBlock + threadIdx.x);
+  }
+}
+
 // Rejection-free weighted mean for any frame count (integrate_mean,
 // cpp/cuda/integration_kernels.cu:1610-1656; oracle cpu/integration.py:182-194).
 template <bool UNIT>
@@ -544,9 +578,42 @@ void launch_cap(const IntegrateArgs& a, hipStream_t stream) {
   }
 }
 
+template <int CAP>
+void launch_batched_cap(const IntegrateArgs* jobs, const long long* first_block, int n_jobs, long long total_blocks,
+                        bool unit, unsigned grid, hipStream_t stream) {
+  const unsigned g = static_cast<unsigned>(std::min<long long>(std::max(grid, 1u), total_blocks));
+  if (unit) {
+    integrate_batched_kernel<CAP, true><<<g, kBlock, 0, stream>>>(jobs, first_block, n_jobs, total_blocks);
+  } else {
+    integrate_batched_kernel<CAP, false><<<g, kBlock, 0, stream>>>(jobs, first_block, n_jobs, total_blocks);
+  }
+}
+
 }  // namespace
 
 int integrate_max_frames() { return 256; }
+
+int integrate_capacity(int n_active) {
+  return n_active <= 16 ? 16 : n_active <= 32 ? 32 : n_active <= 64 ? 64 : n_active <= 128 ? 128
+         : n_active <= 208 ? 208 : 256;
+}
+
+long long integrate_blocks(long long npix) { return (npix + kBlock - 1) / kBlock; }
+
+void launch_integrate_batched(int capacity, bool unit, const IntegrateArgs* jobs, const long long* first_block,
+                              int n_jobs, long long total_blocks, unsigned grid, hipStream_t stream) {
+  if (total_blocks <= 0) return;
+  switch (capacity) {
+    case 16: launch_batched_cap<16>(jobs, first_block, n_jobs, total_blocks, unit, grid, stream); break;
+    case 32: launch_batched_cap<32>(jobs, first_block, n_jobs, total_blocks, unit, grid, stream); break;
+    case 64: launch_batched_cap<64>(jobs, first_block, n_jobs, total_blocks, unit, grid, stream); break;
+    case 128: launch_batched_cap<128>(jobs, first_block, n_jobs, total_blocks, unit, grid, stream); break;
+    case 208: launch_batched_cap<208>(jobs, first_block, n_jobs, total_blocks, unit, grid, stream); break;
+    case 256: launch_batched_cap<256>(jobs, first_block, n_jobs, total_blocks, unit, grid, stream); break;
+    default: throw ValueError("unsupported integration capacity");
+  }
+  check(hipGetLastError(), "batched integrate launch");
+}
 
 void launch_integrate(const IntegrateArgs& a, hipStream_t stream) {
   if (a.rejection == 0) {

@@ -114,6 +114,31 @@ void launch_snapshot_warp_flags(const WarpBatch& b, int count, unsigned* snap, h
 
 void launch_warp(const WarpBatch& batch, int count, const WarpParams& p, hipStream_t stream);
 
+// ---- batched multi-job warp (one persistent launch over many jobs)
+struct BatchWarpFrame {
+  const float* src;
+  float* dst;
+  double inv[9];
+  const unsigned* src_flag;  // the source frame's non-finite flag (read only)
+  unsigned* dst_flag;        // raised when an output sample is non-finite
+};
+struct BatchWarpJob {
+  int height, width;
+  int tiles_x, tiles;        // output tiles of warp_tile_shape
+  int interpolation;         // 1 bilinear, 3 lanczos3
+  int skip_nonfinite;
+  float fill, clamp;
+  float* coverage;           // nullable, accumulated atomically
+  long long first_item;      // prefix of the items of earlier jobs
+  int first_frame, n_frames; // range in the frame array
+};
+// Output tile width / height of the tiled warp kernels.
+void warp_tile_shape(int* tile_w, int* tile_h);
+long long warp_batched_items(const BatchWarpJob& job);  // ceil(frames / kWarpBatch) * tiles
+unsigned warp_batched_max_blocks();                     // resident blocks that fill the device
+void launch_warp_batched(const BatchWarpJob* jobs, int n_jobs, const BatchWarpFrame* frames, long long total_items,
+                         const float4* lut, unsigned grid, int order_stride, hipStream_t stream);
+
 void launch_fill(float* dst, long long n, float value, hipStream_t stream);
 void launch_add_scalar(float* dst, long long n, float value, hipStream_t stream);
 // x = (x - offset) / divisor in place (f32), flagging non-finite results.
@@ -149,11 +174,23 @@ struct IntegrateArgs {
   unsigned* dq;
   const float* geometric;        // nullable
   int dq_active_frames;
+  // Batched launches: when non-null, the statistics arithmetic is decided on
+  // the device from these n_active frame flags (all_weights_positive && no
+  // flag set -> all-valid), overriding variant_all_valid.
+  const unsigned* const* active_flags;
+  int all_weights_positive;
 };
 
 // Returns the register capacity (max frames) the winsorized kernel supports.
 int integrate_max_frames();
 void launch_integrate(const IntegrateArgs& a, hipStream_t stream);
+// Capacity class (sorting network size) of the winsorized kernel for n_active.
+int integrate_capacity(int n_active);
+long long integrate_blocks(long long npix);
+// One persistent launch over the pixel blocks of all jobs of one capacity
+// class (jobs[]: device table; first_block[]: device prefix of blocks).
+void launch_integrate_batched(int capacity, bool unit, const IntegrateArgs* jobs, const long long* first_block,
+                              int n_jobs, long long total_blocks, unsigned grid, hipStream_t stream);
 
 // Per-tile "any non-finite sample" flags over n_active frames (for the
 // CPUStackEngine per-tile statistics choice); flags[t] = 1 if all finite.

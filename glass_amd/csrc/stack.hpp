@@ -1,0 +1,201 @@
+// Internal state of a gh_stack handle and the C-ABI error plumbing, shared by
+// the translation units that implement include/glass_hip.h.
+//
+// gh_stack mirrors the device state of the reference ResidentCalibratedStack
+// (cpp/src/native_bindings.cpp:2915-2996): an F x H x W float32 frame stack in
+// HBM, optional masters, a geometric warp-coverage accumulator and a stream.
+// Frames are addressed through a slot table so a warp can write a frame into a
+// spare slot and swap, instead of warping into scratch and copying back
+// (native_bindings.cpp:15442-15524 + cpp/cuda/warp_kernels.cu:806-914).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/glass_hip.h"
+#include "common.hpp"
+#include "kernels.hpp"
+
+using gh::check;
+
+namespace ghi {
+
+inline thread_local std::string g_last_error;
+
+inline int fail(int code, const std::string& message) {
+  g_last_error = message;
+  return code;
+}
+
+template <typename F>
+int guarded(F&& body) {
+  try {
+    body();
+    return GH_OK;
+  } catch (const gh::ValueError& e) {
+    return fail(GH_ERR_VALUE, e.what());
+  } catch (const gh::IndexError& e) {
+    return fail(GH_ERR_INDEX, e.what());
+  } catch (const std::bad_alloc& e) {
+    return fail(GH_ERR_RUNTIME, std::string("host allocation failed: ") + e.what());
+  } catch (const std::exception& e) {
+    return fail(GH_ERR_RUNTIME, e.what());
+  }
+}
+
+inline bool is_device_pointer(const void* ptr) {
+  if (ptr == nullptr) return false;
+  hipPointerAttribute_t attr;
+  const hipError_t err = hipPointerGetAttributes(&attr, ptr);
+  if (err != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+template <typename T>
+struct DeviceBuffer {
+  T* ptr = nullptr;
+  size_t count = 0;
+  void ensure(size_t n, const char* what) {
+    if (n <= count && ptr != nullptr) return;
+    release();
+    check(hipMalloc(reinterpret_cast<void**>(&ptr), std::max<size_t>(n, 1) * sizeof(T)), what);
+    count = n;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    count = 0;
+  }
+};
+
+}  // namespace ghi
+
+using ghi::DeviceBuffer;
+using ghi::is_device_pointer;
+
+struct gh_stack {
+  int64_t frames = 0, height = 0, width = 0, npix = 0;
+  float* pool = nullptr;                 // frames x npix allocation
+  std::vector<float*> slot;              // frame -> current slot
+  std::vector<float*> spare;             // free slots for out-of-place warps
+  std::vector<float*> spare_allocations; // separately allocated spare slots
+  std::vector<char> loaded;
+  int64_t loaded_count = 0;
+  unsigned* nonfinite = nullptr;         // [frames] flags
+  float* bias = nullptr;
+  float* dark = nullptr;
+  float* flat = nullptr;
+  float* geometric = nullptr;            // warp coverage accumulator
+  int64_t warp_coverage_frames = 0;
+  hipStream_t stream = nullptr;
+
+  DeviceBuffer<gh::StatPartial> partials;
+  DeviceBuffer<gh::StatPartial> calib_stats;  // [frames]
+  std::vector<char> calib_stats_valid;
+  DeviceBuffer<unsigned char> raw_staging;
+  DeviceBuffer<const float*> frame_table;
+  DeviceBuffer<float> frame_weights;
+  DeviceBuffer<unsigned char> tile_flags;
+  DeviceBuffer<float> out_f32[5];
+  DeviceBuffer<unsigned> out_dq;
+  DeviceBuffer<gh::FrameMoments> moments;
+  DeviceBuffer<unsigned> radix;
+  DeviceBuffer<double> adj_ws;              // refinement partials + sums
+  DeviceBuffer<unsigned> star_cand;         // [1 + capacity] count, candidate pixels
+  DeviceBuffer<float> star_windows;         // gathered candidate windows
+  DeviceBuffer<unsigned> warp_flags;        // [kWarpBatch] source-flag snapshot
+  DeviceBuffer<float> lanczos_lut;          // normalised Lanczos3 weight cubics
+
+  hipStream_t pick(void* s) const { return s ? static_cast<hipStream_t>(s) : stream; }
+
+  size_t frame_bytes() const { return static_cast<size_t>(npix) * sizeof(float); }
+
+  void require_index(int64_t i) const {
+    if (i < 0 || i >= frames) throw gh::IndexError("resident frame index is out of range");
+  }
+  void require_loaded(int64_t i, const char* op) const {
+    require_index(i);
+    if (!loaded[i]) throw std::runtime_error(std::string("resident frame must be loaded before ") + op);
+  }
+  void mark_loaded(int64_t i) {
+    if (!loaded[i]) {
+      loaded[i] = 1;
+      ++loaded_count;
+    }
+  }
+
+  int64_t bytes_allocated() const {
+    int64_t total = static_cast<int64_t>(frames) * static_cast<int64_t>(frame_bytes());
+    total += static_cast<int64_t>(spare_allocations.size()) * static_cast<int64_t>(frame_bytes());
+    if (bias) total += frame_bytes();
+    if (dark) total += frame_bytes();
+    if (flat) total += frame_bytes();
+    if (geometric) total += frame_bytes();
+    return total;
+  }
+
+  void ensure_geometric() {
+    if (geometric) return;
+    check(hipMalloc(reinterpret_cast<void**>(&geometric), frame_bytes()), "hipMalloc(warp coverage)");
+    gh::launch_fill(geometric, npix, 0.0f, stream);
+    warp_coverage_frames = 0;
+  }
+
+  void ensure_spares(size_t n) {
+    while (spare.size() < n) {
+      float* p = nullptr;
+      check(hipMalloc(reinterpret_cast<void**>(&p), frame_bytes()), "hipMalloc(warp spare slot)");
+      spare.push_back(p);
+      spare_allocations.push_back(p);
+    }
+  }
+
+  void upload_master(const float* src, float** dst, const char* what) {
+    if (src == nullptr) {
+      if (*dst) (void)hipFree(*dst);
+      *dst = nullptr;
+      return;
+    }
+    if (*dst == nullptr) check(hipMalloc(reinterpret_cast<void**>(dst), frame_bytes()), what);
+    check(hipMemcpyAsync(*dst, src, frame_bytes(), is_device_pointer(src) ? hipMemcpyDeviceToDevice
+                                                                            : hipMemcpyHostToDevice,
+                         stream),
+          what);
+  }
+
+  ~gh_stack() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    (void)hipFree(pool);
+    for (float* p : spare_allocations) (void)hipFree(p);
+    (void)hipFree(nonfinite);
+    (void)hipFree(bias);
+    (void)hipFree(dark);
+    (void)hipFree(flat);
+    (void)hipFree(geometric);
+    partials.release();
+    calib_stats.release();
+    raw_staging.release();
+    frame_table.release();
+    frame_weights.release();
+    tile_flags.release();
+    for (auto& b : out_f32) b.release();
+    out_dq.release();
+    moments.release();
+    radix.release();
+    warp_flags.release();
+    lanczos_lut.release();
+    star_cand.release();
+    star_windows.release();
+    adj_ws.release();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+

@@ -19,6 +19,7 @@
 // read-modify-write per pixel per launch (no atomics).  Frames are written out
 // of place into a spare slot; the host swaps slots afterwards.
 
+#include <algorithm>
 #include <cmath>
 #include <utility>
 
@@ -286,15 +287,26 @@ struct TileFrame {
 // [ox0, ox1] x [oy0, oy1].  An affine map takes its extremes at the corners;
 // interior points computed in FP64 stay within ~1e-11 px of the corners' hull,
 // so one pixel beyond the +-3 tap reach covers every tap.
-__device__ __forceinline__ TileFrame tile_frame(const WarpBatch& b, int f, bool clamp_on, int ox0, int oy0, int ox1,
+// One frame of a tile's work: source, destination, FP64 inverse (9, the
+// affine rows used), the source's "holds a non-finite sample" flag and the
+// destination's flag to raise.
+struct FrameRef {
+  const float* src;
+  float* dst;
+  const double* inv;
+  unsigned src_flag;
+  unsigned* dst_flag;
+};
+
+__device__ __forceinline__ TileFrame tile_frame(const FrameRef& ref, bool clamp_on, int ox0, int oy0, int ox1,
                                                 int oy1, int W, int H) {
   TileFrame fr;
-  fr.src = b.src[f];
-  fr.dst = b.dst[f];
-  const double* m = b.inv[f];
+  fr.src = ref.src;
+  fr.dst = ref.dst;
+  const double* m = ref.inv;
   fr.m0 = m[0]; fr.m1 = m[1]; fr.m2 = m[2];
   fr.m3 = m[3]; fr.m4 = m[4]; fr.m5 = m[5];
-  fr.fast = !clamp_on && b.src_flags[f] == 0u;
+  fr.fast = !clamp_on && ref.src_flag == 0u;
   double mnx = INFINITY, mxx = -INFINITY, mny = INFINITY, mxy = -INFINITY;
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
@@ -511,24 +523,13 @@ __device__ __forceinline__ bool column_direct_rows(const TileFrame& fr, const Wa
   return bad;
 }
 
-__global__ __launch_bounds__(kTiledThreads) void warp_lanczos3_tiled_kernel(WarpBatch b, int count, WarpParams p,
-                                                                             int tiles_x) {
-  __shared__ float4 lut[kLutRows * kLutVec];
-  __shared__ float tile[kMaxTile];
-  // host layout [interval][vec] -> LDS planes [vec][interval]
-  for (int i = threadIdx.x; i < kLutRows * kLutVec; i += kTiledThreads) {
-    lut[(i % kLutVec) * kLutRows + i / kLutVec] = p.lanczos_lut[i];
-  }
-
-  // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
-  // contiguous run of tiles so vertically adjacent tiles (which share source
-  // halo rows) meet in the same L2.
-  const unsigned nb = gridDim.x, bid = blockIdx.x;
-  const unsigned xcd = bid & 7u, q = nb >> 3, r = nb & 7u;
-  const unsigned logical = xcd * q + min(xcd, r) + (bid >> 3);
-  const int ox0 = static_cast<int>(logical % tiles_x) * kOutW;
-  const int oy0 = static_cast<int>(logical / tiles_x) * kOutH;
-
+// The tiled Lanczos3 work of one output tile over `count` frames (get(f)
+// yields frame f's FrameRef).  Uses the block's LDS table and box buffer; every
+// thread of the block must call it.  ATOMIC_COVERAGE: other blocks may add to
+// the same tile's coverage concurrently (batched launches).
+template <bool ATOMIC_COVERAGE, typename GetFrame>
+__device__ __forceinline__ void lanczos_tile(const GetFrame& get, int count, const WarpParams& p, int ox0, int oy0,
+                                             const float4* lut, float* tile) {
   const int tx = threadIdx.x & 63;
   const int ty = threadIdx.x >> 6;
   const int x = ox0 + 64 * (ty % kColGroups) + tx;
@@ -539,7 +540,8 @@ __global__ __launch_bounds__(kTiledThreads) void warp_lanczos3_tiled_kernel(Warp
   unsigned long long cov = 0;
 
   for (int f = 0; f < count; ++f) {
-    const TileFrame cur = tile_frame(b, f, clamp_on, ox0, oy0, ox1, oy1, W, H);
+    const FrameRef ref = get(f);
+    const TileFrame cur = tile_frame(ref, clamp_on, ox0, oy0, ox1, oy1, W, H);
     __syncthreads();  // previous frame's taps consumed (first pass: table stored)
     if (cur.use_tile) stage_box(cur, W, tile);
     __syncthreads();
@@ -554,7 +556,7 @@ __global__ __launch_bounds__(kTiledThreads) void warp_lanczos3_tiled_kernel(Warp
     } else {
       bad = column_direct_rows(cur, p, x, y_first, cov);
     }
-    if (__any(bad) && tx == 0) atomicOr(b.nonfinite[f], 1u);
+    if (__any(bad) && tx == 0) atomicOr(ref.dst_flag, 1u);
   }
   if (p.coverage && x < W) {
 #pragma unroll 1
@@ -562,7 +564,136 @@ __global__ __launch_bounds__(kTiledThreads) void warp_lanczos3_tiled_kernel(Warp
       const int y = y_first + k;
       if (y >= H) break;
       const long long o = static_cast<long long>(y) * W + x;
-      p.coverage[o] = p.coverage[o] + static_cast<float>((cov >> (4 * k)) & 15ull);
+      const float add = static_cast<float>((cov >> (4 * k)) & 15ull);
+      if constexpr (ATOMIC_COVERAGE) {
+        if (add != 0.0f) atomicAdd(p.coverage + o, add);
+      } else {
+        p.coverage[o] = p.coverage[o] + add;
+      }
+    }
+  }
+}
+
+// Bilinear over one output tile (same thread layout, no LDS), for batched
+// launches that mix interpolations.
+template <typename GetFrame>
+__device__ __forceinline__ void bilinear_tile(const GetFrame& get, int count, const WarpParams& p, int ox0, int oy0) {
+  const int tx = threadIdx.x & 63;
+  const int ty = threadIdx.x >> 6;
+  const int x = ox0 + 64 * (ty % kColGroups) + tx;
+  const int y_first = oy0 + kLaneRows * (ty / kColGroups);
+  const int W = p.width, H = p.height;
+  if (x >= W) return;
+  unsigned long long cov = 0;
+  for (int f = 0; f < count; ++f) {
+    const FrameRef ref = get(f);
+    bool bad = false;
+#pragma unroll 1
+    for (int k = 0; k < kLaneRows; ++k) {
+      const int y = y_first + k;
+      if (y >= H) break;
+      double sx, sy;
+      source_coords(ref.inv, 1, x, y, sx, sy);
+      const bool ok = isfinite(sx) && isfinite(sy) && sx >= 0.0 && sx <= static_cast<double>(W - 1) && sy >= 0.0 &&
+                      sy <= static_cast<double>(H - 1);
+      float value = p.fill;
+      if (ok) {
+        const double fx0 = floor(sx), fy0 = floor(sy);
+        const int x0 = static_cast<int>(fx0), y0 = static_cast<int>(fy0);
+        const int x1 = min(x0 + 1, W - 1), y1 = min(y0 + 1, H - 1);
+        const float tx_ = static_cast<float>(sx - fx0), ty_ = static_cast<float>(sy - fy0);
+        const float* src = ref.src;
+        const float a00 = src[static_cast<long long>(y0) * W + x0], a01 = src[static_cast<long long>(y0) * W + x1];
+        const float a10 = src[static_cast<long long>(y1) * W + x0], a11 = src[static_cast<long long>(y1) * W + x1];
+        const float top = a00 * (1.0f - tx_) + a01 * tx_;
+        const float bottom = a10 * (1.0f - tx_) + a11 * tx_;
+        value = top * (1.0f - ty_) + bottom * ty_;
+        cov += 1ull << (4 * k);
+      }
+      ref.dst[static_cast<long long>(y) * W + x] = value;
+      bad |= !isfinite(value);
+    }
+    if (bad) atomicOr(ref.dst_flag, 1u);
+  }
+  if (p.coverage) {
+#pragma unroll 1
+    for (int k = 0; k < kLaneRows; ++k) {
+      const int y = y_first + k;
+      if (y >= H) break;
+      const float add = static_cast<float>((cov >> (4 * k)) & 15ull);
+      if (add != 0.0f) atomicAdd(p.coverage + static_cast<long long>(y) * W + x, add);
+    }
+  }
+}
+
+__device__ __forceinline__ void load_lut(const float4* lut_g, float4* lut) {
+  // host layout [interval][vec] -> LDS planes [vec][interval]
+  for (int i = threadIdx.x; i < kLutRows * kLutVec; i += kTiledThreads) {
+    lut[(i % kLutVec) * kLutRows + i / kLutVec] = lut_g[i];
+  }
+}
+
+__global__ __launch_bounds__(kTiledThreads) void warp_lanczos3_tiled_kernel(WarpBatch b, int count, WarpParams p,
+                                                                             int tiles_x) {
+  __shared__ float4 lut[kLutRows * kLutVec];
+  __shared__ float tile[kMaxTile];
+  load_lut(p.lanczos_lut, lut);
+  // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+  // contiguous run of tiles so vertically adjacent tiles (which share source
+  // halo rows) meet in the same L2.
+  const unsigned nb = gridDim.x, bid = blockIdx.x;
+  const unsigned xcd = bid & 7u, q = nb >> 3, r = nb & 7u;
+  const unsigned logical = xcd * q + min(xcd, r) + (bid >> 3);
+  const int ox0 = static_cast<int>(logical % tiles_x) * kOutW;
+  const int oy0 = static_cast<int>(logical / tiles_x) * kOutH;
+  auto get = [&](int f) { return FrameRef{b.src[f], b.dst[f], b.inv[f], b.src_flags[f], b.nonfinite[f]}; };
+  lanczos_tile<false>(get, count, p, ox0, oy0, lut, tile);
+}
+
+// ------------------------------------------------- batched multi-job warp
+// Persistent kernel over the work items of many jobs: item = (job, batch of up
+// to kWarpBatch frames, output tile).  Blocks stride over the item list; a
+// job's items are found by binary search over the per-job first-item offsets.
+__global__ __launch_bounds__(kTiledThreads) void warp_batched_kernel(const BatchWarpJob* __restrict__ jobs,
+                                                                     int n_jobs, const BatchWarpFrame* __restrict__ frames,
+                                                                     long long total_items, const float4* lut_g,
+                                                                     int order_stride) {
+  __shared__ float4 lut[kLutRows * kLutVec];
+  __shared__ float tile[kMaxTile];
+  load_lut(lut_g, lut);
+  for (long long it = blockIdx.x; it < total_items; it += gridDim.x) {
+    // Optional interleaving: visit items with a stride coprime to the total so
+    // concurrent blocks spread over jobs.
+    const long long item = order_stride > 1 ? (it * order_stride) % total_items : it;
+    int lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (jobs[mid].first_item <= item) lo = mid;
+      else hi = mid - 1;
+    }
+    const BatchWarpJob& job = jobs[lo];
+    const long long local = item - job.first_item;
+    const int batch = static_cast<int>(local / job.tiles), t = static_cast<int>(local % job.tiles);
+    const int f0 = job.first_frame + batch * kWarpBatch;
+    const int count = min(kWarpBatch, job.first_frame + job.n_frames - f0);
+    const int ox0 = (t % job.tiles_x) * kOutW, oy0 = (t / job.tiles_x) * kOutH;
+    WarpParams p{};
+    p.height = job.height;
+    p.width = job.width;
+    p.interpolation = job.interpolation;
+    p.fill = job.fill;
+    p.clamp = job.clamp;
+    p.skip_nonfinite = job.skip_nonfinite;
+    p.affine = 1;
+    p.coverage = job.coverage;
+    auto get = [&](int f) {
+      const BatchWarpFrame& fr = frames[f0 + f];
+      return FrameRef{fr.src, fr.dst, fr.inv, *fr.src_flag, fr.dst_flag};
+    };
+    if (job.interpolation == 3) {
+      lanczos_tile<true>(get, count, p, ox0, oy0, lut, tile);
+    } else {
+      bilinear_tile(get, count, p, ox0, oy0);
     }
   }
 }
@@ -619,6 +750,32 @@ void make_lanczos_lut(float* table) {
       }
     }
   }
+}
+
+void warp_tile_shape(int* tile_w, int* tile_h) {
+  *tile_w = kOutW;
+  *tile_h = kOutH;
+}
+
+long long warp_batched_items(const BatchWarpJob& job) {
+  return static_cast<long long>((job.n_frames + kWarpBatch - 1) / kWarpBatch) * job.tiles;
+}
+
+unsigned warp_batched_max_blocks() {
+  int dev = 0, cus = 0, per_cu = 0;
+  check(hipGetDevice(&dev), "hipGetDevice");
+  check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "device CU count");
+  check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, warp_batched_kernel, kTiledThreads, 0),
+        "warp occupancy");
+  return static_cast<unsigned>(std::max(1, cus) * std::max(1, per_cu));
+}
+
+void launch_warp_batched(const BatchWarpJob* jobs, int n_jobs, const BatchWarpFrame* frames, long long total_items,
+                         const float4* lut, unsigned grid, int order_stride, hipStream_t stream) {
+  if (total_items <= 0) return;
+  const unsigned g = static_cast<unsigned>(std::min<long long>(std::max(grid, 1u), total_items));
+  warp_batched_kernel<<<g, kTiledThreads, 0, stream>>>(jobs, n_jobs, frames, total_items, lut, order_stride);
+  check(hipGetLastError(), "batched warp launch");
 }
 
 void launch_snapshot_warp_flags(const WarpBatch& b, int count, unsigned* snap, hipStream_t stream) {

@@ -261,6 +261,44 @@ int gh_affine_lanczos3_backward(const float* src, int64_t height, int64_t width,
 int gh_refine_affine(gh_stack* stack, int64_t reference_index, int64_t moving_index, const double* matrix_in,
                      int32_t max_iterations, double tolerance_px, double* matrix_out, double* report);
 
+/* ------------------------------------------------------ batched multi-job
+ * No counterpart in the reference (which runs one resident job per call):
+ * many stacking jobs -- each a loaded gh_stack, the frames to warp with their
+ * FP64 inverses, frame weights and integration parameters/outputs -- run as
+ * one persistent batched warp launch over all jobs' (frame batch, tile) items
+ * plus one persistent integration launch per capacity class, optionally
+ * captured once into a hipGraph and replayed.  Results equal the per-job
+ * calls (gh_warp_frames + gh_integrate with statistics_variant -1, tile_size 0).
+ * The batch owns the warped copies of the listed frames; the stacks' frames
+ * are read, not modified. */
+typedef struct gh_batch gh_batch;
+
+typedef struct gh_batch_job {
+  gh_stack* stack;
+  const int64_t* warp_indices;   /* host [n_warp]; other frames are used as loaded */
+  const double* inverse;         /* host [n_warp * 9], FP64 inverses (last row 0 0 1) */
+  int64_t n_warp;
+  int32_t interpolation;         /* GH_INTERP_BILINEAR or GH_INTERP_LANCZOS3 */
+  uint32_t warp_flags;           /* GH_WARP_SKIP_NONFINITE_TAPS (resident semantics) */
+  float fill;
+  float clamping_threshold;      /* < 0: off (Lanczos3 resident semantics only) */
+  const float* weights;          /* host [frame_count] or NULL (unit) */
+  gh_integrate_params params;    /* tile_size must be 0 */
+  gh_integrate_outputs outputs;  /* device pointers; dq_use_geometric uses the
+                                    batch's coverage (unwarped frames count 1) */
+} gh_batch_job;
+
+#define GH_BATCH_USE_GRAPH 1u  /* capture the launch sequence once, replay it */
+#define GH_BATCH_AUTOTUNE 2u   /* time candidate schedules on the first run */
+
+int gh_batch_create(const gh_batch_job* jobs, int64_t n_jobs, uint32_t flags, gh_batch** out);
+/* Enqueue one execution on `stream` (NULL: the batch's stream); asynchronous. */
+int gh_batch_run(gh_batch* batch, void* stream);
+/* out[8]: jobs, warp work items, integration launches, graph captured (0/1),
+ * warp grid, warp item stride, integration grid, autotune trials run. */
+int gh_batch_info(gh_batch* batch, int64_t* out);
+int gh_batch_destroy(gh_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif
