@@ -48,6 +48,9 @@ def parse_args():
     p.add_argument("--no-refine", action="store_true", help="skip the refined-registration (config 4) leg")
     p.add_argument("--refine-frames", type=int, default=64)
     p.add_argument("--refine-size", type=int, default=4096)
+    p.add_argument("--no-batch", action="store_true", help="skip the batched multi-job (config 3) leg")
+    p.add_argument("--batch-jobs", type=int, default=64)
+    p.add_argument("--batch-reps", type=int, default=3)
     return p.parse_args()
 
 
@@ -176,6 +179,171 @@ def refine_leg(args):
     }
 
 
+def batched_leg(args):
+    """Batched multi-job form (config 3): `--batch-jobs` resident jobs with
+    F ~ U{8..120} frames of 1024x1024, 2048x1536 or 4096x3072, each frame
+    rendered from a per-job star field through its own affine map (drift
+    +-20 px, rotation +-0.5 deg), 0-10% of frames given zero weight and 0-10%
+    carrying a masked (NaN) satellite trail, bilinear or Lanczos3 per job,
+    winsorized sigma 3/3 with every output map.  The same jobs run as one
+    StackingBatch (autotuned, captured into a hipGraph, replayed) and as serial
+    per-job warp_frames + integrate_to_device calls; the maps of both forms are
+    compared bit for bit and both throughputs are reported (frame-Mpix of
+    input frames per second)."""
+    import math
+
+    import numpy as np
+    import torch
+
+    import glass_amd
+    from glass_amd.batch import StackingBatch
+    from glass_amd.synthetic_gpu import render_stars_
+
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(args.seed + 500)
+    sizes = [(1024, 1024), (1536, 2048), (3072, 4096)]  # (height, width)
+
+    def out_maps(h, w):
+        o = {k: torch.empty((h, w), dtype=torch.float32, device=dev) for k in ("master", "weight")}
+        for k in ("coverage", "low", "high"):
+            o[k] = torch.empty((h, w), dtype=torch.int16, device=dev)
+        o["dq"] = torch.empty((h, w), dtype=torch.int32, device=dev)
+        return o
+
+    def render_job(spec, stack):
+        # frames are a function of the job's seed: re-rendered, they restore
+        # the stack's inputs exactly
+        h, w, f = spec["h"], spec["w"], spec["f"]
+        g = torch.Generator(device=dev).manual_seed(spec["seed"])
+        n_stars = 400
+        sx = torch.rand(n_stars, generator=g, device=dev) * w
+        sy = torch.rand(n_stars, generator=g, device=dev) * h
+        peaks = 10 ** (torch.rand(n_stars, generator=g, device=dev) * 1.5 - 1.0)
+        sig = (torch.rand(n_stars, generator=g, device=dev) * 2.0 + 2.0) / 2.3548
+        for k in range(f):
+            inv = np.linalg.inv(spec["mats"][k])
+            img = 0.02 + 0.01 * torch.randn((h, w), generator=g, device=dev)
+            render_stars_(img, (inv[0, 0] * sx + inv[0, 1] * sy + inv[0, 2]).float(),
+                          (inv[1, 0] * sx + inv[1, 1] * sy + inv[1, 2]).float(), peaks, sig, 8)
+            row = spec["masked"].get(k)
+            if row is not None:
+                img[row:row + 3, :] = math.nan
+            stack.upload_calibrated_frame(k, img)
+
+    jobs, specs, frame_px, n_zero_total, n_masked_total = [], [], 0, 0, 0
+    for j in range(args.batch_jobs):
+        h, w = sizes[int(rng.integers(3))]
+        f = int(rng.integers(8, 121))
+        interp = "lanczos3" if rng.random() < 0.5 else "bilinear"
+        mats = [np.eye(3)]
+        for _ in range(1, f):
+            a = math.radians(rng.uniform(-0.5, 0.5))
+            mats.append(np.array([[math.cos(a), -math.sin(a), rng.uniform(-20, 20)],
+                                  [math.sin(a), math.cos(a), rng.uniform(-20, 20)], [0.0, 0.0, 1.0]]))
+        masked_frames = rng.choice(np.arange(1, f), size=int(rng.integers(0, f // 10 + 1)), replace=False)
+        spec = {"h": h, "w": w, "f": f, "mats": mats, "seed": args.seed + 1000 + j,
+                "masked": {int(k): int(rng.integers(0, h - 3)) for k in masked_frames}}
+        stack = glass_amd.ResidentCalibratedStack(f, h, w)
+        render_job(spec, stack)
+        weights = np.ones(f, np.float32)
+        zero = rng.choice(np.arange(1, f), size=int(rng.integers(0, f // 10 + 1)), replace=False)
+        weights[zero] = 0.0
+        n_zero_total += zero.size
+        n_masked_total += len(spec["masked"])
+        frame_px += f * h * w
+        specs.append(spec)
+        jobs.append({"stack": stack, "warp_indices": list(range(1, f)), "matrices": mats[1:],
+                     "interpolation": interp, "semantics": "cpu", "fill": math.nan, "weights": weights,
+                     "outputs": out_maps(h, w),
+                     "dq_geometric": True, "dq_active_frames": int(np.count_nonzero(weights))})
+    torch.cuda.synchronize()
+    log(f"batched leg: {len(jobs)} jobs, {frame_px / 1e9:.2f} G frame-px generated")
+
+    stream = torch.cuda.Stream(device=dev)
+    sh = stream.cuda_stream
+    batch = StackingBatch(jobs, use_graph=True, autotune=True)
+    t0 = time.perf_counter()
+    batch.run(sh)
+    torch.cuda.synchronize()
+    first_s = time.perf_counter() - t0
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    reps = max(args.batch_reps, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for _ in range(reps):
+        batch.run(sh)
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    batched_ms = (time.perf_counter() - t0) * 1e3 / reps
+    batched_gpu_ms = ev[0].elapsed_time(ev[1]) / reps
+    info = batch.info()
+    log(f"batched: {batched_ms:.2f} ms per run (first run with autotune + capture {first_s:.2f} s), {info}")
+
+    def serial(outs):
+        for job, o in zip(jobs, outs):
+            s = job["stack"]
+            s.reset_warp_coverage()
+            s.accumulate_full_warp_coverage_frame()
+            s.warp_frames(job["warp_indices"], job["matrices"], job["interpolation"], "cpu", fill=job["fill"],
+                          stream=sh)
+            s.integrate_to_device(o, weights=job["weights"], dq_geometric=True,
+                                  dq_active_frames=job["dq_active_frames"], stream=sh)
+
+    # The serial form warps each stack in place (slot swap).  Its first run
+    # reads the frames the batch read (the batch leaves the stacks untouched)
+    # and is compared with it bit for bit; before the timed run the stacks are
+    # re-rendered (same scene and maps; the star scatter-add is not
+    # bit-reproducible, so that run is timed, not compared).
+    def restore():
+        for spec, job in zip(specs, jobs):
+            render_job(spec, job["stack"])
+        torch.cuda.synchronize()
+
+    serial_outs = [out_maps(j["stack"].height, j["stack"].width) for j in jobs]
+    serial(serial_outs)
+    torch.cuda.synchronize()
+    mismatched = []
+    for k, (job, o) in enumerate(zip(jobs, serial_outs)):
+        for name, t in job["outputs"].items():
+            u = o[name]
+            same = (t == u) | (t.isnan() & u.isnan()) if t.is_floating_point() else (t == u)
+            if not bool(same.all()):
+                mismatched.append((k, name))
+    restore()
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    serial(serial_outs)
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    serial_ms = (time.perf_counter() - t0) * 1e3
+    serial_gpu_ms = ev[0].elapsed_time(ev[1])
+    log(f"serial: {serial_ms:.2f} ms per run; mismatched maps vs batched: {len(mismatched)}")
+    batch.close()
+    for job in jobs:
+        job["stack"].close()
+    del jobs, serial_outs
+    torch.cuda.empty_cache()
+    return {
+        "workload": f"{args.batch_jobs} jobs, F~U{{8..120}}, sizes 1024x1024/2048x1536/4096x3072, "
+                    "mixed bilinear/Lanczos3, 0-10% zero-weight + 0-10% masked frames, winsorized 3/3 + maps",
+        "value": round(frame_px / 1e6 / (batched_ms * 1e-3), 1),
+        "unit": "frame-Mpix/s",
+        "ms_per_run": round(batched_ms, 3),
+        "gpu_ms_per_run": round(batched_gpu_ms, 3),
+        "first_run_s": round(first_s, 3),
+        "serial_value": round(frame_px / 1e6 / (serial_ms * 1e-3), 1),
+        "serial_ms_per_run": round(serial_ms, 3),
+        "serial_gpu_ms_per_run": round(serial_gpu_ms, 3),
+        "speedup_vs_serial": round(serial_ms / batched_ms, 3),
+        "frame_gpx": round(frame_px / 1e9, 3),
+        "zero_weight_frames": int(n_zero_total),
+        "masked_frames": int(n_masked_total),
+        "maps_bit_exact_vs_serial": not mismatched,
+        "schedule": info,
+    }
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -226,7 +394,9 @@ def main():
         "high": torch.empty((h, w), dtype=torch.int16, device=dev),
         "dq": torch.empty((h, w), dtype=torch.int32, device=dev),
     }
-    stream = torch.cuda.current_stream(dev)
+    # a non-null stream: every kernel of the step is launched on it and the
+    # HIP events below are recorded on it
+    stream = torch.cuda.Stream(device=dev)
     sh = stream.cuda_stream
     policy = {"master_dark_includes_bias": True}
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
@@ -328,6 +498,9 @@ def main():
     if not args.no_refine:
         log("refined-registration leg (config 4)")
         line["refined_registration"] = refine_leg(args)
+    if not args.no_batch:
+        log("batched multi-job leg (config 3)")
+        line["batched"] = batched_leg(args)
     if not args.no_cpu_baseline:
         log("timing the CPU baseline sample")
         line["cpu_baseline"] = cpu_baseline(args, light_exp, dark_exp)

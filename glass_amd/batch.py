@@ -6,7 +6,7 @@ dicts, each naming a loaded ``ResidentCalibratedStack`` and what to do with it:
     {"stack": stack,                       # frames already calibrated
      "warp_indices": [...], "matrices": [...],   # moving -> reference, affine
      "interpolation": "lanczos3" | "bilinear",
-     "semantics": "cpu" | "resident", "fill": 0.0, "clamping_threshold": -1.0,
+     "semantics": "cpu" | "resident", "fill": nan, "clamping_threshold": -1.0,
      "weights": None | array[frame_count],
      "low_sigma": 3.0, "high_sigma": 3.0, "min_samples": 3,
      "max_reject_fraction": 0.5, "rejection": "winsorized_sigma" | "none",
@@ -14,10 +14,11 @@ dicts, each naming a loaded ``ResidentCalibratedStack`` and what to do with it:
      "outputs": {"master": t, "weight": t, "coverage": t, "low": t, "high": t, "dq": t},
      "dq_geometric": False, "dq_active_frames": 0}
 
-``run(stream=None)`` executes every job: one persistent batched warp launch
-over all jobs' (frame batch, tile) items and one persistent integration launch
-per sorting-network capacity class, captured into a hipGraph on the first run
-(after the autotuner has timed candidate schedules) and replayed afterwards.
+``run(stream=None)`` executes every job: one batched warp launch over all
+jobs' (frame batch, tile) items, one launch deciding each job's statistics
+arithmetic from the warped frames' non-finite flags, and one integration launch
+per job reading that decision -- captured into a hipGraph on the first run
+(after the autotuner has timed candidate warp schedules) and replayed.
 Outputs equal the per-job ``warp_frames`` + ``integrate_to_device`` calls;
 the jobs' stacks are read, not modified.
 """
@@ -25,6 +26,7 @@ the jobs' stacks are read, not modified.
 from __future__ import annotations
 
 import ctypes
+import math
 
 import numpy as np
 
@@ -74,7 +76,7 @@ def _job_struct(job, keep):
                            int(job.get("dq_active_frames", 0)))
     return N.BatchJob(stack._h, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                       inv.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), idx.size, interp, flags,
-                      float(job.get("fill", 0.0)), float(job.get("clamping_threshold", -1.0)), wptr, params, o)
+                      float(job.get("fill", math.nan)), float(job.get("clamping_threshold", -1.0)), wptr, params, o)
 
 
 class StackingBatch:
@@ -88,18 +90,20 @@ class StackingBatch:
         self._keep = [j["stack"] for j in jobs] + [j.get("outputs", {}) for j in jobs]
 
     def run(self, stream=None) -> None:
-        """Enqueue one execution on `stream` (default: torch's current stream)."""
+        """Enqueue one execution on `stream` (default: torch's current stream).
+        A null stream handle (torch's default stream) runs the batch on its
+        own stream and returns when it has finished."""
         if stream is None:
             import torch
 
             stream = torch.cuda.current_stream().cuda_stream
-        N.call("gh_batch_run", self._h, ctypes.c_void_p(stream))
+        N.call("gh_batch_run", self._h, ctypes.c_void_p(stream or None))
 
     def info(self) -> dict:
         out = (ctypes.c_int64 * 8)()
         N.call("gh_batch_info", self._h, out)
         keys = ("jobs", "warp_items", "integrate_launches", "graph", "warp_grid", "warp_item_stride",
-                "integrate_grid", "autotune_trials")
+                "device_variant_jobs", "autotune_trials")
         return {k: int(v) for k, v in zip(keys, out)}
 
     def close(self) -> None:

@@ -1,8 +1,12 @@
-// Batched multi-job stacking (gh_batch_*): many resident jobs as one
-// persistent warp launch plus one persistent integration launch per capacity
-// class, optionally captured into a hipGraph, with a small autotuner that
-// times candidate schedules (persistent grid sizes, work-item order) on the
-// first run and caches the choice per batch shape for the process.
+// Batched multi-job stacking (gh_batch_*): the warps of many resident jobs
+// run as one launch over (job, frame batch, output tile) work items; a small
+// kernel then decides each job's statistics arithmetic from the frames'
+// non-finite flags, and each job integrates with the per-job winsorized
+// kernel reading that decision from device memory, so the whole sequence
+// needs no host round trip and can be captured into a hipGraph once and
+// replayed.  An autotuner times candidate warp schedules (grid size,
+// work-item order) on the first run and caches the choice per batch shape
+// for the process.
 
 #include <chrono>
 #include <map>
@@ -20,7 +24,6 @@ namespace {
 struct Schedule {
   unsigned warp_grid = 0;
   int warp_stride = 1;
-  std::vector<unsigned> int_grid;  // per integration class
 };
 
 std::mutex g_tune_mutex;
@@ -43,24 +46,18 @@ struct gh_batch {
     float* warped = nullptr;          // n_warp frames, owned
     float* coverage = nullptr;        // owned, when geometric coverage is needed
     int n_unwarped = 0;               // frames adding full coverage
-    bool mean_only = false;           // rejection none: per-job mean kernel
     gh::IntegrateArgs args{};
     DeviceBuffer<const float*> table;
     DeviceBuffer<float> weights;
     DeviceBuffer<const unsigned*> flag_ptrs;
   };
-  struct Class {
-    int capacity = 0;
-    bool unit = false;
-    std::vector<int> jobs;
-    long long total_blocks = 0;
-    DeviceBuffer<gh::IntegrateArgs> args;
-    DeviceBuffer<long long> first_block;
-  };
   std::vector<Job> jobs;
-  std::vector<Class> classes;
   DeviceBuffer<gh::BatchWarpJob> warp_jobs;
   DeviceBuffer<gh::BatchWarpFrame> warp_frames;
+  DeviceBuffer<unsigned short> item_job;   // warp work item -> warp job
+  DeviceBuffer<gh::VariantJob> variant_jobs;
+  DeviceBuffer<int> variants;              // per variant job, written on the device
+  int n_variant_jobs = 0;
   DeviceBuffer<unsigned> warped_flags;
   DeviceBuffer<float> lut;
   int n_warp_jobs = 0;
@@ -86,12 +83,11 @@ struct gh_batch {
       j.weights.release();
       j.flag_ptrs.release();
     }
-    for (Class& c : classes) {
-      c.args.release();
-      c.first_block.release();
-    }
     warp_jobs.release();
     warp_frames.release();
+    item_job.release();
+    variant_jobs.release();
+    variants.release();
     warped_flags.release();
     lut.release();
     if (stream) (void)hipStreamDestroy(stream);
@@ -107,69 +103,64 @@ struct gh_batch {
       gh::launch_fill(j.coverage, j.stack->npix, static_cast<float>(j.n_unwarped), st);
     }
     if (total_items) {
-      gh::launch_warp_batched(warp_jobs.ptr, n_warp_jobs, warp_frames.ptr, total_items,
+      gh::launch_warp_batched(warp_jobs.ptr, item_job.ptr, warp_frames.ptr, total_items,
                               reinterpret_cast<const float4*>(lut.ptr), sc.warp_grid, sc.warp_stride, st);
     }
-    for (size_t c = 0; c < classes.size(); ++c) {
-      const Class& k = classes[c];
-      gh::launch_integrate_batched(k.capacity, k.unit, k.args.ptr, k.first_block.ptr, static_cast<int>(k.jobs.size()),
-                                   k.total_blocks, sc.int_grid[c], st);
-    }
-    for (Job& j : jobs) {
-      if (j.mean_only) gh::launch_integrate(j.args, st);
-    }
+    gh::launch_decide_variants(variant_jobs.ptr, n_variant_jobs, variants.ptr, st);
+    for (Job& j : jobs) gh::launch_integrate(j.args, st);
   }
 
-  float time_ms(const Schedule& sc) {
+  // Device time of `launch` on the batch's stream (best of two runs).
+  template <typename F>
+  float time_launch(F&& launch) {
     hipEvent_t a, b;
     check(hipEventCreate(&a), "hipEventCreate");
     check(hipEventCreate(&b), "hipEventCreate");
-    check(hipEventRecord(a, stream), "hipEventRecord");
-    enqueue(stream, sc);
-    check(hipEventRecord(b, stream), "hipEventRecord");
-    check(hipEventSynchronize(b), "hipEventSynchronize");
-    float ms = 0.0f;
-    check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+    float best = 0.0f;
+    for (int rep = 0; rep < 2; ++rep) {
+      check(hipEventRecord(a, stream), "hipEventRecord");
+      launch();
+      check(hipEventRecord(b, stream), "hipEventRecord");
+      check(hipEventSynchronize(b), "hipEventSynchronize");
+      float ms = 0.0f;
+      check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+      best = rep == 0 ? ms : std::min(best, ms);
+    }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     ++trials;
-    return ms;
+    return best;
   }
 
-  // Coordinate descent over the warp schedule, then each class's grid.
+  // The warp launch is tuned on its own timing over its grid (persistent
+  // multiples of the resident-block count, or one workgroup per item) and
+  // item order.  Tuning runs write the batch's own buffers; every real run
+  // starts by clearing them.
   void autotune() {
-    const unsigned base_w = gh::warp_batched_max_blocks();
-    Schedule best = sched;
-    float best_ms = time_ms(best);  // also the warm-up
-    best_ms = std::min(best_ms, time_ms(best));
-    auto try_sched = [&](const Schedule& cand) {
-      const float ms = std::min(time_ms(cand), time_ms(cand));
-      if (ms < best_ms) {
-        best_ms = ms;
-        best = cand;
-      }
-    };
-    for (unsigned mult : {1u, 2u, 4u}) {
-      for (int stride : {1, coprime_stride(total_items)}) {
-        Schedule c = best;
-        c.warp_grid = base_w * mult;
-        c.warp_stride = stride;
-        if (c.warp_grid == best.warp_grid && c.warp_stride == best.warp_stride) continue;
-        try_sched(c);
-      }
-    }
-    int dev = 0, cus = 1;
-    check(hipGetDevice(&dev), "hipGetDevice");
-    check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "device CU count");
-    for (size_t k = 0; k < classes.size(); ++k) {
-      for (unsigned mult : {2u, 4u, 8u}) {
-        Schedule c = best;
-        c.int_grid[k] = static_cast<unsigned>(cus) * mult;
-        if (c.int_grid[k] == best.int_grid[k]) continue;
-        try_sched(c);
+    enqueue(stream, sched);  // warm-up
+    check(hipStreamSynchronize(stream), "synchronize(batch)");
+    if (total_items) {
+      const unsigned base_w = gh::warp_batched_max_blocks();
+      auto warp_time = [&](unsigned grid, int stride) {
+        return time_launch([&] {
+          gh::launch_warp_batched(warp_jobs.ptr, item_job.ptr, warp_frames.ptr, total_items,
+                                  reinterpret_cast<const float4*>(lut.ptr), grid, stride, stream);
+        });
+      };
+      float best = warp_time(sched.warp_grid, sched.warp_stride);
+      const unsigned all_items = static_cast<unsigned>(std::min<long long>(total_items, 0x7fffffffLL));
+      for (unsigned grid : {base_w, base_w * 2u, base_w * 4u, base_w * 8u, all_items}) {
+        for (int stride : {1, coprime_stride(total_items)}) {
+          if (grid == sched.warp_grid && stride == sched.warp_stride) continue;
+          const float ms = warp_time(grid, stride);
+          if (ms < best) {
+            best = ms;
+            sched.warp_grid = grid;
+            sched.warp_stride = stride;
+          }
+        }
       }
     }
-    sched = best;
   }
 };
 
@@ -201,7 +192,8 @@ int gh_batch_create(const gh_batch_job* jobs, int64_t n_jobs, uint32_t flags, gh
     std::vector<gh::BatchWarpFrame> wframes;
     size_t flag_base = 0;
     b->jobs.resize(static_cast<size_t>(n_jobs));
-    std::map<std::pair<int, bool>, int> class_of;
+    std::vector<gh::VariantJob> vjobs;
+    std::vector<size_t> vjob_of;  // job index of each variant job
     for (int64_t j = 0; j < n_jobs; ++j) {
       const gh_batch_job& in = jobs[j];
       gh_batch::Job& job = b->jobs[static_cast<size_t>(j)];
@@ -315,11 +307,12 @@ int gh_batch_create(const gh_batch_job* jobs, int64_t n_jobs, uint32_t flags, gh
       a.max_frac = prm.max_reject_fraction;
       a.unit_weights = unit ? 1 : 0;
       a.require_positive_scale = prm.require_positive_scale;
-      a.all_weights_positive = (n_active == s->frames) ? 1 : 0;
-      if (prm.statistics_variant >= 0) {
-        a.variant_all_valid = prm.statistics_variant;
+      if (prm.statistics_variant >= 0 || prm.rejection == GH_REJECT_NONE) {
+        a.variant_all_valid = std::max(prm.statistics_variant, 0);  // unused by the mean kernel
       } else {
-        a.active_flags = job.flag_ptrs.ptr;
+        // decided on the device after the warp (gh_integrate's host rule)
+        vjobs.push_back(gh::VariantJob{job.flag_ptrs.ptr, n_active, n_active == s->frames ? 1 : 0});
+        vjob_of.push_back(static_cast<size_t>(j));
       }
       const gh_integrate_outputs& o = in.outputs;
       for (const void* ptr : {static_cast<const void*>(o.master), static_cast<const void*>(o.weight),
@@ -336,20 +329,6 @@ int gh_batch_create(const gh_batch_job* jobs, int64_t n_jobs, uint32_t flags, gh
       a.count_u16 = (prm.count_dtype == GH_COUNT_U16) ? 1 : 0;
       a.geometric = need_cov ? job.coverage : nullptr;
       a.dq_active_frames = o.dq_active_frames;
-      if (prm.rejection == GH_REJECT_NONE) {
-        job.mean_only = true;
-        if (a.active_flags) a.variant_all_valid = 0;  // unused by the mean kernel
-      } else {
-        const std::pair<int, bool> key{gh::integrate_capacity(n_active), unit};
-        auto it = class_of.find(key);
-        if (it == class_of.end()) {
-          it = class_of.emplace(key, static_cast<int>(b->classes.size())).first;
-          b->classes.emplace_back();
-          b->classes.back().capacity = key.first;
-          b->classes.back().unit = key.second;
-        }
-        b->classes[it->second].jobs.push_back(static_cast<int>(j));
-      }
       sig << s->height << 'x' << s->width << ':' << in.n_warp << ':' << in.interpolation << ':' << n_active << ';';
     }
     // Device job tables.
@@ -360,7 +339,18 @@ int gh_batch_create(const gh_batch_job* jobs, int64_t n_jobs, uint32_t flags, gh
     }
     b->total_items = items;
     b->n_warp_jobs = static_cast<int>(wjobs.size());
+    if (wjobs.size() > 65535) throw gh::ValueError("a batch holds at most 65535 warped jobs");
     if (!wjobs.empty()) {
+      // Work item -> warp job, so a workgroup finds its job with one load.
+      std::vector<unsigned short> item_job(static_cast<size_t>(items));
+      for (size_t k = 0; k < wjobs.size(); ++k) {
+        const long long end = k + 1 < wjobs.size() ? wjobs[k + 1].first_item : items;
+        std::fill(item_job.begin() + wjobs[k].first_item, item_job.begin() + end, static_cast<unsigned short>(k));
+      }
+      b->item_job.ensure(item_job.size(), "hipMalloc(batch item table)");
+      check(hipMemcpy(b->item_job.ptr, item_job.data(), item_job.size() * sizeof(unsigned short),
+                      hipMemcpyHostToDevice),
+            "hipMemcpy(batch item table)");
       b->warp_jobs.ensure(wjobs.size(), "hipMalloc(batch warp jobs)");
       b->warp_frames.ensure(wframes.size(), "hipMalloc(batch warp frames)");
       check(hipMemcpy(b->warp_jobs.ptr, wjobs.data(), wjobs.size() * sizeof(gh::BatchWarpJob), hipMemcpyHostToDevice),
@@ -369,29 +359,17 @@ int gh_batch_create(const gh_batch_job* jobs, int64_t n_jobs, uint32_t flags, gh
                       hipMemcpyHostToDevice),
             "hipMemcpy(batch warp frames)");
     }
-    int dev = 0, cus = 1;
-    check(hipGetDevice(&dev), "hipGetDevice");
-    check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "device CU count");
+    b->n_variant_jobs = static_cast<int>(vjobs.size());
+    if (!vjobs.empty()) {
+      b->variant_jobs.ensure(vjobs.size(), "hipMalloc(batch variant jobs)");
+      b->variants.ensure(vjobs.size(), "hipMalloc(batch variants)");
+      check(hipMemcpy(b->variant_jobs.ptr, vjobs.data(), vjobs.size() * sizeof(gh::VariantJob),
+                      hipMemcpyHostToDevice),
+            "hipMemcpy(batch variant jobs)");
+      for (size_t k = 0; k < vjob_of.size(); ++k) b->jobs[vjob_of[k]].args.variant_ptr = b->variants.ptr + k;
+    }
     b->sched.warp_grid = gh::warp_batched_max_blocks();
     b->sched.warp_stride = 1;
-    for (gh_batch::Class& c : b->classes) {
-      std::vector<gh::IntegrateArgs> args;
-      std::vector<long long> first;
-      long long blocks = 0;
-      for (int j : c.jobs) {
-        args.push_back(b->jobs[static_cast<size_t>(j)].args);
-        first.push_back(blocks);
-        blocks += gh::integrate_blocks(b->jobs[static_cast<size_t>(j)].args.npix);
-      }
-      c.total_blocks = blocks;
-      c.args.ensure(args.size(), "hipMalloc(batch integrate jobs)");
-      c.first_block.ensure(first.size(), "hipMalloc(batch integrate offsets)");
-      check(hipMemcpy(c.args.ptr, args.data(), args.size() * sizeof(gh::IntegrateArgs), hipMemcpyHostToDevice),
-            "hipMemcpy(batch integrate jobs)");
-      check(hipMemcpy(c.first_block.ptr, first.data(), first.size() * sizeof(long long), hipMemcpyHostToDevice),
-            "hipMemcpy(batch integrate offsets)");
-      b->sched.int_grid.push_back(static_cast<unsigned>(cus) * 4u);
-    }
     b->signature = sig.str();
     *out = b.release();
   });
@@ -410,7 +388,7 @@ int gh_batch_run(gh_batch* b, void* stream_ptr) {
         {
           std::lock_guard<std::mutex> lock(g_tune_mutex);
           auto it = g_tune_cache.find(b->signature);
-          if (it != g_tune_cache.end() && it->second.int_grid.size() == b->sched.int_grid.size()) {
+          if (it != g_tune_cache.end()) {
             b->sched = it->second;
             cached = true;
           }
@@ -443,6 +421,7 @@ int gh_batch_run(gh_batch* b, void* stream_ptr) {
     } else {
       b->enqueue(st, b->sched);
     }
+    if (!stream_ptr) check(hipStreamSynchronize(st), "synchronize(batch)");
   });
 }
 
@@ -451,13 +430,11 @@ int gh_batch_info(gh_batch* b, int64_t* out) {
     if (!b || !out) throw gh::ValueError("argument is NULL");
     out[0] = static_cast<int64_t>(b->jobs.size());
     out[1] = b->total_items;
-    int64_t launches = static_cast<int64_t>(b->classes.size());
-    for (const auto& j : b->jobs) launches += j.mean_only ? 1 : 0;
-    out[2] = launches;
+    out[2] = static_cast<int64_t>(b->jobs.size());  // one integration launch per job
     out[3] = b->exec ? 1 : 0;
     out[4] = b->sched.warp_grid;
     out[5] = b->sched.warp_stride;
-    out[6] = b->sched.int_grid.empty() ? 0 : b->sched.int_grid[0];
+    out[6] = b->n_variant_jobs;
     out[7] = b->trials;
   });
 }

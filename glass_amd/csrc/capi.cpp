@@ -294,12 +294,14 @@ int gh_warp_frames(gh_stack* s, const int64_t* indices, int64_t count, const dou
       }
       if (flags & GH_WARP_TRACK_COVERAGE) s->warp_coverage_frames += n;
     }
+    s->mark_caller(st);
   });
 }
 
 int gh_reset_warp_coverage(gh_stack* s) {
   return guarded([&] {
     s->ensure_geometric();
+    s->wait_caller();
     gh::launch_fill(s->geometric, s->npix, 0.0f, s->stream);
     s->warp_coverage_frames = 0;
     check(hipStreamSynchronize(s->stream), "reset_warp_coverage");
@@ -309,6 +311,7 @@ int gh_reset_warp_coverage(gh_stack* s) {
 int gh_accumulate_full_warp_coverage(gh_stack* s) {
   return guarded([&] {
     s->ensure_geometric();
+    s->wait_caller();
     gh::launch_add_scalar(s->geometric, s->npix, 1.0f, s->stream);
     ++s->warp_coverage_frames;
     check(hipStreamSynchronize(s->stream), "accumulate_full_warp_coverage_frame");
@@ -318,6 +321,7 @@ int gh_accumulate_full_warp_coverage(gh_stack* s) {
 int gh_warp_coverage_map(gh_stack* s, float* out) {
   return guarded([&] {
     if (!out) throw gh::ValueError("output pointer is NULL");
+    s->wait_caller();
     check(hipStreamSynchronize(s->stream), "synchronize(warp coverage)");
     if (!s->geometric) {
       if (is_device_pointer(out)) {
@@ -464,6 +468,7 @@ int gh_integrate(gh_stack* s, const float* weights, const gh_integrate_params* p
       check(hipMemcpyAsync(host_dst[i], dev_src[i], host_bytes[i], hipMemcpyDeviceToHost, st),
             "hipMemcpy(integration output)");
     }
+    s->mark_caller(st);
     if (any_host || !stream_ptr) check(hipStreamSynchronize(st), "integrate synchronize");
   });
 }

@@ -27,6 +27,46 @@ inline void check(hipError_t err, const char* what) {
 
 constexpr int kBlock = 256;
 
+// Wave-uniform reads of device descriptor tables.  A descriptor read from
+// memory (rather than passed as a kernel argument) holds generic pointers;
+// loads through generic pointers are flat loads, which the compiler treats as
+// lane-varying, so every address derived from them would live in VGPR pairs.
+// Reading the table through the global address space and broadcasting the
+// first active lane's words marks the values wave-uniform: pointers become
+// scalar (SGPR) bases.  Valid only where every lane reads the same entry.
+typedef __attribute__((address_space(1))) const unsigned global_u32;
+typedef __attribute__((address_space(1))) const unsigned long long global_u64;
+
+typedef __attribute__((address_space(1))) float global_f32;
+typedef __attribute__((address_space(1))) const float global_cf32;
+
+// Frame memory is global: these accessors keep loads and stores off the flat
+// path when the frame pointer was read from a descriptor table.
+__device__ __forceinline__ float ld_global(const float* base, long long i) { return ((const global_cf32*)base)[i]; }
+__device__ __forceinline__ void st_global(float* base, long long i, float v) { ((global_f32*)base)[i] = v; }
+
+template <typename T>
+__device__ __forceinline__ T load_uniform(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "descriptor must be whole 32-bit words");
+  constexpr int kWords = static_cast<int>(sizeof(T) / 4);
+  unsigned w[kWords];
+  const global_u32* src = (const global_u32*)p;
+#pragma unroll
+  for (int i = 0; i < kWords; ++i) w[i] = __builtin_amdgcn_readfirstlane(src[i]);
+  T out;
+  __builtin_memcpy(&out, w, sizeof(T));
+  return out;
+}
+
+// Entry k of a device table of pointers, as a wave-uniform pointer.
+template <typename T>
+__device__ __forceinline__ T* table_at(T* const* table, long long k) {
+  const unsigned long long v = ((const global_u64*)table)[k];
+  const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(v));
+  const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(v >> 32));
+  return reinterpret_cast<T*>((static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
 inline unsigned grid_for(long long n, int per_block) {
   long long g = (n + per_block - 1) / per_block;
   if (g < 1) g = 1;

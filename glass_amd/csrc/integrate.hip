@@ -220,10 +220,11 @@ __device__ __forceinline__ float nan_to_num(float x) {
 }
 
 __device__ __forceinline__ void store_count(void* base, long long p, int count_u16, float value) {
+  typedef __attribute__((address_space(1))) unsigned short global_u16;
   if (count_u16) {
-    reinterpret_cast<unsigned short*>(base)[p] = static_cast<unsigned short>(value);
+    reinterpret_cast<global_u16*>(reinterpret_cast<__UINTPTR_TYPE__>(base))[p] = static_cast<unsigned short>(value);
   } else {
-    reinterpret_cast<float*>(base)[p] = value;
+    st_global(reinterpret_cast<float*>(base), p, value);
   }
 }
 
@@ -238,7 +239,7 @@ __device__ __forceinline__ unsigned dq_flags(const IntegrateArgs& a, long long p
     invalid = true;
   }
   if (a.geometric) {
-    const float g = a.geometric[p];
+    const float g = ld_global(a.geometric, p);
     if (!(isfinite(g) && g > 0.5f)) {
       dq |= kWarpEdge;
       invalid = true;
@@ -254,12 +255,13 @@ __device__ __forceinline__ unsigned dq_flags(const IntegrateArgs& a, long long p
 
 __device__ __forceinline__ void write_outputs(const IntegrateArgs& a, long long p, float master, float weight,
                                               float coverage, float low, float high) {
-  if (a.master) a.master[p] = master;
-  if (a.weight) a.weight[p] = weight;
+  if (a.master) st_global(a.master, p, master);
+  if (a.weight) st_global(a.weight, p, weight);
   if (a.coverage) store_count(a.coverage, p, a.count_u16, coverage);
   if (a.low) store_count(a.low, p, a.count_u16, low);
   if (a.high) store_count(a.high, p, a.count_u16, high);
-  if (a.dq) a.dq[p] = dq_flags(a, p, master, weight, coverage, low, high);
+  typedef __attribute__((address_space(1))) unsigned global_dq;
+  if (a.dq) ((global_dq*)a.dq)[p] = dq_flags(a, p, master, weight, coverage, low, high);
 }
 
 // Population standard deviation of the valid samples of pixel p, FP64 sums in
@@ -270,14 +272,14 @@ __device__ __attribute__((noinline)) float frame_axis_std(const IntegrateArgs& a
   const unsigned off = static_cast<unsigned>(p) * 4u;
   double s = 0.0;
   for (int k = 0; k < a.n_active; ++k) {
-    const float x = load_global_off(a.frame_ptr[k], off);
+    const float x = load_global_off(table_at(a.frame_ptr, k), off);
     s += static_cast<double>(isfinite(x) ? x : 0.0f);
   }
   double mean = s / n;
   if (!all_valid) mean = static_cast<double>(static_cast<float>(mean));
   double ss = 0.0;
   for (int k = 0; k < a.n_active; ++k) {
-    const float x = load_global_off(a.frame_ptr[k], off);
+    const float x = load_global_off(table_at(a.frame_ptr, k), off);
     const double d = isfinite(x) ? static_cast<double>(x) - mean : 0.0;
     ss += d * d;
   }
@@ -306,7 +308,7 @@ __device__ __forceinline__ void winsorized_pixel(const IntegrateArgs& a, long lo
   static_for<CAP>([&](auto K) GH_ALWAYS_INLINE {
     constexpr int k = decltype(K)::value;
     if constexpr (k % 16 == 0) asm volatile("" : "+s"(zero));
-    v.template set<k>(load_global_off(a.frame_ptr[k + zero], byte_offset));
+    v.template set<k>(load_global_off(table_at(a.frame_ptr, k + zero), byte_offset));
   });
   // Invalid samples add +0.0, as the oracle's np.where(valid, x, 0.0) * 0.0 does.
   static_for<CAP>([&](auto K) GH_ALWAYS_INLINE {
@@ -318,7 +320,7 @@ __device__ __forceinline__ void winsorized_pixel(const IntegrateArgs& a, long lo
     if constexpr (UNIT) {
       swx += static_cast<double>(ok ? x : 0.0f);
     } else {
-      const double w = static_cast<double>(a.frame_w[src]);
+      const double w = static_cast<double>(ld_global(a.frame_w, src));
       swx += ok ? static_cast<double>(x) * w : 0.0;
       sw += ok ? w : 0.0;
     }
@@ -338,7 +340,7 @@ __device__ __forceinline__ void winsorized_pixel(const IntegrateArgs& a, long lo
 
   // Per-tile choice of the statistics arithmetic (CPUStackEngine tiles) or
   // one choice for the whole frame (weighted_integrate_stack).
-  bool all_valid = a.variant_all_valid != 0;
+  bool all_valid = (a.variant_ptr ? *(const global_u32*)a.variant_ptr : static_cast<unsigned>(a.variant_all_valid)) != 0u;
   if (a.tile_flags) {
     const int y = static_cast<int>(pp / a.width);
     const int x = static_cast<int>(pp - static_cast<long long>(y) * a.width);
@@ -469,9 +471,9 @@ __device__ __forceinline__ void winsorized_pixel(const IntegrateArgs& a, long lo
     // Survivors re-summed in frame order with their weights.
     double s = 0.0, wsum = 0.0;
     for (int k = 0; k < a.n_active; ++k) {
-      const float x = load_global(a.frame_ptr[k], pp);
+      const float x = load_global(table_at(a.frame_ptr, k), pp);
       if (isfinite(x) && x >= lo_t && x <= hi_t) {
-        const double w = static_cast<double>(a.frame_w[k]);
+        const double w = static_cast<double>(ld_global(a.frame_w, k));
         s += static_cast<double>(x) * w;
         wsum += w;
       }
@@ -494,33 +496,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CAP > 20
   winsorized_pixel<CAP, UNIT>(a, static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x);
 }
 
-// Persistent kernel over the pixel blocks of many jobs of one capacity class:
-// virtual block vb belongs to the job whose [first_block, next first_block)
-// range holds it.  When a job carries active_flags, its statistics arithmetic
-// (all-valid vs valid, rejection.py:39-44) is decided here from the frames'
-// non-finite flags, so the launch needs no host decision between the warp and
-// the integration.
-template <int CAP, bool UNIT>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CAP > 208 ? 1 : 2, 2))) void integrate_batched_kernel(
-    const IntegrateArgs* __restrict__ jobs, const long long* __restrict__ first_block, int n_jobs,
-    long long total_blocks) {
-  for (long long vb = blockIdx.x; vb < total_blocks; vb += gridDim.x) {
-    int lo = 0, hi = n_jobs - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (first_block[mid] <= vb) lo = mid;
-      else hi = mid - 1;
-    }
-    IntegrateArgs a = jobs[lo];
-    if (a.active_flags) {
-      unsigned any = 0u;
-      for (int k = 0; k < a.n_active; ++k) any |= *a.active_flags[k];
-      a.variant_all_valid = (a.all_weights_positive && any == 0u) ? 1 : 0;
-    }
-    winsorized_pixel<CAP, UNIT>(a, (vb - first_block[lo]) * kBlock + threadIdx.x);
-  }
-}
-
 // Rejection-free weighted mean for any frame count (integrate_mean,
 // cpp/cuda/integration_kernels.cu:1610-1656; oracle cpu/integration.py:182-194).
 template <bool UNIT>
@@ -530,14 +505,14 @@ __global__ __launch_bounds__(kBlock) void integrate_mean_kernel(IntegrateArgs a)
   double swx = 0.0, sw = 0.0;
   int cnt = 0;
   for (int k = 0; k < a.n_active; ++k) {
-    const float x = load_global(a.frame_ptr[k], p);
+    const float x = load_global(table_at(a.frame_ptr, k), p);
     if (isfinite(x)) {
       ++cnt;
       if constexpr (UNIT) {
         swx += static_cast<double>(x);
         sw += 1.0;
       } else {
-        const double w = static_cast<double>(a.frame_w[k]);
+        const double w = static_cast<double>(ld_global(a.frame_w, k));
         swx += static_cast<double>(x) * w;
         sw += w;
       }
@@ -545,6 +520,17 @@ __global__ __launch_bounds__(kBlock) void integrate_mean_kernel(IntegrateArgs a)
   }
   const float master = (sw > 0.0) ? static_cast<float>(swx / sw) : 0.0f;
   write_outputs(a, p, master, static_cast<float>(sw), static_cast<float>(cnt), 0.0f, 0.0f);
+}
+
+// One wave per job: OR of the job's frame flags -> statistics variant
+// (rejection.py:39-44: np.all(valid) with positive weights selects the
+// all-valid arithmetic).
+__global__ __launch_bounds__(64) void decide_variants_kernel(const VariantJob* __restrict__ jobs, int* out) {
+  const VariantJob v = load_uniform(jobs + blockIdx.x);
+  unsigned any = 0u;
+  for (int k = threadIdx.x; k < v.n_flags; k += 64) any |= *(const global_u32*)v.flags[k];  // lanes differ
+  const bool nonfinite = __any(any != 0u);
+  if (threadIdx.x == 0) out[blockIdx.x] = (v.weights_positive && !nonfinite) ? 1 : 0;
 }
 
 __global__ __launch_bounds__(kBlock) void tile_all_finite_kernel(const float* const* frame_ptr, int n_active,
@@ -578,41 +564,14 @@ void launch_cap(const IntegrateArgs& a, hipStream_t stream) {
   }
 }
 
-template <int CAP>
-void launch_batched_cap(const IntegrateArgs* jobs, const long long* first_block, int n_jobs, long long total_blocks,
-                        bool unit, unsigned grid, hipStream_t stream) {
-  const unsigned g = static_cast<unsigned>(std::min<long long>(std::max(grid, 1u), total_blocks));
-  if (unit) {
-    integrate_batched_kernel<CAP, true><<<g, kBlock, 0, stream>>>(jobs, first_block, n_jobs, total_blocks);
-  } else {
-    integrate_batched_kernel<CAP, false><<<g, kBlock, 0, stream>>>(jobs, first_block, n_jobs, total_blocks);
-  }
-}
-
 }  // namespace
 
 int integrate_max_frames() { return 256; }
 
-int integrate_capacity(int n_active) {
-  return n_active <= 16 ? 16 : n_active <= 32 ? 32 : n_active <= 64 ? 64 : n_active <= 128 ? 128
-         : n_active <= 208 ? 208 : 256;
-}
-
-long long integrate_blocks(long long npix) { return (npix + kBlock - 1) / kBlock; }
-
-void launch_integrate_batched(int capacity, bool unit, const IntegrateArgs* jobs, const long long* first_block,
-                              int n_jobs, long long total_blocks, unsigned grid, hipStream_t stream) {
-  if (total_blocks <= 0) return;
-  switch (capacity) {
-    case 16: launch_batched_cap<16>(jobs, first_block, n_jobs, total_blocks, unit, grid, stream); break;
-    case 32: launch_batched_cap<32>(jobs, first_block, n_jobs, total_blocks, unit, grid, stream); break;
-    case 64: launch_batched_cap<64>(jobs, first_block, n_jobs, total_blocks, unit, grid, stream); break;
-    case 128: launch_batched_cap<128>(jobs, first_block, n_jobs, total_blocks, unit, grid, stream); break;
-    case 208: launch_batched_cap<208>(jobs, first_block, n_jobs, total_blocks, unit, grid, stream); break;
-    case 256: launch_batched_cap<256>(jobs, first_block, n_jobs, total_blocks, unit, grid, stream); break;
-    default: throw ValueError("unsupported integration capacity");
-  }
-  check(hipGetLastError(), "batched integrate launch");
+void launch_decide_variants(const VariantJob* jobs, int n_jobs, int* out, hipStream_t stream) {
+  if (n_jobs <= 0) return;
+  decide_variants_kernel<<<n_jobs, 64, 0, stream>>>(jobs, out);
+  check(hipGetLastError(), "decide variants launch");
 }
 
 void launch_integrate(const IntegrateArgs& a, hipStream_t stream) {

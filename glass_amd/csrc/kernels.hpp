@@ -136,8 +136,10 @@ struct BatchWarpJob {
 void warp_tile_shape(int* tile_w, int* tile_h);
 long long warp_batched_items(const BatchWarpJob& job);  // ceil(frames / kWarpBatch) * tiles
 unsigned warp_batched_max_blocks();                     // resident blocks that fill the device
-void launch_warp_batched(const BatchWarpJob* jobs, int n_jobs, const BatchWarpFrame* frames, long long total_items,
-                         const float4* lut, unsigned grid, int order_stride, hipStream_t stream);
+// item_job[i]: the job of work item i (jobs are in first_item order).
+void launch_warp_batched(const BatchWarpJob* jobs, const unsigned short* item_job, const BatchWarpFrame* frames,
+                         long long total_items, const float4* lut, unsigned grid, int order_stride,
+                         hipStream_t stream);
 
 void launch_fill(float* dst, long long n, float value, hipStream_t stream);
 void launch_add_scalar(float* dst, long long n, float value, hipStream_t stream);
@@ -174,23 +176,25 @@ struct IntegrateArgs {
   unsigned* dq;
   const float* geometric;        // nullable
   int dq_active_frames;
-  // Batched launches: when non-null, the statistics arithmetic is decided on
-  // the device from these n_active frame flags (all_weights_positive && no
-  // flag set -> all-valid), overriding variant_all_valid.
-  const unsigned* const* active_flags;
-  int all_weights_positive;
+  // When non-null, the statistics arithmetic is read from this device word at
+  // run time (decided on the device after a batched warp, see
+  // launch_decide_variants), overriding variant_all_valid.
+  const int* variant_ptr;
 };
+
+// Statistics variant of one batched job: all-valid arithmetic when every
+// frame weight is positive and none of its n_flags active frames holds a
+// non-finite sample.
+struct VariantJob {
+  const unsigned* const* flags;  // device table of per-frame flag words
+  int n_flags;
+  int weights_positive;
+};
+void launch_decide_variants(const VariantJob* jobs, int n_jobs, int* out, hipStream_t stream);
 
 // Returns the register capacity (max frames) the winsorized kernel supports.
 int integrate_max_frames();
 void launch_integrate(const IntegrateArgs& a, hipStream_t stream);
-// Capacity class (sorting network size) of the winsorized kernel for n_active.
-int integrate_capacity(int n_active);
-long long integrate_blocks(long long npix);
-// One persistent launch over the pixel blocks of all jobs of one capacity
-// class (jobs[]: device table; first_block[]: device prefix of blocks).
-void launch_integrate_batched(int capacity, bool unit, const IntegrateArgs* jobs, const long long* first_block,
-                              int n_jobs, long long total_blocks, unsigned grid, hipStream_t stream);
 
 // Per-tile "any non-finite sample" flags over n_active frames (for the
 // CPUStackEngine per-tile statistics choice); flags[t] = 1 if all finite.

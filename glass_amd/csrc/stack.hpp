@@ -142,6 +142,22 @@ struct gh_stack {
     return total;
   }
 
+  // The coverage entry points (reset / full-frame accumulate) take no stream
+  // and run on the stack's own stream; a warp or integrate still queued on a
+  // caller stream may touch the coverage map, so they wait on this event,
+  // recorded on that caller stream after such work.
+  hipEvent_t caller_done = nullptr;
+
+  void mark_caller(hipStream_t st) {
+    if (st == stream) return;
+    if (!caller_done) check(hipEventCreateWithFlags(&caller_done, hipEventDisableTiming), "hipEventCreate");
+    check(hipEventRecord(caller_done, st), "hipEventRecord(caller stream)");
+  }
+
+  void wait_caller() {
+    if (caller_done) check(hipStreamWaitEvent(stream, caller_done, 0), "hipStreamWaitEvent(caller stream)");
+  }
+
   void ensure_geometric() {
     if (geometric) return;
     check(hipMalloc(reinterpret_cast<void**>(&geometric), frame_bytes()), "hipMalloc(warp coverage)");
@@ -195,6 +211,7 @@ struct gh_stack {
     star_cand.release();
     star_windows.release();
     adj_ws.release();
+    if (caller_done) (void)hipEventDestroy(caller_done);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };

@@ -122,12 +122,18 @@ __device__ __forceinline__ float lanczos_direct(const float* frame, int width, d
   float wx[6], wy[6];
   lanczos_axis(static_cast<float>(sx - fx0), thr, wx);
   lanczos_axis(static_cast<float>(sy - fy0), thr, wy);
-  const float* src = frame + static_cast<long long>(y0 - 2) * width + (x0 - 2);
+  const global_cf32* src = (const global_cf32*)(frame + static_cast<long long>(y0 - 2) * width + (x0 - 2));
   float acc = 0.0f, wsum = 0.0f;
   float vmin = INFINITY, vmax = -INFINITY;
+  // CPU semantics take a tap when its FP64 weight is non-zero
+  // (warp.py:121-152).  In FP64, sin(pi d) never rounds to 0 for an integer
+  // d != 0, so that weight is 0 only at |d| >= 3: the last tap (d = f - 3) of
+  // an axis whose fraction f is exactly 0.  (The f32 weights here are exactly
+  // 0 at every integer tap; testing them would stop NaN the reference keeps.)
+  const bool last_x = sx != fx0, last_y = sy != fy0;
 #pragma unroll
   for (int j = 0; j < 6; ++j) {
-    const float* row = src + static_cast<long long>(j) * width;
+    const global_cf32* row = src + static_cast<long long>(j) * width;
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
       const float v = row[i];
@@ -139,7 +145,7 @@ __device__ __forceinline__ float lanczos_direct(const float* frame, int width, d
           vmin = fminf(vmin, v);
           vmax = fmaxf(vmax, v);
         }
-      } else if (wx[i] != 0.0f && wy[j] != 0.0f) {
+      } else if ((i < 5 || last_x) && (j < 5 || last_y)) {
         acc = fmaf(v, w, acc);
         wsum += w;
       }
@@ -261,11 +267,13 @@ __device__ __forceinline__ void lut_weights(const float4* lut, float f, float (&
   for (int k = 0; k < 6; ++k) w[k] = fmaf(u, fmaf(u, c[3 * k + 2], c[3 * k + 1]), c[3 * k]);
 }
 
-__device__ __forceinline__ float lanczos_taps(const float* t, int stride, const float (&wx)[6], const float (&wy)[6]) {
+// T: an LDS pointer (staged tile) or a global-address-space frame pointer.
+template <typename T>
+__device__ __forceinline__ float lanczos_taps(T t, int stride, const float (&wx)[6], const float (&wy)[6]) {
   float acc = 0.0f;
 #pragma unroll
   for (int j = 0; j < 6; ++j) {
-    const float* r = t + j * stride;
+    const auto r = t + j * stride;
     float h = r[0] * wx[0];
 #pragma unroll
     for (int i = 1; i < 6; ++i) h = fmaf(r[i], wx[i], h);
@@ -279,34 +287,38 @@ struct TileFrame {
   float* dst;
   double m0, m1, m2, m3, m4, m5;
   int bx0, by0, bw, bh;  // staged box origin, row stride and rows
-  bool fast;             // table weights (finite source, no clamp)
+  bool fast;             // table weights (no clamp; finite source unless CPU semantics)
   bool use_tile;         // fast and the box fits one LDS buffer
+  bool recheck;          // fast over a source holding non-finite samples
 };
 
 // Frame f's matrix, path choice and source bounding box for the output tile
 // [ox0, ox1] x [oy0, oy1].  An affine map takes its extremes at the corners;
 // interior points computed in FP64 stay within ~1e-11 px of the corners' hull,
 // so one pixel beyond the +-3 tap reach covers every tap.
-// One frame of a tile's work: source, destination, FP64 inverse (9, the
-// affine rows used), the source's "holds a non-finite sample" flag and the
+// One frame of a tile's work: source, destination, FP64 inverse (the two
+// affine rows), the source's "holds a non-finite sample" flag and the
 // destination's flag to raise.
 struct FrameRef {
   const float* src;
   float* dst;
-  const double* inv;
+  double inv[6];
   unsigned src_flag;
   unsigned* dst_flag;
 };
 
-__device__ __forceinline__ TileFrame tile_frame(const FrameRef& ref, bool clamp_on, int ox0, int oy0, int ox1,
-                                                int oy1, int W, int H) {
+__device__ __forceinline__ TileFrame tile_frame(const FrameRef& ref, bool skip_nonfinite, bool clamp_on, int ox0,
+                                                int oy0, int ox1, int oy1, int W, int H) {
   TileFrame fr;
   fr.src = ref.src;
   fr.dst = ref.dst;
   const double* m = ref.inv;
   fr.m0 = m[0]; fr.m1 = m[1]; fr.m2 = m[2];
   fr.m3 = m[3]; fr.m4 = m[4]; fr.m5 = m[5];
-  fr.fast = !clamp_on && ref.src_flag == 0u;
+  // CPU semantics over a non-finite source still take the table path; the
+  // pixels it cannot reproduce are redone by recheck_column.
+  fr.fast = !clamp_on && (ref.src_flag == 0u || !skip_nonfinite);
+  fr.recheck = ref.src_flag != 0u;
   double mnx = INFINITY, mxx = -INFINITY, mny = INFINITY, mxy = -INFINITY;
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
@@ -340,7 +352,7 @@ __device__ __forceinline__ void stage_box(const TileFrame& fr, int W, float* til
   constexpr int kUnroll = 8;
   const int total = fr.bw * fr.bh;
   const int dq = kTiledThreads / fr.bw, dr = kTiledThreads - dq * fr.bw;
-  const float* origin = fr.src + static_cast<long long>(fr.by0) * W + fr.bx0;
+  const global_cf32* origin = (const global_cf32*)(fr.src + static_cast<long long>(fr.by0) * W + fr.bx0);
   int e = threadIdx.x;
   int row = e / fr.bw, col = e - row * fr.bw;
   for (; e < total; e += kTiledThreads * kUnroll) {
@@ -462,7 +474,7 @@ __device__ __forceinline__ bool strip_window_rows(const TileFrame& fr, const flo
     value = ok ? value : fill;
     if (x < W) {
       cov += ok ? (1ull << (4 * (kbase + r))) : 0ull;
-      fr.dst[static_cast<long long>(y) * W + x] = value;
+      st_global(fr.dst, static_cast<long long>(y) * W + x, value);
       bad |= !isfinite(value);
     }
   });
@@ -489,11 +501,46 @@ __device__ __forceinline__ bool column_global_rows(const TileFrame& fr, const fl
       float wx[6], wy[6];
       lut_weights(lut, static_cast<float>(sx - fx0), wx);
       lut_weights(lut, static_cast<float>(sy - fy0), wy);
-      value = lanczos_taps(fr.src + static_cast<long long>(y0 - 2) * W + (x0 - 2), W, wx, wy);
+      value = lanczos_taps((const global_cf32*)(fr.src + static_cast<long long>(y0 - 2) * W + (x0 - 2)), W, wx, wy);
       cov += 1ull << (4 * k);
     }
-    fr.dst[static_cast<long long>(y) * W + x] = value;
+    st_global(fr.dst, static_cast<long long>(y) * W + x, value);
     bad |= !isfinite(value);
+  }
+  return bad;
+}
+
+// CPU-semantics frames whose source holds non-finite samples run the table
+// paths too.  A table value is finite exactly when its 36 taps are (a NaN tap,
+// or an infinite one, makes the weighted sum non-finite whatever its weight),
+// so only in-footprint pixels whose stored value is non-finite are recomputed,
+// per tap with zero-weight taps skipped as warp.py:121-152 does.  The lane
+// re-reads the column it has just written (its own stores, program order).
+// Returns whether any final value of the column is non-finite.
+__device__ __forceinline__ bool recheck_column(const TileFrame& fr, int x, int y_first, int W, int H) {
+  if (x >= W) return false;
+  WarpParams q{};
+  q.skip_nonfinite = 0;
+  q.clamp = -1.0f;
+  const double xmax = static_cast<double>(W - 3), ymax = static_cast<double>(H - 3);
+  bool bad = false;
+#pragma unroll 1
+  for (int k = 0; k < kLaneRows; ++k) {
+    const int y = y_first + k;
+    if (y >= H) break;
+    const long long o = static_cast<long long>(y) * W + x;
+    float v = ld_global(fr.dst, o);
+    if (!isfinite(v)) {
+      // Same expression and footprint test as the table paths.
+      const double sx = (fr.m0 * static_cast<double>(x) + fr.m1 * static_cast<double>(y)) + fr.m2;
+      const double sy = (fr.m3 * static_cast<double>(x) + fr.m4 * static_cast<double>(y)) + fr.m5;
+      if (sx >= 2.0 && sx < xmax && sy >= 2.0 && sy < ymax) {
+        bool covered;
+        v = lanczos_direct(fr.src, W, sx, sy, q, 1.0e-8f, covered);
+        st_global(fr.dst, o, v);
+      }
+    }
+    bad |= !isfinite(v);
   }
   return bad;
 }
@@ -517,7 +564,7 @@ __device__ __forceinline__ bool column_direct_rows(const TileFrame& fr, const Wa
       value = lanczos_direct(fr.src, p.width, sx, sy, p, thr, covered);
       if (covered) cov += 1ull << (4 * k);
     }
-    fr.dst[static_cast<long long>(y) * p.width + x] = value;
+    st_global(fr.dst, static_cast<long long>(y) * p.width + x, value);
     bad |= !isfinite(value);
   }
   return bad;
@@ -541,7 +588,7 @@ __device__ __forceinline__ void lanczos_tile(const GetFrame& get, int count, con
 
   for (int f = 0; f < count; ++f) {
     const FrameRef ref = get(f);
-    const TileFrame cur = tile_frame(ref, clamp_on, ox0, oy0, ox1, oy1, W, H);
+    const TileFrame cur = tile_frame(ref, p.skip_nonfinite != 0, clamp_on, ox0, oy0, ox1, oy1, W, H);
     __syncthreads();  // previous frame's taps consumed (first pass: table stored)
     if (cur.use_tile) stage_box(cur, W, tile);
     __syncthreads();
@@ -556,6 +603,7 @@ __device__ __forceinline__ void lanczos_tile(const GetFrame& get, int count, con
     } else {
       bad = column_direct_rows(cur, p, x, y_first, cov);
     }
+    if (cur.fast && cur.recheck && bad) bad = recheck_column(cur, x, y_first, W, H);
     if (__any(bad) && tx == 0) atomicOr(ref.dst_flag, 1u);
   }
   if (p.coverage && x < W) {
@@ -568,7 +616,7 @@ __device__ __forceinline__ void lanczos_tile(const GetFrame& get, int count, con
       if constexpr (ATOMIC_COVERAGE) {
         if (add != 0.0f) atomicAdd(p.coverage + o, add);
       } else {
-        p.coverage[o] = p.coverage[o] + add;
+        st_global(p.coverage, o, ld_global(p.coverage, o) + add);
       }
     }
   }
@@ -603,14 +651,16 @@ __device__ __forceinline__ void bilinear_tile(const GetFrame& get, int count, co
         const int x1 = min(x0 + 1, W - 1), y1 = min(y0 + 1, H - 1);
         const float tx_ = static_cast<float>(sx - fx0), ty_ = static_cast<float>(sy - fy0);
         const float* src = ref.src;
-        const float a00 = src[static_cast<long long>(y0) * W + x0], a01 = src[static_cast<long long>(y0) * W + x1];
-        const float a10 = src[static_cast<long long>(y1) * W + x0], a11 = src[static_cast<long long>(y1) * W + x1];
+        const float a00 = ld_global(src, static_cast<long long>(y0) * W + x0);
+        const float a01 = ld_global(src, static_cast<long long>(y0) * W + x1);
+        const float a10 = ld_global(src, static_cast<long long>(y1) * W + x0);
+        const float a11 = ld_global(src, static_cast<long long>(y1) * W + x1);
         const float top = a00 * (1.0f - tx_) + a01 * tx_;
         const float bottom = a10 * (1.0f - tx_) + a11 * tx_;
         value = top * (1.0f - ty_) + bottom * ty_;
         cov += 1ull << (4 * k);
       }
-      ref.dst[static_cast<long long>(y) * W + x] = value;
+      st_global(ref.dst, static_cast<long long>(y) * W + x, value);
       bad |= !isfinite(value);
     }
     if (bad) atomicOr(ref.dst_flag, 1u);
@@ -646,32 +696,35 @@ __global__ __launch_bounds__(kTiledThreads) void warp_lanczos3_tiled_kernel(Warp
   const unsigned logical = xcd * q + min(xcd, r) + (bid >> 3);
   const int ox0 = static_cast<int>(logical % tiles_x) * kOutW;
   const int oy0 = static_cast<int>(logical / tiles_x) * kOutH;
-  auto get = [&](int f) { return FrameRef{b.src[f], b.dst[f], b.inv[f], b.src_flags[f], b.nonfinite[f]}; };
+  auto get = [&](int f) {
+    const double* m = b.inv[f];
+    return FrameRef{b.src[f], b.dst[f], {m[0], m[1], m[2], m[3], m[4], m[5]}, b.src_flags[f], b.nonfinite[f]};
+  };
   lanczos_tile<false>(get, count, p, ox0, oy0, lut, tile);
 }
 
 // ------------------------------------------------- batched multi-job warp
-// Persistent kernel over the work items of many jobs: item = (job, batch of up
-// to kWarpBatch frames, output tile).  Blocks stride over the item list; a
-// job's items are found by binary search over the per-job first-item offsets.
-__global__ __launch_bounds__(kTiledThreads) void warp_batched_kernel(const BatchWarpJob* __restrict__ jobs,
-                                                                     int n_jobs, const BatchWarpFrame* __restrict__ frames,
-                                                                     long long total_items, const float4* lut_g,
-                                                                     int order_stride) {
+// Kernel over the work items of many jobs: item = (job, batch of up to
+// kWarpBatch frames, output tile).  Blocks stride over the item list (one
+// item per block when the grid covers it); item_job[] names each item's job,
+// so a block re-reads the job descriptor only when the job changes.
+__global__ __launch_bounds__(kTiledThreads) __attribute__((amdgpu_waves_per_eu(3))) void warp_batched_kernel(
+    const BatchWarpJob* __restrict__ jobs, const unsigned short* __restrict__ item_job,
+    const BatchWarpFrame* __restrict__ frames, long long total_items, const float4* lut_g, int order_stride) {
   __shared__ float4 lut[kLutRows * kLutVec];
   __shared__ float tile[kMaxTile];
   load_lut(lut_g, lut);
+  int cur = -1;
+  BatchWarpJob job{};
   for (long long it = blockIdx.x; it < total_items; it += gridDim.x) {
     // Optional interleaving: visit items with a stride coprime to the total so
     // concurrent blocks spread over jobs.
     const long long item = order_stride > 1 ? (it * order_stride) % total_items : it;
-    int lo = 0, hi = n_jobs - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (jobs[mid].first_item <= item) lo = mid;
-      else hi = mid - 1;
+    const int j = __builtin_amdgcn_readfirstlane(static_cast<int>(item_job[item]));
+    if (j != cur) {
+      job = load_uniform(jobs + j);
+      cur = j;
     }
-    const BatchWarpJob& job = jobs[lo];
     const long long local = item - job.first_item;
     const int batch = static_cast<int>(local / job.tiles), t = static_cast<int>(local % job.tiles);
     const int f0 = job.first_frame + batch * kWarpBatch;
@@ -687,8 +740,9 @@ __global__ __launch_bounds__(kTiledThreads) void warp_batched_kernel(const Batch
     p.affine = 1;
     p.coverage = job.coverage;
     auto get = [&](int f) {
-      const BatchWarpFrame& fr = frames[f0 + f];
-      return FrameRef{fr.src, fr.dst, fr.inv, *fr.src_flag, fr.dst_flag};
+      const BatchWarpFrame fr = load_uniform(frames + (f0 + f));
+      return FrameRef{fr.src, fr.dst, {fr.inv[0], fr.inv[1], fr.inv[2], fr.inv[3], fr.inv[4], fr.inv[5]},
+                      static_cast<unsigned>(__builtin_amdgcn_readfirstlane(*(const global_u32*)fr.src_flag)), fr.dst_flag};
     };
     if (job.interpolation == 3) {
       lanczos_tile<true>(get, count, p, ox0, oy0, lut, tile);
@@ -770,11 +824,12 @@ unsigned warp_batched_max_blocks() {
   return static_cast<unsigned>(std::max(1, cus) * std::max(1, per_cu));
 }
 
-void launch_warp_batched(const BatchWarpJob* jobs, int n_jobs, const BatchWarpFrame* frames, long long total_items,
-                         const float4* lut, unsigned grid, int order_stride, hipStream_t stream) {
+void launch_warp_batched(const BatchWarpJob* jobs, const unsigned short* item_job, const BatchWarpFrame* frames,
+                         long long total_items, const float4* lut, unsigned grid, int order_stride,
+                         hipStream_t stream) {
   if (total_items <= 0) return;
   const unsigned g = static_cast<unsigned>(std::min<long long>(std::max(grid, 1u), total_items));
-  warp_batched_kernel<<<g, kTiledThreads, 0, stream>>>(jobs, n_jobs, frames, total_items, lut, order_stride);
+  warp_batched_kernel<<<g, kTiledThreads, 0, stream>>>(jobs, item_job, frames, total_items, lut, order_stride);
   check(hipGetLastError(), "batched warp launch");
 }
 

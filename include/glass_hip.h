@@ -265,9 +265,10 @@ int gh_refine_affine(gh_stack* stack, int64_t reference_index, int64_t moving_in
  * No counterpart in the reference (which runs one resident job per call):
  * many stacking jobs -- each a loaded gh_stack, the frames to warp with their
  * FP64 inverses, frame weights and integration parameters/outputs -- run as
- * one persistent batched warp launch over all jobs' (frame batch, tile) items
- * plus one persistent integration launch per capacity class, optionally
- * captured once into a hipGraph and replayed.  Results equal the per-job
+ * one batched warp launch over all jobs' (frame batch, tile) items, one
+ * launch that decides each job's statistics arithmetic on the device from the
+ * warped frames' non-finite flags, and one integration launch per job that
+ * reads that decision, optionally captured once into a hipGraph and replayed.  Results equal the per-job
  * calls (gh_warp_frames + gh_integrate with statistics_variant -1, tile_size 0).
  * The batch owns the warped copies of the listed frames; the stacks' frames
  * are read, not modified. */
@@ -292,10 +293,12 @@ typedef struct gh_batch_job {
 #define GH_BATCH_AUTOTUNE 2u   /* time candidate schedules on the first run */
 
 int gh_batch_create(const gh_batch_job* jobs, int64_t n_jobs, uint32_t flags, gh_batch** out);
-/* Enqueue one execution on `stream` (NULL: the batch's stream); asynchronous. */
+/* Enqueue one execution on `stream`, asynchronously; NULL runs it on the
+   batch's own stream and returns when it has finished. */
 int gh_batch_run(gh_batch* batch, void* stream);
 /* out[8]: jobs, warp work items, integration launches, graph captured (0/1),
- * warp grid, warp item stride, integration grid, autotune trials run. */
+ * warp grid, warp item stride, jobs whose statistics arithmetic is decided
+ * on the device, autotune trials run. */
 int gh_batch_info(gh_batch* batch, int64_t* out);
 int gh_batch_destroy(gh_batch* batch);
 

@@ -35,7 +35,8 @@ def _outputs(h, w):
 
 SPECS = [
     # (h, w, frames, interpolation, semantics, fill, weights kind, nan pixel)
-    (70, 90, 5, "lanczos3", "cpu", 0.0, None, False),
+    (70, 90, 5, "lanczos3", "cpu", 0.0, None, True),
+    (96, 200, 12, "lanczos3", "cpu", math.nan, "zero", True),
     (128, 160, 17, "bilinear", "cpu", 0.0, "zero", True),
     (33, 47, 9, "lanczos3", "resident", math.nan, None, True),
     (200, 130, 40, "lanczos3", "cpu", 0.0, "varied", False),

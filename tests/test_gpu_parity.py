@@ -193,8 +193,9 @@ def test_warp_cpu_semantics_matches_oracle(gpu, interp):
 def test_warp_lanczos3_tiled_paths_match_oracle(gpu):
     # 150 x 300 spans 5 x 3 output tiles; each frame drives one branch of the
     # tiled kernel: LDS-staged taps (small rotation, zoom), global taps (40 deg
-    # rotation: the source box exceeds the LDS tile), per-pixel direct taps
-    # (a frame holding NaN), and an integer shift (delta weights: exact copy).
+    # rotation: the source box exceeds the LDS tile), table taps with the
+    # non-finite recheck (a frame holding NaN), and an integer shift (delta
+    # weights: exact copy).
     rng = np.random.default_rng(11)
     h, w = 150, 300
     frames = rng.normal(100, 10, size=(5, h, w)).astype(np.float32)
@@ -216,6 +217,45 @@ def test_warp_lanczos3_tiled_paths_match_oracle(gpu):
     assert np.array_equal(geo, cov_sum)
     # integer shift: output (y, x) takes source (y + 2, x - 3) exactly
     assert np.array_equal(got[3][2:h - 5, 5:w - 3], frames[3][4:h - 3, 2:w - 6])
+
+
+def test_warp_lanczos3_cpu_semantics_nonfinite_sources(gpu):
+    # CPU semantics over sources holding NaN / +-inf: the table paths keep
+    # finite pixels and recompute the non-finite ones per tap.  The reference
+    # takes every tap with a non-zero FP64 weight, which at an integer shift
+    # is every tap but the last of each axis: a NaN spreads to the pixels
+    # whose 5 x 5 taps hold it, exactly as in the oracle.
+    rng = np.random.default_rng(13)
+    h, w = 140, 290
+    frames = rng.normal(100, 10, size=(4, h, w)).astype(np.float32)
+    frames[0, 60:63, :] = np.nan
+    frames[0, 20, 40] = np.inf
+    frames[0, 100, 200] = -np.inf
+    for f in (1, 2, 3):
+        ys, xs = rng.integers(5, h - 5, 25), rng.integers(5, w - 5, 25)
+        frames[f, ys, xs] = np.nan
+    frames[3, 30, 10:280] = np.nan
+    mats = [_rot(0.5, 1.7, -2.3), _rot(0.0, 3.0, -2.0), _rot(40.0, 60.0, -80.0), _rot(0.3, -4.0, 2.5, 1.3)]
+    with _stack_of(frames) as s:
+        s.reset_warp_coverage()
+        s.apply_matrix_frames_cpu_semantics(list(range(4)), np.stack(mats), "lanczos3", fill=np.nan)
+        got = s.download_frames_tile(list(range(4)), 0, 0, w, h)
+        geo = s.warp_coverage_map()
+    cov_sum = np.zeros((h, w), np.float32)
+    for i in range(4):
+        exp, cov = oracle.warp_matrix(frames[i], mats[i], fill=np.nan, interpolation="lanczos3")
+        cov_sum += cov
+        assert np.array_equal(np.isfinite(got[i]), np.isfinite(exp)), i
+        nf = ~np.isfinite(exp)
+        assert np.array_equal(got[i][nf], exp[nf].astype(np.float32), equal_nan=True), i
+        ok = ~nf
+        assert np.allclose(got[i][ok], exp[ok], rtol=0, atol=2e-4), (i, np.max(np.abs(got[i][ok] - exp[ok])))
+        assert np.isnan(got[i]).any() and ok.sum() > h * w // 2, i
+    assert np.array_equal(geo, cov_sum)
+    # integer shift: every finite output pixel is its source pixel exactly
+    shifted, src = got[1][2:h - 5, 5:w - 3], frames[1][4:h - 3, 2:w - 6]
+    fin = np.isfinite(shifted)
+    assert np.array_equal(shifted[fin], src[fin]) and (~fin).sum() > 25 * 20
 
 
 def test_warp_lanczos3_resident_tiled_matches_oracle(gpu):
