@@ -51,6 +51,9 @@ def parse_args():
     p.add_argument("--no-batch", action="store_true", help="skip the batched multi-job (config 3) leg")
     p.add_argument("--batch-jobs", type=int, default=64)
     p.add_argument("--batch-reps", type=int, default=3)
+    p.add_argument("--copy-probe-mib", type=int, default=0,
+                   help="issue one untimed device-to-device copy of this many MiB before the timed steps "
+                        "(a known-size reference for the FETCH_SIZE / WRITE_SIZE counters)")
     return p.parse_args()
 
 
@@ -344,6 +347,24 @@ def batched_leg(args):
     }
 
 
+# Measured HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, rocprofv3 --pmc, each
+# scaled by a known-size copy) of the default workload's kernels.
+_TRAFFIC_KERNELS = {"calibrate": "gh::calibrate_batch_kernel<2, false, 16>", "warp": "gh::warp_lanczos3_tiled_kernel",
+                    "integrate": "gh::integrate_winsorized_kernel<208, true>"}
+
+
+def measured_traffic(args, stage):
+    path = ROOT / "profiles" / "hbm_traffic_200x9600x6422.json"
+    default = (args.frames, args.height, args.width, args.interpolation, args.zero_weight) == (200, 6422, 9600,
+                                                                                                "lanczos3", 7)
+    if not default or not path.exists():
+        return None, None
+    k = json.loads(path.read_text())["kernels"].get(_TRAFFIC_KERNELS[stage])
+    if k is None:
+        return None, None
+    return int(k["read_bytes"] + k["write_bytes"]), str(path.relative_to(ROOT))
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -424,6 +445,13 @@ def main():
             stage_ms["warp"].append(ev[1].elapsed_time(ev[2]))
             stage_ms["integrate"].append(ev[2].elapsed_time(ev[3]))
 
+    if args.copy_probe_mib > 0:
+        probe_src = torch.ones(args.copy_probe_mib * 2**18, dtype=torch.float32, device=dev)
+        probe_dst = torch.empty_like(probe_src)
+        probe_dst.copy_(probe_src)
+        torch.cuda.synchronize()
+        del probe_src, probe_dst
+        log(f"copy probe: {args.copy_probe_mib} MiB device-to-device")
     for i in range(args.warmup):
         step(False)
         torch.cuda.synchronize()
@@ -457,6 +485,7 @@ def main():
     per_launch_bytes = {"calibrate": bytes_cal, "warp": bytes_warp, "integrate": bytes_int}[dominant] / launches
     avg_launch_ms = med[dominant] / launches
     total_bytes = bytes_cal + bytes_warp + bytes_int
+    traffic, traffic_src = measured_traffic(args, dominant)
     line = {
         "metric": "calib+warp+reject+integrate frame-Mpix/s & HBM %peak, 200x9600x6422",
         "value": round(value, 2),
@@ -487,7 +516,8 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(per_launch_bytes / (avg_launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "launches_per_step": launches,
             "avg_launch_ms": round(avg_launch_ms, 4),
         },

@@ -464,3 +464,60 @@ def test_end_to_end_config0_and_dq(gpu):
     assert np.array_equal(res.geometric, geo)
     edq = oracle.resident_dq_map(exp[0], exp[1], exp[2], exp[3], exp[4], geo, active_frame_count=len(cal))
     assert np.array_equal(res.dq, edq)
+
+
+def test_end_to_end_config1_stagewise(gpu):
+    # Config 1's recipe at a 256 x 256 crop (32 lights, 16 + 16 + 16 calibration
+    # frames, raw u16 ADU, drift +-20 px / +-0.5 deg / scale 1 +- 5e-4, cosmic
+    # rays, 2 satellite trails, Lanczos3 with NaN fill, variance-aware weights).
+    # Each stage is compared with the oracle on that stage's own inputs, so a
+    # last-bit warp difference cannot flip a rejection decision downstream.
+    data = make_set(seed=1, n_lights=32, height=256, width=256, n_cal=16, n_stars=25, fwhm=3.0, max_shift=20.0,
+                    max_rot_deg=0.5, max_scale=5e-4, cosmic_density=1e-4, trails=2)
+    f, h, w = data.light.shape
+    adu = to_adu(data.light)
+    bias, dark, flat = (to_adu(x).astype(np.float32) for x in (data.bias, data.dark, data.flat))
+    pol = {"tile_size": 128}
+    mb = P.build_master_bias(bias, pol)
+    md = P.build_master_dark(dark, mb, pol)
+    mf, _ = P.build_master_flat(flat, mb, pol)
+    assert np.array_equal(mb, oracle.build_master_bias(bias, pol))
+    assert np.array_equal(md, oracle.build_master_dark(dark, mb, pol))
+    assert np.array_equal(mf, oracle.build_master_flat(flat, mb, pol)[0])
+    moving = [i for i in range(f) if not np.array_equal(data.matrices[i], np.eye(3))]
+    with ResidentCalibratedStack(f, h, w) as s:
+        s.set_calibration_masters(mb, md, mf)
+        for i in range(f):
+            s.calibrate_frame(i, adu[i], data.light_exposure_s, data.dark_exposure_s, None)
+        cal = s.download_frames_tile(list(range(f)), 0, 0, w, h)
+        rms = [s.frame_global_stats(i)["std"] for i in range(f)]
+        s.reset_warp_coverage()
+        for _ in range(f - len(moving)):
+            s.accumulate_full_warp_coverage_frame()
+        s.apply_matrix_frames_cpu_semantics(moving, np.stack([data.matrices[i] for i in moving]), "lanczos3",
+                                            fill=np.nan)
+        warped = s.download_frames_tile(list(range(f)), 0, 0, w, h)
+        weights = P.variance_aware_weights(rms)
+        got = s.integrate_hardened_winsorized_sigma(weights, 3.0, 3.0, 3, 0.5)
+    # calibration: bit-exact
+    for i in range(f):
+        exp = oracle.calibrate_light(adu[i], mb, md, mf, data.light_exposure_s, data.dark_exposure_s)
+        assert np.array_equal(cal[i], exp), i
+    # weights: variance_aware over the frames' FP64 rms
+    ref_rms = [oracle.frame_global_stats(c)["std"] for c in cal]
+    assert np.allclose(rms, ref_rms, rtol=1e-9)
+    exp_w = oracle.quality_weights([{"background_rms": r} for r in ref_rms], "variance_aware")
+    assert np.allclose(weights, exp_w, rtol=1e-6)
+    # warp: tolerance (f32 table weights vs FP64), non-finite pattern exact
+    for i in moving:
+        exp, _cov = oracle.warp_matrix(cal[i], data.matrices[i], fill=np.nan, interpolation="lanczos3")
+        fin = np.isfinite(exp)
+        assert np.array_equal(np.isfinite(warped[i]), fin), i
+        err = np.abs(warped[i][fin] - exp[fin])
+        scale = np.max(np.abs(exp[fin]))
+        assert err.max() <= 3e-6 * scale and np.median(err) <= 1e-6 * np.median(np.abs(exp[fin])), \
+            (i, err.max(), scale)
+    # integration of the GPU's warped stack
+    exp = oracle.weighted_integrate_stack(warped, weights=weights, rejection="winsorized_sigma")
+    _compare(got[:5], exp)
+    assert got[3].sum() + got[4].sum() > 0  # cosmic rays / trails are rejected
