@@ -278,8 +278,8 @@ int gh_batch_create(const gh_batch_job* jobs, int64_t n_jobs, uint32_t flags, gh
       flag_base += static_cast<size_t>(std::max<int64_t>(in.n_warp, 0));
       const int n_active = static_cast<int>(table.size());
       if (prm.rejection == GH_REJECT_WINSORIZED_SIGMA && n_active > gh::integrate_max_frames())
-        throw gh::ValueError("winsorized integration supports at most 256 positive-weight frames");
-      const size_t padded = static_cast<size_t>(std::max(gh::integrate_max_frames(), n_active));
+        throw gh::ValueError("winsorized integration supports at most 16384 positive-weight frames");
+      const size_t padded = static_cast<size_t>(std::max(gh::integrate_register_capacity(), n_active));
       const float* pad_ptr = n_active ? table.back() : s->slot[0];
       table.resize(padded, pad_ptr);
       w.resize(padded, 0.0f);

@@ -371,12 +371,12 @@ int gh_integrate(gh_stack* s, const float* weights, const gh_integrate_params* p
     }
     const int n_active = static_cast<int>(table.size());
     if (prm.rejection == GH_REJECT_WINSORIZED_SIGMA && n_active > gh::integrate_max_frames())
-      throw gh::ValueError("winsorized integration supports at most 256 positive-weight frames");
+      throw gh::ValueError("winsorized integration supports at most 16384 positive-weight frames");
     if (s->npix >= (int64_t(1) << 30)) throw gh::ValueError("frames of 2^30 pixels or more are not supported");
     // The winsorized kernel reads table entries [0, capacity) unconditionally
     // (compile-time indices); pad to the largest capacity with a valid frame
     // (the last active one, or frame 0 when none is active) and weight 0.
-    const size_t padded = static_cast<size_t>(std::max(gh::integrate_max_frames(), n_active));
+    const size_t padded = static_cast<size_t>(std::max(gh::integrate_register_capacity(), n_active));
     const float* pad_ptr = n_active ? table.back() : s->slot[0];
     table.resize(padded, pad_ptr);
     w.resize(padded, 0.0f);

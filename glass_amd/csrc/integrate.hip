@@ -87,6 +87,11 @@ constexpr Network<N> make_network() {
 template <int N>
 inline constexpr Network<N> kNetwork = make_network<N>();
 
+// The LDS-sort path (more than 256 frames): one block's pixel segments fill
+// 64 KB (two blocks per CU); fpad <= 16384 keys leaves at least one pixel.
+constexpr int kLdsSortBytes = 64 * 1024;
+constexpr int kLdsSortMaxFrames = kLdsSortBytes / 4;
+
 // Frame slots are global memory.  Pointers read from the frame table are
 // generic; casting them to the global address space turns flat loads (which
 // also occupy the LDS/scalar counter and serialise behind every scalar load)
@@ -496,6 +501,178 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CAP > 20
   winsorized_pixel<CAP, UNIT>(a, static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x);
 }
 
+// ---------------------------------------------------- more than 256 frames
+// Stacks with more positive-weight frames than the register kernels hold sort
+// each pixel's samples in LDS.  A block of kBlock threads takes P = 2^log2_p
+// consecutive pixels: it stages their samples with coalesced loads (P floats
+// of one frame per segment), keyed as in the register kernels (invalid ->
+// +inf), into P segments of fpad = next power of two >= n_active keys, and
+// sorts every segment with a bitonic network run by T = kBlock / P threads.
+// Segments are XOR-swizzled (key k of pixel p lives at p * fpad + (k ^ (p & 31)))
+// so that both the staging writes and the sort's pair accesses spread over the
+// LDS banks without padding.  The first thread of each pixel then runs the
+// winsorized arithmetic sequentially over the sorted keys in the register
+// kernels' order and precision.
+__device__ __forceinline__ int swz(int pix, int k, int fpad) { return pix * fpad + (k ^ (pix & 31)); }
+
+template <bool UNIT>
+__global__ __launch_bounds__(kBlock) void integrate_winsorized_lds_kernel(IntegrateArgs a, int fpad, int log2_p) {
+  extern __shared__ float keys[];
+  const int P = 1 << log2_p;
+  const int T = kBlock >> log2_p;
+  const int n_act = a.n_active;
+  const long long p0 = static_cast<long long>(blockIdx.x) * P;
+  // Stage: element e -> pixel e % P, frame e / P (a wave reads contiguous
+  // pixels of one or a few frames).
+  for (int e = threadIdx.x; e < P * fpad; e += kBlock) {
+    const int pix = e & (P - 1), k = e >> log2_p;
+    const long long p = p0 + pix;
+    float key = INFINITY;
+    if (k < n_act && p < a.npix) {
+      const float x = load_global(a.frame_ptr[k], p);  // lanes may differ in k
+      key = isfinite(x) ? x : INFINITY;
+    }
+    keys[swz(pix, k, fpad)] = key;
+  }
+  __syncthreads();
+  const int pix = threadIdx.x / T, lane = threadIdx.x - pix * T;
+  const long long p = p0 + pix;
+  const bool leader = lane == 0 && p < a.npix;
+  // Frame-order sums of the rejection-free mean, before the keys are sorted
+  // (the register kernels' load pass; invalid samples add +0.0).
+  double swx = 0.0, sw = 0.0;
+  int cnt = 0;
+  if (leader) {
+    for (int k = 0; k < n_act; ++k) {
+      const float x = keys[swz(pix, k, fpad)];
+      const bool ok = x < INFINITY;
+      cnt += ok ? 1 : 0;
+      if constexpr (UNIT) {
+        swx += static_cast<double>(ok ? x : 0.0f);
+      } else {
+        const double w = static_cast<double>(ld_global(a.frame_w, k));
+        swx += ok ? static_cast<double>(x) * w : 0.0;
+        sw += ok ? w : 0.0;
+      }
+    }
+    if constexpr (UNIT) sw = static_cast<double>(cnt);
+  }
+  // Bitonic sort of every segment (ascending).
+  for (int size = 2; size <= fpad; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      __syncthreads();
+      for (int q = lane; q < (fpad >> 1); q += T) {
+        const int i = ((q & ~(stride - 1)) << 1) | (q & (stride - 1));
+        const int j = i + stride;
+        const int si = swz(pix, i, fpad), sj = swz(pix, j, fpad);
+        const float x = keys[si], y = keys[sj];
+        const float lo = fminf(x, y), hi = fmaxf(x, y);
+        const bool up = (i & size) == 0;
+        keys[si] = up ? lo : hi;
+        keys[sj] = up ? hi : lo;
+      }
+    }
+  }
+  __syncthreads();
+  if (!leader) return;
+  auto key = [&](int k) { return keys[swz(pix, k, fpad)]; };
+
+  bool all_valid = (a.variant_ptr ? *(const global_u32*)a.variant_ptr : static_cast<unsigned>(a.variant_all_valid)) != 0u;
+  if (a.tile_flags) {
+    const int y = static_cast<int>(p / a.width);
+    const int x = static_cast<int>(p - static_cast<long long>(y) * a.width);
+    all_valid = a.tile_flags[(y / a.tile_size) * a.tiles_x + (x / a.tile_size)] != 0;
+  }
+  if (cnt == 0) {
+    write_outputs(a, p, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f);
+    return;
+  }
+  // Linear-percentile ranks (pick_ranks).
+  const int last = cnt - 1;
+  const int p25 = last, p50 = last * 2, p75 = last * 3;
+  const int l25 = p25 >> 2, l50 = p50 >> 2, l75 = p75 >> 2;
+  const float center0 = lerp_percentile(key(l50), key(min(l50 + 1, last)), static_cast<float>(p50 & 3) * 0.25f,
+                                        all_valid);
+  const float q25 = lerp_percentile(key(l25), key(min(l25 + 1, last)), static_cast<float>(p25 & 3) * 0.25f,
+                                    all_valid);
+  const float q75 = lerp_percentile(key(l75), key(min(l75 + 1, last)), static_cast<float>(p75 & 3) * 0.25f,
+                                    all_valid);
+  float scale0 = (q75 - q25) / 1.349f;
+  const double n = static_cast<double>(cnt);
+  const bool scale0_ok = all_valid ? (scale0 > 0.0f) : (scale0 > 0.0f && isfinite(scale0));
+  if (!scale0_ok) scale0 = frame_axis_std(a, p, n, all_valid);
+  const float lo_clip = center0 - a.low_sigma * scale0;
+  const float hi_clip = center0 + a.high_sigma * scale0;
+  double ws = 0.0;
+  for (int k = 0; k < cnt; ++k) ws += static_cast<double>(__builtin_amdgcn_fmed3f(key(k), lo_clip, hi_clip));
+  double wmean = ws / n;
+  float center = static_cast<float>(wmean);
+  if (!all_valid) wmean = static_cast<double>(center);
+  double wss = 0.0;
+  for (int k = 0; k < cnt; ++k) {
+    const double d = static_cast<double>(__builtin_amdgcn_fmed3f(key(k), lo_clip, hi_clip)) - wmean;
+    wss += d * d;
+  }
+  float scale = static_cast<float>(sqrt(wss / n));
+  if (all_valid) {
+    if (!(scale > 0.0f)) scale = scale0;
+  } else {
+    if (!(scale > 0.0f && isfinite(scale))) scale = scale0;
+    center = nan_to_num(center);
+    scale = nan_to_num(scale);
+  }
+  const float lo_t = center - a.low_sigma * scale;
+  const float hi_t = center + a.high_sigma * scale;
+  int n_low = 0, n_high = 0;
+  const bool usable = !a.require_positive_scale || (isfinite(scale) && scale > 0.0f);
+  if (usable && !(key(0) >= lo_t && key(last) <= hi_t)) {
+    for (int k = 0; k < cnt; ++k) {
+      const float x = key(k);
+      n_low += (x < lo_t) ? 1 : 0;
+      n_high += (x > hi_t) ? 1 : 0;
+    }
+    const int rejected = n_low + n_high;
+    if (rejected > 0) {
+      const int remaining = cnt - rejected;
+      const float fraction = static_cast<float>(static_cast<double>(rejected) / n);
+      if (!(remaining >= a.min_samples && fraction <= a.max_frac)) {
+        n_low = 0;
+        n_high = 0;
+      }
+    }
+  }
+  float master, weight, coverage;
+  if (n_low + n_high == 0) {
+    master = (sw > 0.0) ? static_cast<float>(swx / sw) : 0.0f;
+    weight = static_cast<float>(sw);
+    coverage = static_cast<float>(cnt);
+  } else if constexpr (UNIT) {
+    double s = 0.0;
+    for (int k = 0; k < cnt; ++k) {
+      const float x = key(k);
+      s += static_cast<double>((x >= lo_t && x <= hi_t) ? x : 0.0f);
+    }
+    const int kept = cnt - n_low - n_high;
+    master = static_cast<float>(s / static_cast<double>(kept));
+    weight = static_cast<float>(kept);
+    coverage = static_cast<float>(kept);
+  } else {
+    double s = 0.0, wsum = 0.0;
+    for (int k = 0; k < n_act; ++k) {
+      const float x = load_global(a.frame_ptr[k], p);
+      if (isfinite(x) && x >= lo_t && x <= hi_t) {
+        const double w = static_cast<double>(ld_global(a.frame_w, k));
+        s += static_cast<double>(x) * w;
+        wsum += w;
+      }
+    }
+    master = (wsum > 0.0) ? static_cast<float>(s / wsum) : 0.0f;
+    weight = static_cast<float>(wsum);
+    coverage = static_cast<float>(cnt - n_low - n_high);
+  }
+  write_outputs(a, p, master, weight, coverage, static_cast<float>(n_low), static_cast<float>(n_high));
+}
+
 // Rejection-free weighted mean for any frame count (integrate_mean,
 // cpp/cuda/integration_kernels.cu:1610-1656; oracle cpu/integration.py:182-194).
 template <bool UNIT>
@@ -566,7 +743,8 @@ void launch_cap(const IntegrateArgs& a, hipStream_t stream) {
 
 }  // namespace
 
-int integrate_max_frames() { return 256; }
+int integrate_register_capacity() { return 256; }
+int integrate_max_frames() { return kLdsSortMaxFrames; }
 
 void launch_decide_variants(const VariantJob* jobs, int n_jobs, int* out, hipStream_t stream) {
   if (n_jobs <= 0) return;
@@ -591,8 +769,20 @@ void launch_integrate(const IntegrateArgs& a, hipStream_t stream) {
     launch_cap<208>(a, stream);
   } else if (a.n_active <= 256) {
     launch_cap<256>(a, stream);
+  } else if (a.n_active <= kLdsSortMaxFrames) {
+    int fpad = 512;
+    while (fpad < a.n_active) fpad <<= 1;
+    int log2_p = 0;  // pixels per block: P * fpad keys fill kLdsSortBytes
+    while ((static_cast<long long>(2) << log2_p) * fpad * 4 <= kLdsSortBytes && (2 << log2_p) <= 64) ++log2_p;
+    const long long blocks = (a.npix + (1LL << log2_p) - 1) >> log2_p;
+    const size_t lds = (static_cast<size_t>(fpad) << log2_p) * sizeof(float);
+    if (a.unit_weights) {
+      integrate_winsorized_lds_kernel<true><<<static_cast<unsigned>(blocks), kBlock, lds, stream>>>(a, fpad, log2_p);
+    } else {
+      integrate_winsorized_lds_kernel<false><<<static_cast<unsigned>(blocks), kBlock, lds, stream>>>(a, fpad, log2_p);
+    }
   } else {
-    throw ValueError("winsorized integration supports at most 256 positive-weight frames");
+    throw ValueError("winsorized integration supports at most 16384 positive-weight frames");
   }
   check(hipGetLastError(), "integrate kernel launch");
 }

@@ -192,8 +192,9 @@ struct VariantJob {
 };
 void launch_decide_variants(const VariantJob* jobs, int n_jobs, int* out, hipStream_t stream);
 
-// Returns the register capacity (max frames) the winsorized kernel supports.
-int integrate_max_frames();
+// Frame-count limits of the winsorized integration.
+int integrate_max_frames();         // positive-weight frames the winsorized path accepts
+int integrate_register_capacity();  // frames the register kernels hold; frame tables pad to it
 void launch_integrate(const IntegrateArgs& a, hipStream_t stream);
 
 // Per-tile "any non-finite sample" flags over n_active frames (for the

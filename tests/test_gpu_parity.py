@@ -396,6 +396,44 @@ def test_winsorized_empty_and_guard_edge_cases(gpu):
     assert mean[0].shape == em.shape
 
 
+@pytest.mark.parametrize("frames", [257, 300, 512, 513, 1100])
+def test_winsorized_many_frames_lds_path(gpu, frames):
+    # Beyond the register kernels' 256 samples: the LDS-sort path (512, 1024
+    # and 2048-key segments; 32, 16 and 8 pixels per block), NaN samples.
+    stack = _random_stack(frames, frames, 5, 13)
+    print(f"winsorized lds frames={frames}", flush=True)
+    with _stack_of(stack) as s:
+        got = s.integrate_hardened_winsorized_sigma(None, 3.0, 3.0, 3, 0.5)
+    exp = oracle.weighted_integrate_stack(stack, rejection="winsorized_sigma")
+    _compare(got[:5], exp)
+
+
+def test_winsorized_many_frames_weighted_all_valid_and_degenerate(gpu):
+    rng = np.random.default_rng(41)
+    f = 400
+    # weighted with zero-weight frames, no NaN (all-valid arithmetic)
+    stack = _random_stack(42, f, 6, 11, nan_frac=0.0)
+    weights = rng.uniform(0.2, 3.0, f).astype(np.float32)
+    weights[::37] = 0.0
+    with _stack_of(stack) as s:
+        got = s.integrate_hardened_winsorized_sigma(weights, 3.0, 2.5, 3, 0.5)
+    _compare(got[:5], oracle.weighted_integrate_stack(stack, weights=weights, rejection="winsorized_sigma",
+                                                      high_sigma=2.5))
+    # unit weights, all valid; a degenerate IQR (constant pixels) takes the std fallback
+    stack = _random_stack(43, f, 4, 9, nan_frac=0.0)
+    stack[:, 1, :] = 7.0
+    stack[:3, 1, 2] = 500.0
+    with _stack_of(stack) as s:
+        got = s.integrate_hardened_winsorized_sigma(None, 3.0, 3.0, 3, 0.5)
+    _compare(got[:5], oracle.weighted_integrate_stack(stack, rejection="winsorized_sigma"))
+
+
+def test_winsorized_frame_limit(gpu):
+    with _stack_of(np.ones((16385, 1, 2), np.float32)) as s:
+        with pytest.raises(ValueError):
+            s.integrate_hardened_winsorized_sigma(None, 3.0, 3.0, 3, 0.5)
+
+
 def test_integrate_mean_matches_oracle(gpu):
     stack = _random_stack(9, 11, 7, 9)
     weights = np.linspace(0.5, 1.5, 11).astype(np.float32)
