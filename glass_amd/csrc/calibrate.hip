@@ -174,18 +174,25 @@ __global__ __launch_bounds__(kBlock) void calibrate_batch_kernel(CalibBatch batc
 #pragma unroll
     for (int k0 = 0; k0 < MB; k0 += kAhead) {
       if (k0 >= count) break;
+      // The frames' pointers and scales are indexed through an opaque zero so
+      // their kernel-argument loads stay in this group: hoisted out of the
+      // pixel loop, MB frames' pointers overflow the SGPRs and are re-read
+      // from VGPR lanes on every iteration.
+      int zero = 0;
+      asm volatile("" : "+s"(zero));
       float v[kAhead][kCalPx];
 #pragma unroll
       for (int a = 0; a < kAhead; ++a) {
-        if (k0 + a < count) load_raw_px<FMT>(batch.src[k0 + a], g, v[a]);
+        if (k0 + a < count) load_raw_px<FMT>(batch.src[k0 + a + zero], g, v[a]);
       }
 #pragma unroll
       for (int a = 0; a < kAhead; ++a) {
         const int k = k0 + a;
         if (k >= count) break;
+        const float dark_scale = batch.dark_scale[k + zero];
 #pragma unroll
         for (int j = 0; j < kCalPx; ++j) {
-          v[a][j] = calibrate_one(v[a][j], b[j], d[j], fl[j], p, batch.dark_scale[k]);
+          v[a][j] = calibrate_one(v[a][j], b[j], d[j], fl[j], p, dark_scale);
           if (isfinite(v[a][j])) {
             if constexpr (STATS) {
               const double x = static_cast<double>(v[a][j]);
@@ -198,7 +205,7 @@ __global__ __launch_bounds__(kBlock) void calibrate_batch_kernel(CalibBatch batc
             if constexpr (STATS) ++nonfin[k];
           }
         }
-        store_px(batch.dst[k], g, v[a]);
+        store_px(batch.dst[k + zero], g, v[a]);
       }
     }
   }

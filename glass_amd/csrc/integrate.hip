@@ -387,18 +387,19 @@ __device__ __forceinline__ void winsorized_pixel(const IntegrateArgs& a, long lo
   const float hi_clip = center0 + a.high_sigma * scale0;
 
   // Winsorized mean and population variance in FP64.  np.clip(x, lo, hi) is
-  // the median of (x, lo, hi) because lo <= hi.
-  // Each pass below masks slots k >= cnt through its own opaque copy of cnt:
-  // with one shared cnt the compiler keeps the CAP lane masks (k < cnt) alive
-  // across the passes, which overflows the SGPRs.
+  // the median of (x, lo, hi) because lo <= hi.  The passes run over all CAP
+  // slots without masking: slots k >= cnt hold +inf (sorted last), clip to
+  // hi_clip, and their known contribution is removed once at the end.  The
+  // FP64 sums of clipped f32 samples are exact (and order-free) whenever the
+  // samples' exponents span less than the 29 spare bits of FP64, which the
+  // clip bounds guarantee in practice; the sorted-order sums rely on the same.
+  const double pad = static_cast<double>(CAP - cnt);
   double ws = 0.0;
-  int cnt_ws = cnt;
-  asm volatile("" : "+v"(cnt_ws));
   static_for<CAP>([&](auto K) GH_ALWAYS_INLINE {
     constexpr int k = decltype(K)::value;
-    const float w = __builtin_amdgcn_fmed3f(v.template get<k>(), lo_clip, hi_clip);
-    ws += static_cast<double>((k < cnt_ws) ? w : 0.0f);
+    ws += static_cast<double>(__builtin_amdgcn_fmed3f(v.template get<k>(), lo_clip, hi_clip));
   });
+  ws -= pad * static_cast<double>(hi_clip);
   double wmean = ws / n;
   float center = static_cast<float>(wmean);
   if (!all_valid) wmean = static_cast<double>(center);
@@ -406,15 +407,16 @@ __device__ __forceinline__ void winsorized_pixel(const IntegrateArgs& a, long lo
   // (one med3 each) instead of being kept live from the previous pass, which
   // would hold CAP doubles in registers and spill.
   float lo_clip2 = lo_clip, hi_clip2 = hi_clip;
-  int cnt_wss = cnt;
-  asm volatile("" : "+v"(lo_clip2), "+v"(hi_clip2), "+v"(cnt_wss));
+  asm volatile("" : "+v"(lo_clip2), "+v"(hi_clip2));
   double wss = 0.0;
   static_for<CAP>([&](auto K) GH_ALWAYS_INLINE {
     constexpr int k = decltype(K)::value;
-    const float w = __builtin_amdgcn_fmed3f(v.template get<k>(), lo_clip2, hi_clip2);
-    const double d = (k < cnt_wss) ? static_cast<double>(w) - wmean : 0.0;
+    const double d = static_cast<double>(__builtin_amdgcn_fmed3f(v.template get<k>(), lo_clip2, hi_clip2)) - wmean;
     wss += d * d;
   });
+  const double d_pad = static_cast<double>(hi_clip2) - wmean;
+  wss -= pad * (d_pad * d_pad);
+  wss = wss > 0.0 ? wss : 0.0;
   float scale = static_cast<float>(sqrt(wss / n));
   if (all_valid) {
     if (!(scale > 0.0f)) scale = scale0;
@@ -431,15 +433,17 @@ __device__ __forceinline__ void winsorized_pixel(const IntegrateArgs& a, long lo
   // Sorted keys: the smallest valid sample is slot 0, the largest rank cnt-1.
   const float mn = v.template get<0>(), mx = q.mx;
   if (usable && !(mn >= lo_t && mx <= hi_t)) {
-    int cnt_count = cnt;
-    asm volatile("" : "+v"(cnt_count));
-    // Branch-free: invalid slots hold +inf, which is never below lo_t.
+    // Branch-free: invalid slots hold +inf, which is never below lo_t and
+    // always above hi_c (hi_t capped at FLT_MAX, which leaves every finite
+    // comparison unchanged), so exactly CAP - cnt high counts are removed.
+    const float hi_c = fminf(hi_t, 3.402823466e38f);
     static_for<CAP>([&](auto K) GH_ALWAYS_INLINE {
       constexpr int k = decltype(K)::value;
       const float x = v.template get<k>();
       n_low += (x < lo_t) ? 1 : 0;
-      n_high += (x > hi_t && k < cnt_count) ? 1 : 0;
+      n_high += (x > hi_c) ? 1 : 0;
     });
+    n_high -= CAP - cnt;
     const int rejected = n_low + n_high;
     if (rejected > 0) {
       const int remaining = cnt - rejected;
@@ -460,13 +464,13 @@ __device__ __forceinline__ void winsorized_pixel(const IntegrateArgs& a, long lo
     // Opaque threshold copies keep the compares of the counting pass from
     // being kept alive (CAP lane masks) for this pass.
     float lo_t2 = lo_t, hi_t2 = hi_t;
-    int cnt_sum = cnt;
-    asm volatile("" : "+v"(lo_t2), "+v"(hi_t2), "+v"(cnt_sum));
+    asm volatile("" : "+v"(lo_t2), "+v"(hi_t2));
+    hi_t2 = fminf(hi_t2, 3.402823466e38f);  // excludes the +inf slots k >= cnt
     double s = 0.0;
     static_for<CAP>([&](auto K) GH_ALWAYS_INLINE {
       constexpr int k = decltype(K)::value;
       const float x = v.template get<k>();
-      s += static_cast<double>((k < cnt_sum && x >= lo_t2 && x <= hi_t2) ? x : 0.0f);
+      s += static_cast<double>((x >= lo_t2 && x <= hi_t2) ? x : 0.0f);
     });
     const int kept = cnt - n_low - n_high;
     master = static_cast<float>(s / static_cast<double>(kept));
