@@ -37,6 +37,8 @@ int gh_stack_create(int64_t frame_count, int64_t height, int64_t width, gh_stack
     if (frame_count <= 0 || height <= 0 || width <= 0)
       throw gh::ValueError("resident stack dimensions must be non-empty");
     if (height > (1 << 30) || width > (1 << 30)) throw gh::ValueError("resident stack dimensions are too large");
+    // Kernels address a frame with 32-bit byte offsets.
+    if (height * width >= (int64_t(1) << 30)) throw gh::ValueError("frames of 2^30 pixels or more are not supported");
     auto* s = new gh_stack();
     try {
       s->frames = frame_count;

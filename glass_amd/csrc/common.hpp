@@ -44,6 +44,11 @@ typedef __attribute__((address_space(1))) const float global_cf32;
 // path when the frame pointer was read from a descriptor table.
 __device__ __forceinline__ float ld_global(const float* base, long long i) { return ((const global_cf32*)base)[i]; }
 __device__ __forceinline__ void st_global(float* base, long long i, float v) { ((global_f32*)base)[i] = v; }
+// Element i < 2^30 of a wave-uniform base: scalar base + 32-bit byte offset.
+__device__ __forceinline__ void st_global_u32(float* base, unsigned i, float v) {
+  typedef __attribute__((address_space(1))) char global_char_rw;
+  *(global_f32*)((global_char_rw*)base + (i << 2)) = v;
+}
 
 template <typename T>
 __device__ __forceinline__ T load_uniform(const T* p) {

@@ -411,21 +411,21 @@ __device__ __forceinline__ float window_taps(const float (&win)[NR][NC], const f
 // registers and loads one new row per pixel instead of 36 taps.  At each row a
 // wave-wide vote checks that every lane inside the footprint finds its taps at
 // window row r and the same column offset (0 or 1); otherwise that row reads
-// its taps from LDS directly.  Pixels outside the footprint evaluate a safe
-// in-box position and write `fill`.  Coverage counts go to bits 4 k of cov,
-// k = kbase + r.  Returns whether any written value is non-finite.
+// its taps from LDS directly.  Pixels outside the footprint write `fill`
+// (their LDS-fallback taps are read at an in-box position).  Row r's coverage
+// count goes to bits 4 r
+// of cov32 (the caller shifts the strip into its lane's 64-bit counts).
+// Returns whether any written value is non-finite.
 constexpr int kStripRows = 8;
 constexpr int kWinRows = kStripRows + 5;
 constexpr int kWinCols = 7;
 static_assert(kLaneRows % kStripRows == 0, "a lane's rows split into whole strips");
 
 __device__ __forceinline__ bool strip_window_rows(const TileFrame& fr, const float* tile, const float4* lut, int x,
-                                                  int y_first, int W, int H, float fill, int kbase,
-                                                  unsigned long long& cov) {
+                                                  int y_first, int W, int H, float fill, unsigned& cov32) {
   const double xd = static_cast<double>(x);
   const double px = fr.m0 * xd, py = fr.m3 * xd;
   const double xmax = static_cast<double>(W - 3), ymax = static_cast<double>(H - 3);
-  const double safe_x = static_cast<double>(fr.bx0 + 2), safe_y = static_cast<double>(fr.by0 + 2);
   const int base = -(fr.by0 + 2) * fr.bw - (fr.bx0 + 2);
   // Window origin (box-relative) from the strip's first pixel; the spare
   // column goes on the side the columns drift to (sx moves by m1 per row).
@@ -441,6 +441,8 @@ __device__ __forceinline__ bool strip_window_rows(const TileFrame& fr, const flo
     win[j][c] = tile[woff + j * fr.bw + c];
   });
   bool bad = false;
+  // 32-bit element offset of (x, y_first); row r adds r * W.
+  const unsigned o_first = __umul24(static_cast<unsigned>(y_first), static_cast<unsigned>(W)) + static_cast<unsigned>(x);
   static_for<kStripRows>([&](auto R) __attribute__((always_inline)) {
     constexpr int r = decltype(R)::value;
     // The row's offset passes through an opaque move so its loads stay in this
@@ -455,12 +457,15 @@ __device__ __forceinline__ bool strip_window_rows(const TileFrame& fr, const flo
     const double sx = (px + fr.m1 * static_cast<double>(y)) + fr.m2;
     const double sy = (py + fr.m4 * static_cast<double>(y)) + fr.m5;
     const bool ok = x < W && sx >= 2.0 && sx < xmax && sy >= 2.0 && sy < ymax;
-    const double sxc = ok ? sx : safe_x, syc = ok ? sy : safe_y;
-    const double fx0 = floor(sxc), fy0 = floor(syc);
+    // In the footprint s >= 2, so truncation is floor and fract(s) is exactly
+    // s - floor(s).  Outside it the weights, the window vote and the value are
+    // ignored; only the LDS fallback below must stay inside the box.
     float wx[6], wy[6];
-    lut_weights(lut, static_cast<float>(sxc - fx0), wx);
-    lut_weights(lut, static_cast<float>(syc - fy0), wy);
-    const int ix = static_cast<int>(fx0), iy = static_cast<int>(fy0);
+    lut_weights(lut, static_cast<float>(__builtin_amdgcn_fract(sx)), wx);
+    lut_weights(lut, static_cast<float>(__builtin_amdgcn_fract(sy)), wy);
+    int ix, iy;  // the hardware conversion saturates out-of-range coordinates
+    asm("v_cvt_i32_f64 %0, %1" : "=v"(ix) : "v"(sx));
+    asm("v_cvt_i32_f64 %0, %1" : "=v"(iy) : "v"(sy));
     const int o = ix - 2 - fr.bx0 - xb;
     const bool row_ok = iy - 2 - fr.by0 - yb == r;
     float value;
@@ -469,12 +474,13 @@ __device__ __forceinline__ bool strip_window_rows(const TileFrame& fr, const flo
     } else if (__all(!ok || (row_ok && o == 1))) {
       value = window_taps<r, 1>(win, wx, wy);
     } else {
-      value = lanczos_taps(tile + (iy * fr.bw + ix + base), fr.bw, wx, wy);
+      const int ixc = ok ? ix : fr.bx0 + 2, iyc = ok ? iy : fr.by0 + 2;
+      value = lanczos_taps(tile + (iyc * fr.bw + ixc + base), fr.bw, wx, wy);
     }
     value = ok ? value : fill;
     if (x < W) {
-      cov += ok ? (1ull << (4 * (kbase + r))) : 0ull;
-      st_global(fr.dst, static_cast<long long>(y) * W + x, value);
+      cov32 += ok ? (1u << (4 * r)) : 0u;
+      st_global_u32(fr.dst, o_first + static_cast<unsigned>(r * W), value);
       bad |= !isfinite(value);
     }
   });
@@ -596,7 +602,9 @@ __device__ __forceinline__ void lanczos_tile(const GetFrame& get, int count, con
     if (cur.use_tile) {
 #pragma unroll 1
       for (int s = 0; s < kLaneRows / kStripRows; ++s) {
-        bad |= strip_window_rows(cur, tile, lut, x, y_first + s * kStripRows, W, H, p.fill, s * kStripRows, cov);
+        unsigned cov32 = 0u;
+        bad |= strip_window_rows(cur, tile, lut, x, y_first + s * kStripRows, W, H, p.fill, cov32);
+        cov += static_cast<unsigned long long>(cov32) << (4 * kStripRows * s);
       }
     } else if (cur.fast) {
       bad = column_global_rows(cur, lut, x, y_first, W, H, p.fill, cov);
