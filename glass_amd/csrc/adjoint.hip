@@ -133,10 +133,10 @@ __device__ __forceinline__ void block_reduce_store(double (&acc)[K], double* par
 
 __global__ __launch_bounds__(kAdjBlock) void forward_kernel(const float* __restrict__ src, int h, int w, AffineMap m,
                                                            float fill, float* __restrict__ out) {
-  const long long n = static_cast<long long>(h) * w;
-  for (long long p = static_cast<long long>(blockIdx.x) * kAdjBlock + threadIdx.x; p < n;
-       p += static_cast<long long>(gridDim.x) * kAdjBlock) {
-    const int y = static_cast<int>(p / w), x = static_cast<int>(p - static_cast<long long>(y) * w);
+  // Blocks stride over rows, threads over columns: no per-pixel division.
+  for (int y = blockIdx.x; y < h; y += gridDim.x)
+  for (int x = threadIdx.x; x < w; x += kAdjBlock) {
+    const long long p = static_cast<long long>(y) * w + x;
     const Stencil st = stencil_at(m, x, y, h, w);
     float v = fill;
     if (st.ok) {
@@ -150,13 +150,12 @@ __global__ __launch_bounds__(kAdjBlock) void forward_kernel(const float* __restr
 __global__ __launch_bounds__(kAdjBlock) void backward_kernel(const float* __restrict__ src, int h, int w, AffineMap m,
                                                             const float* __restrict__ grad_out, float* grad_src,
                                                             double* partials) {
-  const long long n = static_cast<long long>(h) * w;
   double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  for (long long p = static_cast<long long>(blockIdx.x) * kAdjBlock + threadIdx.x; p < n;
-       p += static_cast<long long>(gridDim.x) * kAdjBlock) {
+  for (int y = blockIdx.x; y < h; y += gridDim.x)
+  for (int x = threadIdx.x; x < w; x += kAdjBlock) {
+    const long long p = static_cast<long long>(y) * w + x;
     const float g = grad_out[p];
     if (g == 0.0f) continue;
-    const int y = static_cast<int>(p / w), x = static_cast<int>(p - static_cast<long long>(y) * w);
     const Stencil st = stencil_at(m, x, y, h, w);
     if (!st.ok) continue;
     if (grad_src) {
@@ -190,13 +189,13 @@ __global__ __launch_bounds__(kAdjBlock) void normal_equations_kernel(const float
                                                                     const float* __restrict__ mov, int h, int w,
                                                                     AffineMap m, double cx, double cy,
                                                                     double* partials) {
-  const long long n = static_cast<long long>(h) * w;
   double acc[kNormalSums];
 #pragma unroll
   for (int k = 0; k < kNormalSums; ++k) acc[k] = 0.0;
-  for (long long p = static_cast<long long>(blockIdx.x) * kAdjBlock + threadIdx.x; p < n;
-       p += static_cast<long long>(gridDim.x) * kAdjBlock) {
-    const int y = static_cast<int>(p / w), x = static_cast<int>(p - static_cast<long long>(y) * w);
+  // Blocks stride over rows, threads over columns: no per-pixel division.
+  for (int y = blockIdx.x; y < h; y += gridDim.x)
+  for (int x = threadIdx.x; x < w; x += kAdjBlock) {
+    const long long p = static_cast<long long>(y) * w + x;
     const Stencil st = stencil_at(m, x, y, h, w);
     if (!st.ok) continue;
     const float r0 = ref[p];
@@ -220,12 +219,24 @@ __global__ __launch_bounds__(kAdjBlock) void normal_equations_kernel(const float
   block_reduce_store<kNormalSums>(acc, partials);
 }
 
-__global__ void reduce_partials_kernel(const double* partials, int blocks, int k, double* out) {
-  const int c = threadIdx.x;
-  if (c >= k) return;
-  double s = 0.0;
-  for (int b = 0; b < blocks; ++b) s += partials[static_cast<long long>(b) * k + c];
-  out[c] = s;
+// Column c of the per-block partials (blocks rows of k sums) in block c:
+// thread t sums rows t, t + 256, ..., then a fixed-order tree in LDS, so the
+// result is deterministic and the sums take a few load latencies, not one per
+// row.
+constexpr int kReduceThreads = 256;
+__global__ __launch_bounds__(kReduceThreads) void reduce_partials_kernel(const double* partials, int blocks, int k,
+                                                                         double* out) {
+  __shared__ double part[kReduceThreads];
+  const int c = blockIdx.x;
+  double v = 0.0;
+  for (int b = threadIdx.x; b < blocks; b += kReduceThreads) v += partials[static_cast<long long>(b) * k + c];
+  part[threadIdx.x] = v;
+  __syncthreads();
+  for (int width = kReduceThreads / 2; width > 0; width >>= 1) {
+    if (static_cast<int>(threadIdx.x) < width) part[threadIdx.x] += part[threadIdx.x + width];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = part[0];
 }
 
 unsigned adjoint_blocks(long long n) {
@@ -251,7 +262,7 @@ void launch_affine_backward(const float* src, int h, int w, const AffineMap& m, 
                             float* grad_src, double* workspace, double* grad_map, hipStream_t stream) {
   const unsigned blocks = adjoint_blocks(static_cast<long long>(h) * w);
   backward_kernel<<<blocks, kAdjBlock, 0, stream>>>(src, h, w, m, grad_out, grad_src, grad_map ? workspace : nullptr);
-  if (grad_map) reduce_partials_kernel<<<1, 64, 0, stream>>>(workspace, blocks, 6, grad_map);
+  if (grad_map) reduce_partials_kernel<<<6, kReduceThreads, 0, stream>>>(workspace, blocks, 6, grad_map);
   check(hipGetLastError(), "affine warp backward launch");
 }
 
@@ -259,7 +270,7 @@ void launch_normal_equations(const float* ref, const float* mov, int h, int w, c
                              double cy, double* workspace, double* sums, hipStream_t stream) {
   const unsigned blocks = adjoint_blocks(static_cast<long long>(h) * w);
   normal_equations_kernel<<<blocks, kAdjBlock, 0, stream>>>(ref, mov, h, w, m, cx, cy, workspace);
-  reduce_partials_kernel<<<1, 64, 0, stream>>>(workspace, blocks, kNormalSums, sums);
+  reduce_partials_kernel<<<kNormalSums, kReduceThreads, 0, stream>>>(workspace, blocks, kNormalSums, sums);
   check(hipGetLastError(), "normal equations launch");
 }
 
