@@ -96,19 +96,19 @@ __device__ __forceinline__ float load_raw1(const void* src, long long i) {
 // bias/dark/flat once and calibrates them in every frame of the batch, so the
 // masters cost 12 B per pixel per batch rather than per frame.  Raw loads are
 // issued kCalAhead frames at a time ahead of the arithmetic; calibrated frames
-// are written with non-temporal stores (they are read again only by the next
-// pass over the stack).  STATS adds the fused per-frame FP64 moments
+// are written with plain stores (GH_CAL_NT=1 selects non-temporal ones, which
+// measured slower on MI355X).  STATS adds the fused per-frame FP64 moments
 // (partials[k * gridDim.x + blockIdx.x]); MB is the frames per launch (the
 // statistics variant keeps 4 frames of FP64 sums in registers; the host splits
 // larger batches).
 #ifndef GH_CAL_PX
-#define GH_CAL_PX 4
+#define GH_CAL_PX 8
 #endif
 #ifndef GH_CAL_AHEAD
-#define GH_CAL_AHEAD 4
+#define GH_CAL_AHEAD 8
 #endif
 #ifndef GH_CAL_NT
-#define GH_CAL_NT 1
+#define GH_CAL_NT 0
 #endif
 constexpr int kCalPx = GH_CAL_PX;
 constexpr int kCalAhead = GH_CAL_AHEAD;

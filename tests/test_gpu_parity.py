@@ -130,6 +130,32 @@ def test_calibrate_device_batches_flags_and_stats(gpu, stats):
             assert st[i]["std"] == pytest.approx(ref["std"], rel=1e-9)
 
 
+def test_calibrate_device_pixel_counts_off_the_batch_width(gpu):
+    # 6 x 10 = 60 px (a multiple of 4, not of 8) and 7 x 9 = 63 px take the
+    # per-frame kernels instead of the 8-pixel batched one; device raw frames
+    # sit at a padded 128-byte stride.
+    import torch
+
+    rng = np.random.default_rng(9)
+    for h, w in ((6, 10), (7, 9)):
+        f = 3
+        adu = rng.integers(500, 60000, size=(f, h, w), dtype=np.uint16)
+        padded = np.zeros((f, 128), np.uint8)
+        padded[:, :2 * h * w] = encode_fits_u16be_bzero(adu).reshape(f, -1)
+        raw = torch.from_numpy(padded).cuda()
+        bias = rng.uniform(90, 110, (h, w)).astype(np.float32)
+        dark = rng.uniform(100, 140, (h, w)).astype(np.float32)
+        flat = rng.uniform(0.5, 1.5, (h, w)).astype(np.float32)
+        with ResidentCalibratedStack(f, h, w) as s:
+            s.set_calibration_masters(bias, dark, flat)
+            s.calibrate_frames_device(list(range(f)), raw, "fits_u16be_bzero", [30.0] * f, 60.0,
+                                      {"master_dark_includes_bias": False}, frame_stride_bytes=128)
+            got = s.download_frames_tile(list(range(f)), 0, 0, w, h)
+        for i in range(f):
+            exp = oracle.calibrate_light(adu[i], bias, dark, flat, 30.0, 60.0, master_dark_includes_bias=False)
+            assert np.array_equal(got[i], exp), (h, w, i)
+
+
 # ----------------------------------------------------------------- masters
 def test_masters_golden_314_bit_exact(gpu, golden_314):
     g = golden_314
