@@ -284,6 +284,30 @@ def test_warp_lanczos3_cpu_semantics_nonfinite_sources(gpu):
     assert np.array_equal(shifted[fin], src[fin]) and (~fin).sum() > 25 * 20
 
 
+@pytest.mark.parametrize("interp", ["lanczos3", "bilinear"])
+def test_warp_edge_geometries(gpu, interp):
+    # Frames smaller than the tap window and than one output tile, a 90 degree
+    # rotation, a map that leaves every output pixel outside the footprint, and
+    # a projective map (direct kernel), against the oracle with NaN fill.
+    rng = np.random.default_rng(14)
+    proj = np.array([[1.0, 0.002, 1.5], [-0.003, 1.0, -2.0], [2e-5, -1e-5, 1.0]])
+    cases = [((5, 7), _rot(0.3, 0.4, -0.2)), ((6, 6), _rot(0.0, 1.0, 0.0)), ((20, 300), _rot(90.0, 160.0, -140.0)),
+             ((150, 300), _rot(0.5, 1000.0, 0.0)), ((70, 90), proj), ((37, 250), _rot(-1.0, 3.3, 0.7, 0.9))]
+    for (h, w), m in cases:
+        frame = rng.normal(100, 10, size=(h, w)).astype(np.float32)
+        with _stack_of(frame[None]) as s:
+            s.reset_warp_coverage()
+            s.apply_matrix_frames_cpu_semantics([0], np.asarray(m)[None], interp, fill=np.nan)
+            got = s.download_frame(0)
+            geo = s.warp_coverage_map()
+        exp, cov = oracle.warp_matrix(frame, m, fill=np.nan, interpolation=interp)
+        fin = np.isfinite(exp)
+        assert np.array_equal(np.isfinite(got), fin), (h, w)
+        assert np.array_equal(geo, cov), (h, w)
+        tol = 0.0 if interp == "bilinear" else 2e-4
+        assert np.allclose(got[fin], exp[fin], rtol=0, atol=tol), (h, w, np.max(np.abs(got[fin] - exp[fin]), initial=0))
+
+
 def test_warp_lanczos3_resident_tiled_matches_oracle(gpu):
     rng = np.random.default_rng(12)
     h, w = 100, 260
