@@ -8,7 +8,7 @@ point raises, so nothing silently runs on the CPU.
 from __future__ import annotations
 
 import ctypes
-from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_int32, c_int64, c_size_t, c_uint32, c_void_p
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_int32, c_int64, c_size_t, c_uint32, c_uint64, c_void_p
 from pathlib import Path
 
 LIBRARY_PATH = Path(__file__).resolve().parent / "lib" / "libglass_hip.so"
@@ -155,6 +155,13 @@ _SIGNATURES = {
     "gh_warp_coverage_map": [c_void_p, c_void_p],
     "gh_integrate": [c_void_p, POINTER(c_float), POINTER(IntegrateParams), POINTER(IntegrateOutputs), c_void_p],
     "gh_frame_stats": [c_void_p, c_int64, c_int32, POINTER(FrameStatistics)],
+    "gh_pair_grid_stats": [
+        c_void_p, c_int64, POINTER(c_int64), c_int64, c_int32, c_int32, POINTER(c_float), POINTER(c_float),
+        POINTER(c_float), POINTER(c_float), POINTER(c_uint64),
+    ],
+    "gh_apply_normalization": [
+        c_void_p, POINTER(c_int64), c_int64, c_void_p, c_void_p, c_int32, c_int32, c_void_p,
+    ],
     "gh_calibration_stats": [c_void_p, c_int64, POINTER(c_double), POINTER(c_double), POINTER(c_int64)],
     "gh_synchronize": [c_void_p],
     "gh_detect_stars": [c_void_p, c_int64, c_double, c_int64, c_double, c_double, c_int32, c_double, c_int64,

@@ -1,0 +1,157 @@
+// C ABI of the resident local normalisation (include/glass_hip.h):
+// pair-grid statistics and the in-place global / grid application
+// (native_bindings.cpp:10645-11290).
+
+#include <cmath>
+
+#include "stack.hpp"
+
+using ghi::guarded;
+
+namespace {
+
+// Tiles of the statistics / coefficient grid; a tile dimension beyond the
+// frame maps every pixel to tile 0 on that axis, so it is clamped to the frame
+// (same grid, no 32-bit overflow of tile corners).
+struct LnGrid {
+  int tile_h, tile_w, rows, cols;
+};
+
+LnGrid ln_grid(const gh_stack* s, int32_t tile_height, int32_t tile_width) {
+  if (tile_height <= 0 || tile_width <= 0) throw gh::ValueError("tile dimensions must be positive");
+  LnGrid g{};
+  g.tile_h = static_cast<int>(std::min<int64_t>(tile_height, s->height));
+  g.tile_w = static_cast<int>(std::min<int64_t>(tile_width, s->width));
+  g.rows = static_cast<int>((s->height + g.tile_h - 1) / g.tile_h);
+  g.cols = static_cast<int>((s->width + g.tile_w - 1) / g.tile_w);
+  if (g.rows <= 0 || g.cols <= 0)
+    throw std::runtime_error("resident frame shape produced an empty local-normalization grid");
+  return g;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gh_pair_grid_stats(gh_stack* s, int64_t reference_index, const int64_t* source_indices, int64_t count,
+                       int32_t tile_height, int32_t tile_width, float* source_mean, float* source_std,
+                       float* reference_mean, float* reference_std, uint64_t* valid_pixels) {
+  return guarded([&] {
+    s->require_loaded(reference_index, "grid local normalization reference statistics");
+    if (count < 0) throw gh::ValueError("source count must not be negative");
+    if (count > 0 && !source_indices) throw gh::ValueError("source indices pointer is NULL");
+    const LnGrid g = ln_grid(s, tile_height, tile_width);
+    for (int64_t k = 0; k < count; ++k) {
+      if (source_indices[k] < 0 || source_indices[k] >= s->frames)
+        throw gh::IndexError("source index is out of resident stack range");
+      s->require_loaded(source_indices[k], "grid local normalization source statistics");
+    }
+    if (count == 0) return;
+    const int64_t tiles = static_cast<int64_t>(g.rows) * g.cols;
+    const int64_t chunk = std::min<int64_t>(count, gh::kLnBatch);
+    s->ln_sums.ensure(static_cast<size_t>(chunk * tiles * 4), "hipMalloc(local normalization sums)");
+    s->ln_counts.ensure(static_cast<size_t>(chunk * tiles), "hipMalloc(local normalization counts)");
+    std::vector<double> sums(static_cast<size_t>(chunk * tiles * 4));
+    std::vector<unsigned long long> counts(static_cast<size_t>(chunk * tiles));
+    s->wait_caller();
+    for (int64_t start = 0; start < count; start += chunk) {
+      const int n = static_cast<int>(std::min<int64_t>(chunk, count - start));
+      gh::LnBatch b{};
+      for (int j = 0; j < n; ++j) b.frame[j] = s->slot[source_indices[start + j]];
+      gh::launch_pair_grid_sums(b, n, s->slot[reference_index], static_cast<int>(s->width),
+                                static_cast<int>(s->height), g.tile_w, g.tile_h, g.cols, g.rows, s->ln_sums.ptr,
+                                s->ln_counts.ptr, 0, s->stream);
+      check(hipMemcpyAsync(sums.data(), s->ln_sums.ptr, static_cast<size_t>(n * tiles * 4) * sizeof(double),
+                           hipMemcpyDeviceToHost, s->stream),
+            "hipMemcpy(local normalization sums)");
+      check(hipMemcpyAsync(counts.data(), s->ln_counts.ptr,
+                           static_cast<size_t>(n * tiles) * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           s->stream),
+            "hipMemcpy(local normalization counts)");
+      check(hipStreamSynchronize(s->stream), "pair grid statistics");
+      // Means and standard deviations as native_bindings.cpp:10740-10768:
+      // FP64 moments about zero, variance clamped at 0, rounded to float32.
+      for (int64_t r = 0; r < n * tiles; ++r) {
+        const int64_t o = start * tiles + r;
+        const unsigned long long c = counts[r];
+        float sm = 0.0f, sd = 0.0f, rm = 0.0f, rd = 0.0f;
+        if (c != 0) {
+          const double inv = 1.0 / static_cast<double>(c);
+          const double s_mean = sums[r * 4 + 0] * inv;
+          const double r_mean = sums[r * 4 + 2] * inv;
+          double s_var = sums[r * 4 + 1] * inv - s_mean * s_mean;
+          double r_var = sums[r * 4 + 3] * inv - r_mean * r_mean;
+          if (s_var < 0.0) s_var = 0.0;
+          if (r_var < 0.0) r_var = 0.0;
+          sm = static_cast<float>(s_mean);
+          sd = static_cast<float>(std::sqrt(s_var));
+          rm = static_cast<float>(r_mean);
+          rd = static_cast<float>(std::sqrt(r_var));
+        }
+        if (source_mean) source_mean[o] = sm;
+        if (source_std) source_std[o] = sd;
+        if (reference_mean) reference_mean[o] = rm;
+        if (reference_std) reference_std[o] = rd;
+        if (valid_pixels) valid_pixels[o] = c;
+      }
+    }
+  });
+}
+
+int gh_apply_normalization(gh_stack* s, const int64_t* indices, int64_t count, const float* scales,
+                           const float* offsets, int32_t tile_height, int32_t tile_width, void* stream_ptr) {
+  return guarded([&] {
+    if (count < 0) throw gh::ValueError("frame count must not be negative");
+    const LnGrid g = ln_grid(s, tile_height, tile_width);
+    if (count == 0) return;
+    if (!indices) throw gh::ValueError("frame indices pointer is NULL");
+    if (!scales || !offsets) throw gh::ValueError("scales and offsets must not be NULL");
+    for (int64_t k = 0; k < count; ++k) {
+      if (indices[k] < 0 || indices[k] >= s->frames) throw gh::IndexError("frame index is out of resident stack range");
+      s->require_loaded(indices[k], "grid local normalization");
+    }
+    const int64_t grid = static_cast<int64_t>(g.rows) * g.cols;
+    const bool dev_s = is_device_pointer(scales), dev_o = is_device_pointer(offsets);
+    if (dev_s != dev_o) throw gh::ValueError("scales and offsets must both be host or both be device pointers");
+    hipStream_t st = s->pick(stream_ptr);
+    const float* sc = scales;
+    const float* of = offsets;
+    if (!dev_s) {
+      // Host coefficients are staged in the stack's buffer; the call then
+      // completes before returning, so the buffer is free for the next call.
+      const size_t n = static_cast<size_t>(count * grid);
+      s->ln_coef.ensure(2 * n, "hipMalloc(local normalization coefficients)");
+      check(hipMemcpyAsync(s->ln_coef.ptr, scales, n * sizeof(float), hipMemcpyHostToDevice, st),
+            "hipMemcpy(local normalization scales)");
+      check(hipMemcpyAsync(s->ln_coef.ptr + n, offsets, n * sizeof(float), hipMemcpyHostToDevice, st),
+            "hipMemcpy(local normalization offsets)");
+      sc = s->ln_coef.ptr;
+      of = s->ln_coef.ptr + n;
+    }
+    if (st == s->stream) s->wait_caller();
+    // A frame named twice is normalised twice, in order, as consecutive
+    // launches do in the reference: a repeat starts a new launch.
+    int64_t start = 0;
+    while (start < count) {
+      gh::LnBatch b{};
+      int n = 0;
+      while (start + n < count && n < gh::kLnBatch) {
+        const int64_t f = indices[start + n];
+        bool repeat = false;
+        for (int j = 0; j < n; ++j) repeat = repeat || b.frame[j] == s->slot[f];
+        if (repeat) break;
+        b.frame[n] = s->slot[f];
+        b.flag[n] = s->nonfinite + f;
+        ++n;
+      }
+      gh::launch_apply_grid(b, n, sc, of, static_cast<int>(s->width), static_cast<int>(s->height), g.tile_w,
+                            g.tile_h, g.cols, g.rows, static_cast<int>(start), st);
+      for (int j = 0; j < n; ++j) s->calib_stats_valid[indices[start + j]] = 0;
+      start += n;
+    }
+    s->mark_caller(st);
+    if (!dev_s || !stream_ptr) check(hipStreamSynchronize(st), "apply normalization");
+  });
+}
+
+}  // extern "C"

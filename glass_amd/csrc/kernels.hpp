@@ -192,6 +192,25 @@ struct VariantJob {
 };
 void launch_decide_variants(const VariantJob* jobs, int n_jobs, int* out, hipStream_t stream);
 
+// Resident local normalisation.  A batch names up to kLnBatch resident
+// frames (sources for the statistics, targets for the application) and their
+// non-finite flags (application only).
+constexpr int kLnBatch = 64;
+constexpr int kLnGroup = 4;  // sources per statistics block (one reference read)
+struct LnBatch {
+  const float* frame[kLnBatch];
+  unsigned* flag[kLnBatch];
+};
+// Per-tile FP64 sums (s, s^2, r, r^2) and counts of (source, reference) pairs
+// over pixels finite in both; batch entry j writes record (first + j).
+void launch_pair_grid_sums(const LnBatch& b, int n, const float* reference, int width, int height, int tile_w,
+                           int tile_h, int grid_cols, int grid_rows, double* sums, unsigned long long* counts,
+                           int first, hipStream_t stream);
+// In place x * scale + offset with per-tile coefficients (device grids
+// [first + j][grid_rows][grid_cols] for batch entry j).
+void launch_apply_grid(const LnBatch& b, int n, const float* scales, const float* offsets, int width, int height,
+                       int tile_w, int tile_h, int grid_cols, int grid_rows, int first, hipStream_t stream);
+
 // Frame-count limits of the winsorized integration.
 int integrate_max_frames();         // positive-weight frames the winsorized path accepts
 int integrate_register_capacity();  // frames the register kernels hold; frame tables pad to it

@@ -113,6 +113,9 @@ struct gh_stack {
   DeviceBuffer<float> star_windows;         // gathered candidate windows
   DeviceBuffer<unsigned> warp_flags;        // [kWarpBatch] source-flag snapshot
   DeviceBuffer<float> lanczos_lut;          // normalised Lanczos3 weight cubics
+  DeviceBuffer<double> ln_sums;             // local normalisation: per-tile FP64 sums
+  DeviceBuffer<unsigned long long> ln_counts;  //   and paired-pixel counts
+  DeviceBuffer<float> ln_coef;              //   host coefficients staged for the application
 
   hipStream_t pick(void* s) const { return s ? static_cast<hipStream_t>(s) : stream; }
 
@@ -211,6 +214,9 @@ struct gh_stack {
     star_cand.release();
     star_windows.release();
     adj_ws.release();
+    ln_sums.release();
+    ln_counts.release();
+    ln_coef.release();
     if (caller_done) (void)hipEventDestroy(caller_done);
     if (stream) (void)hipStreamDestroy(stream);
   }

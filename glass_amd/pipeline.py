@@ -13,6 +13,7 @@ Semantics follow the CPU path, which is the oracle:
   are resampled on the reference grid; geometric coverage is accumulated;
 * weights: engine/integration.py:80-106 (none / variance_aware / simple_snr),
   with zero weight for masked frames;
+* optional resident local normalisation (glass_amd.local_norm);
 * integration: cpu/integration.py:112-194; DQ: resident_cuda.py:3110-3297.
 All arithmetic runs in libglass_hip.so; numpy only shapes host inputs/outputs.
 """
@@ -25,6 +26,7 @@ from typing import Sequence
 
 import numpy as np
 
+from .local_norm import MODES as LN_MODES, resident_local_normalization
 from .resident import ResidentCalibratedStack
 
 DEFAULT_POLICY = {
@@ -216,8 +218,18 @@ def run_resident_stack(
     policy: dict | None = None,
     raw_format: str = "float32",
     warp_semantics: str = "cpu",
+    local_normalization: str = "off",
+    local_normalization_tile_size: int = 512,
+    reference_index: int = 0,
 ) -> StackResult:
-    """One resident stacking job from host frames (the drop-in driver)."""
+    """One resident stacking job from host frames (the drop-in driver).
+
+    ``local_normalization``: "off", "global_mean_std" or "grid_mean_std" --
+    frames are normalised in place to frame ``reference_index`` after the
+    weights and before integration (resident_cuda.py:12902-13110); frames left
+    without pixels get weight 0."""
+    if local_normalization not in ("off", *LN_MODES):
+        raise ValueError("local_normalization must be off, global_mean_std or grid_mean_std")
     pol = _policy(policy)
     lights = list(lights)
     f = len(lights)
@@ -255,6 +267,13 @@ def run_resident_stack(
             weights = variance_aware_weights([s["std"] for s in stats], active)
         else:
             raise ValueError(f"unsupported weighting: {weighting}")
+        timing_ln = 0.0
+        if local_normalization != "off":
+            t_ln = time.perf_counter()
+            ln = resident_local_normalization(stack, reference_index, weights, local_normalization,
+                                              local_normalization_tile_size)
+            weights = ln["weights"]
+            timing_ln = time.perf_counter() - t_ln
         n_active = int(np.count_nonzero(weights > 0))
         master, weight, coverage, low, high, prof = stack.integrate_hardened_winsorized_sigma_timed(
             weights, low_sigma, high_sigma, min_samples, max_reject_fraction)
@@ -262,7 +281,8 @@ def run_resident_stack(
         t_int = time.perf_counter()
         dq = dq_map(stack, weights, low_sigma, high_sigma, min_samples, max_reject_fraction, n_active)
         return StackResult(master, weight, coverage, low, high, dq, geometric, weights,
-                           {"calibrate_s": t_cal - t0, "warp_s": t_warp - t_cal, "integrate_s": t_int - t_warp})
+                           {"calibrate_s": t_cal - t0, "warp_s": t_warp - t_cal, "local_norm_s": timing_ln,
+                            "integrate_s": t_int - t_warp - timing_ln})
     finally:
         stack.close()
 

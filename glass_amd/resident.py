@@ -417,6 +417,183 @@ class ResidentCalibratedStack:
             out["mad"] = st.mad
         return out
 
+    # ------------------------------------------------- local normalisation
+    def _grid_shape(self, tile_height: int, tile_width: int) -> tuple[int, int]:
+        if int(tile_height) <= 0 or int(tile_width) <= 0:
+            raise ValueError("tile dimensions must be positive")
+        return ((self.height + int(tile_height) - 1) // int(tile_height),
+                (self.width + int(tile_width) - 1) // int(tile_width))
+
+    def _pair_grid_stats(self, reference_index: int, sources: np.ndarray, tile_height: int, tile_width: int):
+        rows, cols = self._grid_shape(tile_height, tile_width)
+        n = int(sources.size)
+        out = {k: np.zeros((n, rows, cols), dtype=np.float32)
+               for k in ("source_mean", "source_std", "reference_mean", "reference_std")}
+        valid = np.zeros((n, rows, cols), dtype=np.uint64)
+        fp = ctypes.POINTER(ctypes.c_float)
+        t0 = time.perf_counter()
+        N.call("gh_pair_grid_stats", self._h, int(reference_index), _i64p(sources), n, int(tile_height),
+               int(tile_width), out["source_mean"].ctypes.data_as(fp), out["source_std"].ctypes.data_as(fp),
+               out["reference_mean"].ctypes.data_as(fp), out["reference_std"].ctypes.data_as(fp),
+               valid.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+        return out, valid, rows, cols, time.perf_counter() - t0
+
+    def frame_pair_grid_stats(self, reference_index: int, source_index: int, tile_height: int,
+                              tile_width: int) -> dict:
+        """Per-tile float32 mean / std of a (source, reference) pair over the
+        pixels finite in both (native_bindings.cpp:10645-10784)."""
+        src = np.asarray([int(source_index)], dtype=np.int64)
+        out, valid, rows, cols, _ = self._pair_grid_stats(reference_index, src, tile_height, tile_width)
+        result = {k: v[0] for k, v in out.items()}
+        result.update({
+            "valid_pixels": valid[0], "grid_rows": rows, "grid_cols": cols, "tile_height": int(tile_height),
+            "tile_width": int(tile_width), "valid_pixel_total": int(valid[0].sum(dtype=np.uint64)),
+            "model": "resident_grid_pair_mean_std",
+        })
+        return result
+
+    def frame_pair_grid_stats_batch(self, reference_index: int, source_indices, tile_height: int,
+                                    tile_width: int) -> dict:
+        """frame_pair_grid_stats for many sources in one pass over the
+        reference (native_bindings.cpp:10786-11021)."""
+        arr = np.asarray(source_indices)
+        if arr.ndim != 1:
+            raise ValueError("source_indices must be a one-dimensional array")
+        src = np.ascontiguousarray(arr, dtype=np.int64)
+        t0 = time.perf_counter()
+        out, valid, rows, cols, call_s = self._pair_grid_stats(reference_index, src, tile_height, tile_width)
+        frames = []
+        for pos in range(src.size):
+            frame = {k: v[pos] for k, v in out.items()}
+            frame.update({
+                "valid_pixels": valid[pos], "grid_rows": rows, "grid_cols": cols,
+                "tile_height": int(tile_height), "tile_width": int(tile_width),
+                "valid_pixel_total": int(valid[pos].sum(dtype=np.uint64)),
+                "model": "resident_grid_pair_mean_std", "source_index": int(src[pos]),
+                "reference_index": int(reference_index), "batch_position": pos,
+            })
+            frames.append(frame)
+        grid = rows * cols
+        return {
+            "schema_version": 1, "model": "resident_grid_pair_mean_std_batch",
+            "batch_model": "single_kernel_source_frame_tile_grid", "reference_index": int(reference_index),
+            "source_count": int(src.size), "grid_rows": rows, "grid_cols": cols, "grid_count": grid,
+            "tile_height": int(tile_height), "tile_width": int(tile_width),
+            "source_indices": arr.astype(np.int32, copy=False),
+            "allocation_s": 0.0, "index_upload_s": 0.0, "kernel_enqueue_s": call_s, "sync_s": 0.0,
+            "download_s": 0.0, "total_s": time.perf_counter() - t0,
+            "download_bytes": int(src.size) * grid * (4 * 8 + 8), "index_bytes": int(src.size) * 4,
+            "frames": frames,
+        }
+
+    def _apply_normalization(self, indices: np.ndarray, scales, offsets, tile_height: int, tile_width: int,
+                             stream: int | None) -> float:
+        count = int(indices.size)
+        rows, cols = self._grid_shape(tile_height, tile_width)
+        n = count * rows * cols
+        if not _is_torch_cuda(scales):
+            scales = np.asarray(scales, dtype=np.float32).reshape(-1)
+        if not _is_torch_cuda(offsets):
+            offsets = np.asarray(offsets, dtype=np.float32).reshape(-1)
+        sp, keep_s = _buffer(scales, np.float32, (n,), "scales")
+        op, keep_o = _buffer(offsets, np.float32, (n,), "offsets")
+        t0 = time.perf_counter()
+        N.call("gh_apply_normalization", self._h, _i64p(indices), count, sp, op, int(tile_height), int(tile_width),
+               ctypes.c_void_p(stream) if stream else None)
+        del keep_s, keep_o
+        return time.perf_counter() - t0
+
+    def apply_global_normalization_frame(self, index: int, scale: float, offset: float,
+                                         stream: int | None = None) -> dict:
+        """x * scale + offset in place over the whole frame, float32
+        coefficients (native_bindings.cpp:11023-11052)."""
+        t0 = time.perf_counter()
+        idx = np.asarray([int(index)], dtype=np.int64)
+        call_s = self._apply_normalization(idx, np.float32([[[scale]]]), np.float32([[[offset]]]), self.height,
+                                           self.width, stream)
+        return {
+            "schema_version": 1, "mode": "in_place_device_update", "model": "resident_global_mean_std",
+            "frame_index": int(index), "frame_bytes": self.pixels_per_frame * 4, "temporary_output_bytes": 0,
+            "device_to_device_copy_s": 0.0, "kernel_enqueue_s": call_s, "sync_s": 0.0,
+            "total_s": time.perf_counter() - t0,
+        }
+
+    @staticmethod
+    def _coefficients(array, ndim: int, what: str):
+        shape = tuple(array.shape) if _is_torch_cuda(array) else np.asarray(array).shape
+        if len(shape) != ndim:
+            raise ValueError(what)
+        return shape
+
+    def apply_grid_normalization_frame(self, index: int, scales, offsets, tile_height: int, tile_width: int,
+                                       stream: int | None = None) -> dict:
+        """x * scales[tile] + offsets[tile] in place, piecewise constant over
+        the tiles (native_bindings.cpp:11054-11151)."""
+        s_shape = self._coefficients(scales, 2, "scales and offsets must have shape (grid_rows, grid_cols)")
+        o_shape = self._coefficients(offsets, 2, "scales and offsets must have shape (grid_rows, grid_cols)")
+        if s_shape != o_shape:
+            raise ValueError("scales and offsets shapes must match")
+        rows, cols = self._grid_shape(tile_height, tile_width)
+        if s_shape[0] <= 0 or s_shape[1] <= 0:
+            raise ValueError("coefficient grid must not be empty")
+        if s_shape != (rows, cols):
+            raise ValueError("coefficient grid shape does not match resident frame shape and tile dimensions")
+        t0 = time.perf_counter()
+        idx = np.asarray([int(index)], dtype=np.int64)
+        call_s = self._apply_normalization(idx, scales, offsets, tile_height, tile_width, stream)
+        grid = rows * cols
+        return {
+            "schema_version": 1, "mode": "in_place_device_update", "model": "resident_grid_mean_std",
+            "frame_index": int(index), "frame_bytes": self.pixels_per_frame * 4, "temporary_output_bytes": 0,
+            "coefficient_count": grid, "coefficient_bytes": grid * 8, "coefficient_alloc_s": 0.0,
+            "coefficient_upload_s": 0.0, "device_to_device_copy_s": 0.0, "kernel_enqueue_s": call_s,
+            "sync_s": 0.0, "total_s": time.perf_counter() - t0,
+        }
+
+    def apply_grid_normalization_frames(self, frame_indices, scales, offsets, tile_height: int, tile_width: int,
+                                        stream: int | None = None) -> dict:
+        """apply_grid_normalization_frame for many frames in one launch per
+        batch of distinct frames (native_bindings.cpp:11153-11290)."""
+        arr = np.asarray(frame_indices)
+        if arr.ndim != 1:
+            raise ValueError("frame_indices must be a one-dimensional array")
+        s_shape = self._coefficients(scales, 3, "scales and offsets must have shape (frame_count, grid_rows, grid_cols)")
+        o_shape = self._coefficients(offsets, 3, "scales and offsets must have shape (frame_count, grid_rows, grid_cols)")
+        if s_shape[0] != arr.size or o_shape[0] != arr.size:
+            raise ValueError("coefficient frame count must match frame_indices length")
+        if s_shape[1:] != o_shape[1:]:
+            raise ValueError("scales and offsets grid shapes must match")
+        if int(tile_height) <= 0 or int(tile_width) <= 0:
+            raise ValueError("tile dimensions must be positive")
+        count = int(arr.size)
+        rows, cols = int(s_shape[1]), int(s_shape[2])
+        base = {"schema_version": 1, "mode": "in_place_device_update_batch",
+                "model": "resident_grid_mean_std_batch_apply", "frame_count": count, "grid_rows": rows,
+                "grid_cols": cols}
+        if count == 0:
+            return {**base, "grid_count": 0, "coefficient_count": 0, "coefficient_bytes": 0,
+                    "coefficient_alloc_s": 0.0, "coefficient_upload_s": 0.0, "kernel_enqueue_s": 0.0,
+                    "sync_s": 0.0, "total_s": 0.0, "frames": []}
+        if rows <= 0 or cols <= 0:
+            raise ValueError("coefficient grid must not be empty")
+        if (rows, cols) != self._grid_shape(tile_height, tile_width):
+            raise ValueError("coefficient grid shape does not match resident frame shape and tile dimensions")
+        idx = np.ascontiguousarray(arr, dtype=np.int64)
+        t0 = time.perf_counter()
+        call_s = self._apply_normalization(idx, scales, offsets, tile_height, tile_width, stream)
+        total_s = time.perf_counter() - t0
+        grid = rows * cols
+        frames = [{
+            "schema_version": 1, "mode": "in_place_device_update_batch", "model": "resident_grid_mean_std",
+            "frame_index": int(idx[pos]), "batch_position": pos, "frame_bytes": self.pixels_per_frame * 4,
+            "temporary_output_bytes": 0, "coefficient_count": grid, "coefficient_bytes": grid * 8,
+            "coefficient_alloc_s": 0.0, "coefficient_upload_s": 0.0, "device_to_device_copy_s": 0.0,
+            "kernel_enqueue_s": call_s / count, "sync_s": 0.0, "total_s": total_s / count,
+        } for pos in range(count)]
+        return {**base, "grid_count": grid, "coefficient_count": count * grid, "coefficient_bytes": count * grid * 8,
+                "coefficient_alloc_s": 0.0, "coefficient_upload_s": 0.0, "kernel_enqueue_s": call_s,
+                "sync_s": 0.0, "total_s": total_s, "frame_indices": arr, "frames": frames}
+
     # ------------------------------------------------------------ integrate
     def _integrate(self, weights, params: N.IntegrateParams, want, count_dtype, dq=False, dq_geometric=False,
                    dq_active_frames=0, device_outputs: dict | None = None, stream=None):

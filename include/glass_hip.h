@@ -187,6 +187,35 @@ int gh_integrate(gh_stack* stack, const float* weights, const gh_integrate_param
  * MAD (src/glass/engine/quality.py:126-200).  FP64. */
 int gh_frame_stats(gh_stack* stack, int64_t index, int32_t exact_median, gh_frame_statistics* out);
 
+/* frame_pair_grid_stats(reference, source, tile_height, tile_width) and
+ * frame_pair_grid_stats_batch(reference, sources, tile_height, tile_width)
+ * (native_bindings.cpp:10645-10784, :10786-11021; kernel
+ * cpp/cuda/local_norm_kernels.cu:220-304): per tile of the
+ * ceil(H / tile_height) x ceil(W / tile_width) grid (the last row / column of
+ * tiles ends at the frame edge), over the pixels finite in both the source and
+ * the reference frame, FP64 sums turned into float32 mean and standard
+ * deviation of each frame (variance clamped at 0; all 0 for an empty tile) and
+ * the paired pixel count.  Outputs are host arrays [count][grid_rows][grid_cols];
+ * any may be NULL.  Synchronous. */
+int gh_pair_grid_stats(gh_stack* stack, int64_t reference_index, const int64_t* source_indices, int64_t count,
+                       int32_t tile_height, int32_t tile_width, float* source_mean, float* source_std,
+                       float* reference_mean, float* reference_std, uint64_t* valid_pixels);
+
+/* apply_global_normalization_frame(index, scale, offset) and
+ * apply_grid_normalization_frame[s](indices, scales, offsets, tile_height,
+ * tile_width) (native_bindings.cpp:11023-11052, :11054-11151, :11153-11290;
+ * kernels cpp/cuda/local_norm_kernels.cu:6-71): frame indices[k] becomes, in
+ * place, x * scale + offset (product and sum rounded separately) with the
+ * coefficients of tile (min(y / tile_height, rows - 1), min(x / tile_width,
+ * cols - 1)) of grid k, for the grid of gh_pair_grid_stats.  The global model is
+ * tile_height = H, tile_width = W (one tile).  scales / offsets: host or device
+ * float32 [count][grid_rows][grid_cols].  A frame named twice is normalised
+ * twice, in order.  Host coefficients, or a NULL stream, make the call
+ * synchronous; device coefficients on a caller stream are applied
+ * asynchronously on that stream. */
+int gh_apply_normalization(gh_stack* stack, const int64_t* indices, int64_t count, const float* scales,
+                           const float* offsets, int32_t tile_height, int32_t tile_width, void* stream);
+
 /* Fused calibration statistics of the last gh_calibrate_frames call with
  * accumulate_stats=1: sum, sum_sq (FP64) and finite count per frame index. */
 int gh_calibration_stats(gh_stack* stack, int64_t index, double* sum, double* sum_sq, int64_t* finite);
