@@ -51,6 +51,8 @@ def parse_args():
     p.add_argument("--no-batch", action="store_true", help="skip the batched multi-job (config 3) leg")
     p.add_argument("--batch-jobs", type=int, default=64)
     p.add_argument("--batch-reps", type=int, default=3)
+    p.add_argument("--no-local-norm", action="store_true", help="skip the local-normalisation leg")
+    p.add_argument("--local-norm-tile", type=int, default=512)
     p.add_argument("--copy-probe-mib", type=int, default=0,
                    help="issue one untimed device-to-device copy of this many MiB before the timed steps "
                         "(a known-size reference for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -353,6 +355,56 @@ _TRAFFIC_KERNELS = {"calibrate": "gh::calibrate_batch_kernel<2, false, 16>", "wa
                     "integrate": "gh::integrate_winsorized_kernel<208, true>"}
 
 
+def local_norm_leg(args, stack, weights):
+    """Resident local normalisation of every active frame to frame 0 on the
+    calibrated + warped stack left by the timed steps: the grid model
+    (pair statistics of all sources in one batched call, one batched
+    application) and the global model (per-frame statistics and application,
+    the reference's frame loop).  Each call is synchronous; times are wall
+    clock around the calls.  Algorithmic bytes: statistics read every source
+    and the reference once (4 B/px each), the application reads and writes
+    each source (8 B/px)."""
+    import numpy as np
+
+    from glass_amd.local_norm import resident_local_normalization
+
+    npix = stack.pixels_per_frame
+    tile = args.local_norm_tile
+    sources = np.asarray([i for i in range(stack.frame_count) if weights[i] > 0 and i != 0], np.int32)
+    n = int(sources.size)
+    # statistics alone, twice (the first call also allocates its buffers)
+    for _ in range(2):
+        t = time.perf_counter()
+        batch = stack.frame_pair_grid_stats_batch(0, sources, tile, tile)
+        stats_s = time.perf_counter() - t
+    # the batched application alone (unit coefficients leave every value as it is)
+    rows, cols = batch["grid_rows"], batch["grid_cols"]
+    ones = np.ones((n, rows, cols), np.float32)
+    zeros = np.zeros((n, rows, cols), np.float32)
+    for _ in range(2):
+        t = time.perf_counter()
+        stack.apply_grid_normalization_frames(sources, ones, zeros, tile, tile)
+        apply_s = time.perf_counter() - t
+    out = {"sources": n, "tile": tile, "grid": [rows, cols]}
+    for mode in ("grid_mean_std", "global_mean_std"):
+        t = time.perf_counter()
+        res = resident_local_normalization(stack, 0, weights, mode, tile)
+        total_s = time.perf_counter() - t
+        statuses = {}
+        for r in res["frames"]:
+            statuses[r["status"]] = statuses.get(r["status"], 0) + 1
+        out[mode] = {"total_ms": round(total_s * 1e3, 2), "stats_ms": round(res["timing"]["stats_s"] * 1e3, 2),
+                     "apply_ms": round(res["timing"]["apply_s"] * 1e3, 2), "statuses": statuses}
+        log(f"local norm {mode}: {total_s * 1e3:.1f} ms ({statuses})")
+    stats_bytes = 4.0 * npix * (n + 1)
+    apply_bytes = 8.0 * npix * n
+    out["grid_stats_ms"] = round(stats_s * 1e3, 2)
+    out["grid_stats_gbps"] = round(stats_bytes / stats_s / 1e9, 1)
+    out["grid_apply_ms"] = round(apply_s * 1e3, 2)
+    out["grid_apply_gbps"] = round(apply_bytes / apply_s / 1e9, 1)
+    return out
+
+
 def measured_traffic(args, stage):
     path = ROOT / "profiles" / "hbm_traffic_200x9600x6422.json"
     default = (args.frames, args.height, args.width, args.interpolation, args.zero_weight) == (200, 6422, 9600,
@@ -522,6 +574,9 @@ def main():
             "avg_launch_ms": round(avg_launch_ms, 4),
         },
     }
+    if not args.no_local_norm:
+        log("local-normalisation leg (grid and global models on the warped stack)")
+        line["local_normalization"] = local_norm_leg(args, stack, weights)
     stack.close()
     del light_raw
     torch.cuda.empty_cache()

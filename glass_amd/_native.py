@@ -155,6 +155,7 @@ _SIGNATURES = {
     "gh_warp_coverage_map": [c_void_p, c_void_p],
     "gh_integrate": [c_void_p, POINTER(c_float), POINTER(IntegrateParams), POINTER(IntegrateOutputs), c_void_p],
     "gh_frame_stats": [c_void_p, c_int64, c_int32, POINTER(FrameStatistics)],
+    "gh_frames_stats": [c_void_p, POINTER(c_int64), c_int64, POINTER(FrameStatistics)],
     "gh_pair_grid_stats": [
         c_void_p, c_int64, POINTER(c_int64), c_int64, c_int32, c_int32, POINTER(c_float), POINTER(c_float),
         POINTER(c_float), POINTER(c_float), POINTER(c_uint64),

@@ -351,22 +351,34 @@ __global__ __launch_bounds__(kBlock) void stats_reduce_kernel(const StatPartial*
 }
 
 // ------------------------------------------------------------ frame moments
-__global__ __launch_bounds__(kBlock) void frame_moments_kernel(const float* frame, long long npix,
-                                                                FrameMoments* partials) {
+// Block partial of a frame's FP64 moments over the grid-stride slice
+// blockIdx.x of `blocks` blocks; the single-frame and the batched kernels share
+// it, so a frame gets the same bits either way.
+__device__ __forceinline__ void frame_moments_block(const float* frame, long long npix, unsigned blocks,
+                                                    FrameMoments* out) {
   double sum = 0.0, sum_sq = 0.0;
-  unsigned long long finite = 0, nonfinite = 0;
+  unsigned fin32 = 0, nonfin32 = 0;  // per thread at most npix / kBlock < 2^32
   float mn = INFINITY, mx = -INFINITY;
-  const long long stride = static_cast<long long>(gridDim.x) * kBlock;
-  for (long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; i < npix; i += stride) {
-    const float v = frame[i];
+  const long long stride = static_cast<long long>(blocks) * kBlock;
+  auto take = [&](float v) {
     if (isfinite(v)) {
       const double x = static_cast<double>(v);
-      sum += x; sum_sq += x * x; ++finite;
+      sum += x; sum_sq += x * x; ++fin32;
       mn = fminf(mn, v); mx = fmaxf(mx, v);
     } else {
-      ++nonfinite;
+      ++nonfin32;
     }
+  };
+  // Four loads in flight per thread; the values are taken in index order, as
+  // the plain loop would.
+  long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  for (; i + 3 * stride < npix; i += 4 * stride) {
+    const float v0 = ld_global(frame, i), v1 = ld_global(frame, i + stride);
+    const float v2 = ld_global(frame, i + 2 * stride), v3 = ld_global(frame, i + 3 * stride);
+    take(v0); take(v1); take(v2); take(v3);
   }
+  for (; i < npix; i += stride) take(ld_global(frame, i));
+  unsigned long long finite = fin32, nonfinite = nonfin32;
   __shared__ FrameMoments s[kBlock / 64];
   sum = wave_sum(sum); sum_sq = wave_sum(sum_sq); finite = wave_sum(finite); nonfinite = wave_sum(nonfinite);
   mn = wave_min(mn); mx = wave_max(mx);
@@ -379,12 +391,12 @@ __global__ __launch_bounds__(kBlock) void frame_moments_kernel(const float* fram
       r.sum += s[w].sum; r.sum_sq += s[w].sum_sq; r.finite += s[w].finite; r.nonfinite += s[w].nonfinite;
       r.min = fminf(r.min, s[w].min); r.max = fmaxf(r.max, s[w].max);
     }
-    partials[blockIdx.x] = r;
+    *out = r;
   }
 }
 
-__global__ __launch_bounds__(64) void frame_moments_final(const FrameMoments* partials, unsigned blocks,
-                                                          FrameMoments* out) {
+__device__ __forceinline__ void frame_moments_reduce(const FrameMoments* partials, unsigned blocks,
+                                                     FrameMoments* out) {
   double sum = 0.0, sum_sq = 0.0;
   unsigned long long finite = 0, nonfinite = 0;
   float mn = INFINITY, mx = -INFINITY;
@@ -396,6 +408,31 @@ __global__ __launch_bounds__(64) void frame_moments_final(const FrameMoments* pa
   sum = wave_sum(sum); sum_sq = wave_sum(sum_sq); finite = wave_sum(finite); nonfinite = wave_sum(nonfinite);
   mn = wave_min(mn); mx = wave_max(mx);
   if (threadIdx.x == 0) *out = FrameMoments{sum, sum_sq, finite, nonfinite, mn, mx};
+}
+
+__global__ __launch_bounds__(kBlock) void frame_moments_kernel(const float* frame, long long npix,
+                                                                FrameMoments* partials) {
+  frame_moments_block(frame, npix, gridDim.x, partials + blockIdx.x);
+}
+
+__global__ __launch_bounds__(64) void frame_moments_final(const FrameMoments* partials, unsigned blocks,
+                                                          FrameMoments* out) {
+  frame_moments_reduce(partials, blocks, out);
+}
+
+// Batched form: frame blockIdx.y of the table, partials [frame][blocks].
+__global__ __launch_bounds__(kBlock) void frame_moments_batch_kernel(LnBatch b, long long npix,
+                                                                      FrameMoments* partials) {
+  int zero = 0;
+  asm volatile("" : "+s"(zero));
+  const float* frame = b.frame[blockIdx.y + zero];
+  frame_moments_block(frame, npix, gridDim.x,
+                      partials + static_cast<long long>(blockIdx.y) * gridDim.x + blockIdx.x);
+}
+
+__global__ __launch_bounds__(64) void frame_moments_batch_final(const FrameMoments* partials, unsigned blocks,
+                                                                FrameMoments* out) {
+  frame_moments_reduce(partials + static_cast<long long>(blockIdx.x) * blocks, blocks, out + blockIdx.x);
 }
 
 // ------------------------------------------------------------ radix select
@@ -573,6 +610,15 @@ void launch_clear_flags(unsigned* /*flags*/, const CalibBatch& batch, int count,
 }
 
 unsigned frame_moments_blocks(long long npix) { return stream_blocks(npix); }
+
+void launch_frame_moments_batch(const LnBatch& b, int n, long long npix, FrameMoments* scratch_partials,
+                                unsigned blocks, FrameMoments* out, hipStream_t stream) {
+  if (n <= 0) return;
+  frame_moments_batch_kernel<<<dim3(blocks, static_cast<unsigned>(n)), kBlock, 0, stream>>>(b, npix,
+                                                                                             scratch_partials);
+  frame_moments_batch_final<<<static_cast<unsigned>(n), 64, 0, stream>>>(scratch_partials, blocks, out);
+  check(hipGetLastError(), "frame moments batch launch");
+}
 
 void launch_frame_moments(const float* frame, long long npix, FrameMoments* scratch_partials, unsigned blocks,
                           FrameMoments* out, hipStream_t stream) {

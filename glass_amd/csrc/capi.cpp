@@ -487,19 +487,7 @@ int gh_frame_stats(gh_stack* s, int64_t index, int32_t exact_median, gh_frame_st
     check(hipMemcpyAsync(&m, s->moments.ptr, sizeof(m), hipMemcpyDeviceToHost, s->stream), "hipMemcpy(moments)");
     check(hipStreamSynchronize(s->stream), "frame stats");
     const int64_t n = static_cast<int64_t>(m.finite);
-    out->sum = m.sum;
-    out->sum_sq = m.sum_sq;
-    out->valid_pixels = n;
-    out->total_pixels = s->npix;
-    out->nonfinite_pixels = s->npix - n;
-    out->mean = n ? m.sum / static_cast<double>(n) : 0.0;
-    double var = n ? m.sum_sq / static_cast<double>(n) - out->mean * out->mean : 0.0;
-    if (var < 0.0) var = 0.0;
-    out->std = std::sqrt(var);
-    out->min = n ? m.min : 0.0f;
-    out->max = n ? m.max : 0.0f;
-    out->median = 0.0;
-    out->mad = 0.0;
+    ghi::fill_frame_statistics(m, s->npix, out);
     if (!exact_median || n == 0) return;
     s->radix.ensure(256 + 4, "hipMalloc(radix select)");
     auto select = [&](int abs_mode, float center, uint64_t rank) -> float {

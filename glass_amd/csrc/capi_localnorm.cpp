@@ -98,6 +98,32 @@ int gh_pair_grid_stats(gh_stack* s, int64_t reference_index, const int64_t* sour
   });
 }
 
+int gh_frames_stats(gh_stack* s, const int64_t* indices, int64_t count, gh_frame_statistics* out) {
+  return guarded([&] {
+    if (count < 0) throw gh::ValueError("frame count must not be negative");
+    if (count == 0) return;
+    if (!indices || !out) throw gh::ValueError("indices and output pointers must not be NULL");
+    for (int64_t k = 0; k < count; ++k) s->require_loaded(indices[k], "global frame statistics");
+    const unsigned blocks = gh::frame_moments_blocks(s->npix);
+    const int64_t chunk = std::min<int64_t>(count, gh::kLnBatch);
+    s->moments.ensure(static_cast<size_t>(chunk) * (blocks + 1), "hipMalloc(frame moments)");
+    gh::FrameMoments* partials = s->moments.ptr + chunk;
+    std::vector<gh::FrameMoments> m(static_cast<size_t>(chunk));
+    s->wait_caller();
+    for (int64_t start = 0; start < count; start += chunk) {
+      const int n = static_cast<int>(std::min<int64_t>(chunk, count - start));
+      gh::LnBatch b{};
+      for (int j = 0; j < n; ++j) b.frame[j] = s->slot[indices[start + j]];
+      gh::launch_frame_moments_batch(b, n, s->npix, partials, blocks, s->moments.ptr, s->stream);
+      check(hipMemcpyAsync(m.data(), s->moments.ptr, static_cast<size_t>(n) * sizeof(gh::FrameMoments),
+                           hipMemcpyDeviceToHost, s->stream),
+            "hipMemcpy(frame moments)");
+      check(hipStreamSynchronize(s->stream), "frames statistics");
+      for (int j = 0; j < n; ++j) ghi::fill_frame_statistics(m[j], s->npix, out + start + j);
+    }
+  });
+}
+
 int gh_apply_normalization(gh_stack* s, const int64_t* indices, int64_t count, const float* scales,
                            const float* offsets, int32_t tile_height, int32_t tile_width, void* stream_ptr) {
   return guarded([&] {

@@ -201,6 +201,10 @@ struct LnBatch {
   const float* frame[kLnBatch];
   unsigned* flag[kLnBatch];
 };
+// FP64 moments of each frame of the batch (bit-identical to
+// launch_frame_moments of the same frame): partials [n][blocks], out [n].
+void launch_frame_moments_batch(const LnBatch& b, int n, long long npix, FrameMoments* scratch_partials,
+                                unsigned blocks, FrameMoments* out, hipStream_t stream);
 // Per-tile FP64 sums (s, s^2, r, r^2) and counts of (source, reference) pairs
 // over pixels finite in both; batch entry j writes record (first + j).
 void launch_pair_grid_sums(const LnBatch& b, int n, const float* reference, int width, int height, int tile_w,

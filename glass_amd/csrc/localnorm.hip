@@ -133,15 +133,27 @@ __global__ __launch_bounds__(kLnThreads) void pair_grid_sums_kernel(LnBatch b, i
   }
 }
 
+// Column tile of pixel x: x / tile_w through an FP64 reciprocal (exact to
+// within one for any 31-bit x, then corrected), clamped to the last column.
+__device__ __forceinline__ int column_tile(int x, int tile_w, double inv_tile_w, int last) {
+  int c = static_cast<int>(static_cast<double>(x) * inv_tile_w);
+  c -= (c * tile_w > x) ? 1 : 0;
+  c += ((c + 1) * tile_w <= x) ? 1 : 0;
+  return min(c, last);
+}
+
 // x * scale[tile] + offset[tile] in place for the frames of the batch
 // (blockIdx.y); coefficient grids are [first + frame][grid_rows][grid_cols].
 // Blocks stride over rows.  VEC: width and tile width are multiples of 4, so
-// a float4 never straddles a tile and every row starts 16-byte aligned.
+// a float4 never straddles a tile and every row starts 16-byte aligned; each
+// thread keeps kApplyUnroll float4 loads in flight.
+constexpr int kApplyUnroll = 4;
+
 template <bool VEC>
 __global__ __launch_bounds__(kLnThreads) void apply_grid_kernel(LnBatch b, const float* scales,
                                                                 const float* offsets, int width, int height,
-                                                                int tile_w, int tile_h, int grid_cols,
-                                                                int grid_rows, int first) {
+                                                                int tile_w, int tile_h, double inv_tile_w,
+                                                                int grid_cols, int grid_rows, int first) {
   int zero = 0;
   asm volatile("" : "+s"(zero));
   const int frame = blockIdx.y;
@@ -150,6 +162,7 @@ __global__ __launch_bounds__(kLnThreads) void apply_grid_kernel(LnBatch b, const
   const long long grid = static_cast<long long>(grid_rows) * grid_cols;
   const float* sc = scales + (first + frame) * grid;
   const float* of = offsets + (first + frame) * grid;
+  const int last = grid_cols - 1;
   bool bad = false;
   for (int y = blockIdx.x; y < height; y += gridDim.x) {
     const int trow = min(y / tile_h, grid_rows - 1) * grid_cols;
@@ -157,20 +170,32 @@ __global__ __launch_bounds__(kLnThreads) void apply_grid_kernel(LnBatch b, const
     if (VEC) {
       global_f32x4* p = (global_f32x4*)(dst + row);
       const int quads = width >> 2;
-      for (int q = threadIdx.x; q < quads; q += kLnThreads) {
-        const int c = trow + min((q << 2) / tile_w, grid_cols - 1);
-        const float s = ld_global(sc, c), o = ld_global(of, c);
-        f32x4 v = p[q];
-        v.x = v.x * s + o;
-        v.y = v.y * s + o;
-        v.z = v.z * s + o;
-        v.w = v.w * s + o;
-        p[q] = v;
-        bad |= !(isfinite(v.x) && isfinite(v.y) && isfinite(v.z) && isfinite(v.w));
+      for (int q0 = threadIdx.x; q0 < quads; q0 += kApplyUnroll * kLnThreads) {
+        f32x4 v[kApplyUnroll];
+#pragma unroll
+        for (int u = 0; u < kApplyUnroll; ++u) {
+          const int q = q0 + u * kLnThreads;
+          if (q < quads) v[u] = p[q];
+        }
+#pragma unroll
+        for (int u = 0; u < kApplyUnroll; ++u) {
+          const int q = q0 + u * kLnThreads;
+          if (q < quads) {
+            const int c = trow + column_tile(q << 2, tile_w, inv_tile_w, last);
+            const float s = ld_global(sc, c), o = ld_global(of, c);
+            f32x4 r = v[u];
+            r.x = r.x * s + o;
+            r.y = r.y * s + o;
+            r.z = r.z * s + o;
+            r.w = r.w * s + o;
+            p[q] = r;
+            bad |= !(isfinite(r.x) && isfinite(r.y) && isfinite(r.z) && isfinite(r.w));
+          }
+        }
       }
     } else {
       for (int x = threadIdx.x; x < width; x += kLnThreads) {
-        const int c = trow + min(x / tile_w, grid_cols - 1);
+        const int c = trow + column_tile(x, tile_w, inv_tile_w, last);
         const unsigned i = row + static_cast<unsigned>(x);
         const float v = ld_global(dst, i) * ld_global(sc, c) + ld_global(of, c);
         st_global_u32(dst, i, v);
@@ -199,12 +224,13 @@ void launch_apply_grid(const LnBatch& b, int n, const float* scales, const float
                        int tile_w, int tile_h, int grid_cols, int grid_rows, int first, hipStream_t stream) {
   if (n <= 0) return;
   const dim3 grid(static_cast<unsigned>(std::min(height, 1024)), static_cast<unsigned>(n));
+  const double inv_tile_w = 1.0 / static_cast<double>(tile_w);
   if (width % 4 == 0 && tile_w % 4 == 0)
     apply_grid_kernel<true><<<grid, kLnThreads, 0, stream>>>(b, scales, offsets, width, height, tile_w, tile_h,
-                                                              grid_cols, grid_rows, first);
+                                                              inv_tile_w, grid_cols, grid_rows, first);
   else
     apply_grid_kernel<false><<<grid, kLnThreads, 0, stream>>>(b, scales, offsets, width, height, tile_w, tile_h,
-                                                               grid_cols, grid_rows, first);
+                                                               inv_tile_w, grid_cols, grid_rows, first);
   check(hipGetLastError(), "grid normalisation launch");
 }
 

@@ -76,6 +76,25 @@ struct DeviceBuffer {
   }
 };
 
+// frame_global_stats from FP64 moments (native_bindings.cpp:10626-10640):
+// mean = sum / n, std = sqrt(max(sum_sq / n - mean^2, 0)); zeros for n = 0.
+inline void fill_frame_statistics(const gh::FrameMoments& m, int64_t npix, gh_frame_statistics* out) {
+  const int64_t n = static_cast<int64_t>(m.finite);
+  out->sum = m.sum;
+  out->sum_sq = m.sum_sq;
+  out->valid_pixels = n;
+  out->total_pixels = npix;
+  out->nonfinite_pixels = npix - n;
+  out->mean = n ? m.sum / static_cast<double>(n) : 0.0;
+  double var = n ? m.sum_sq / static_cast<double>(n) - out->mean * out->mean : 0.0;
+  if (var < 0.0) var = 0.0;
+  out->std = std::sqrt(var);
+  out->min = n ? m.min : 0.0f;
+  out->max = n ? m.max : 0.0f;
+  out->median = 0.0;
+  out->mad = 0.0;
+}
+
 }  // namespace ghi
 
 using ghi::DeviceBuffer;

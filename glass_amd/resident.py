@@ -417,6 +417,18 @@ class ResidentCalibratedStack:
             out["mad"] = st.mad
         return out
 
+    def frames_global_stats(self, indices) -> list[dict]:
+        """frame_global_stats of many frames in one pass (same values as the
+        per-frame call without exact_median)."""
+        idx = _index_array(indices)
+        arr = (N.FrameStatistics * max(int(idx.size), 1))()
+        N.call("gh_frames_stats", self._h, _i64p(idx), int(idx.size), arr)
+        return [{
+            "mean": st.mean, "std": st.std, "valid_pixels": int(st.valid_pixels),
+            "total_pixels": int(st.total_pixels), "nonfinite_pixels": int(st.nonfinite_pixels),
+            "model": "resident_global_mean_std", "min": float(st.min), "max": float(st.max),
+        } for st in arr[:idx.size]]
+
     # ------------------------------------------------- local normalisation
     def _grid_shape(self, tile_height: int, tile_width: int) -> tuple[int, int]:
         if int(tile_height) <= 0 or int(tile_width) <= 0:
@@ -517,6 +529,22 @@ class ResidentCalibratedStack:
             "device_to_device_copy_s": 0.0, "kernel_enqueue_s": call_s, "sync_s": 0.0,
             "total_s": time.perf_counter() - t0,
         }
+
+    def apply_global_normalization_frames(self, indices, scales, offsets, stream: int | None = None) -> dict:
+        """apply_global_normalization_frame for many frames in one launch per
+        batch of distinct frames: frame indices[k] becomes x * float32(scales[k])
+        + float32(offsets[k])."""
+        idx = _index_array(indices)
+        sc = np.asarray(scales, dtype=np.float32).reshape(-1)
+        of = np.asarray(offsets, dtype=np.float32).reshape(-1)
+        if sc.size != idx.size or of.size != idx.size:
+            raise ValueError("scales and offsets must have one entry per frame")
+        t0 = time.perf_counter()
+        call_s = self._apply_normalization(idx, sc, of, self.height, self.width, stream) if idx.size else 0.0
+        return {"schema_version": 1, "mode": "in_place_device_update_batch", "model": "resident_global_mean_std",
+                "frame_count": int(idx.size), "frame_bytes": self.pixels_per_frame * 4,
+                "temporary_output_bytes": 0, "kernel_enqueue_s": call_s, "sync_s": 0.0,
+                "total_s": time.perf_counter() - t0}
 
     @staticmethod
     def _coefficients(array, ndim: int, what: str):

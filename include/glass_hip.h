@@ -187,6 +187,13 @@ int gh_integrate(gh_stack* stack, const float* weights, const gh_integrate_param
  * MAD (src/glass/engine/quality.py:126-200).  FP64. */
 int gh_frame_stats(gh_stack* stack, int64_t index, int32_t exact_median, gh_frame_statistics* out);
 
+/* frame_global_stats (native_bindings.cpp:10569-10643) of many frames in one
+ * pass: out[k] for frame indices[k], bit-identical to gh_frame_stats with
+ * exact_median = 0 (median and mad are 0).  The reference asks for these one
+ * frame at a time in its local-normalisation and weighting loops
+ * (resident_cuda.py:12928-12996); this is their batched form.  Synchronous. */
+int gh_frames_stats(gh_stack* stack, const int64_t* indices, int64_t count, gh_frame_statistics* out);
+
 /* frame_pair_grid_stats(reference, source, tile_height, tile_width) and
  * frame_pair_grid_stats_batch(reference, sources, tile_height, tile_width)
  * (native_bindings.cpp:10645-10784, :10786-11021; kernel

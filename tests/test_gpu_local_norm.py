@@ -289,3 +289,31 @@ def test_pipeline_with_local_normalization(gpu):
     # integrating normalised frames lands on the reference's level
     assert np.nanmean(res.master) == pytest.approx(float(np.mean(frames[0])), rel=1e-3)
     assert res.weights.tolist() == exp_w.tolist()
+
+
+def test_frames_global_stats_batch_equals_per_frame(gpu):
+    frames = _frames(70, 20, 24, seed=8)
+    frames[4] = np.nan
+    with _stack_of(frames) as s:
+        idx = list(range(69, -1, -1))
+        batch = s.frames_global_stats(idx)
+        for k, i in enumerate(idx):
+            single = s.frame_global_stats(i)
+            for key in ("mean", "std", "valid_pixels", "nonfinite_pixels", "min", "max"):
+                assert batch[k][key] == single[key], (i, key)
+            exp = frame_global_stats(frames[i])
+            assert batch[k]["valid_pixels"] == exp["valid_pixels"]
+            assert batch[k]["mean"] == pytest.approx(exp["mean"], rel=1e-12, abs=1e-12)
+            assert batch[k]["std"] == pytest.approx(exp["std"], rel=1e-9, abs=1e-12)
+
+
+def test_apply_global_batch_equals_per_frame(gpu):
+    frames = _frames(5, 33, 35, seed=12)
+    scales = [0.5, 1.75, 3.0]
+    offsets = [10.0, -7.25, 0.125]
+    with _stack_of(frames) as s:
+        s.apply_global_normalization_frames([3, 0, 4], scales, offsets)
+        got = s.download_frames_tile(list(range(5)), 0, 0, 35, 33)
+    for k, i in enumerate((3, 0, 4)):
+        assert np.array_equal(got[i], LN.apply_global(frames[i], scales[k], offsets[k]), equal_nan=True)
+    assert np.array_equal(got[1], frames[1], equal_nan=True)
