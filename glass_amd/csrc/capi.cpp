@@ -179,8 +179,11 @@ int gh_calibrate_frames(gh_stack* s, const int64_t* indices, int64_t count, cons
       s->partials.ensure(static_cast<size_t>(gh::kCalibBatch) * blocks, "hipMalloc(calibration partials)");
       s->calib_stats.ensure(static_cast<size_t>(s->frames), "hipMalloc(calibration stats)");
     }
+    // Host raw: one frame per launch from the upload ring (the copy of the
+    // next frame runs on the copy stream meanwhile).  Device raw: batches.
     const int64_t batch_cap = device_raw ? gh::kCalibBatch : 1;
-    if (!device_raw) s->raw_staging.ensure(static_cast<size_t>(frame_raw_bytes + 16), "hipMalloc(raw staging)");
+    int slot = -1;
+    if (st == s->stream) s->wait_caller();
     for (int64_t start = 0; start < count; start += batch_cap) {
       const int n = static_cast<int>(std::min<int64_t>(batch_cap, count - start));
       gh::CalibBatch b{};
@@ -189,9 +192,13 @@ int gh_calibrate_frames(gh_stack* s, const int64_t* indices, int64_t count, cons
         const int64_t f = indices[k];
         const unsigned char* src = static_cast<const unsigned char*>(raw) + k * raw_frame_stride_bytes;
         if (!device_raw) {
-          check(hipMemcpyAsync(s->raw_staging.ptr, src, frame_raw_bytes, hipMemcpyHostToDevice, st),
+          unsigned char* dev = s->ring_slot(static_cast<size_t>(frame_raw_bytes), slot);
+          check(hipStreamWaitEvent(s->copy_stream, s->ring_free[slot], 0), "hipStreamWaitEvent(raw ring)");
+          check(hipMemcpyAsync(dev, src, frame_raw_bytes, hipMemcpyHostToDevice, s->copy_stream),
                 "hipMemcpy(raw light)");
-          src = s->raw_staging.ptr;
+          check(hipEventRecord(s->ring_copied[slot], s->copy_stream), "hipEventRecord(raw ring)");
+          check(hipStreamWaitEvent(st, s->ring_copied[slot], 0), "hipStreamWaitEvent(raw ring)");
+          src = dev;
         }
         b.src[j] = src;
         b.dst[j] = s->slot[f];
@@ -207,8 +214,10 @@ int gh_calibrate_frames(gh_stack* s, const int64_t* indices, int64_t count, cons
         s->calib_stats_valid[f] = pol.accumulate_stats ? 1 : 0;
         s->mark_loaded(f);
       }
-      if (!device_raw) check(hipStreamSynchronize(st), "calibrate (host raw)");
+      if (!device_raw) check(hipEventRecord(s->ring_free[slot], st), "hipEventRecord(raw ring)");
     }
+    s->mark_caller(st);
+    if (!stream_ptr) check(hipStreamSynchronize(st), "calibrate");
   });
 }
 

@@ -119,7 +119,7 @@ struct gh_stack {
   DeviceBuffer<gh::StatPartial> partials;
   DeviceBuffer<gh::StatPartial> calib_stats;  // [frames]
   std::vector<char> calib_stats_valid;
-  DeviceBuffer<unsigned char> raw_staging;
+  DeviceBuffer<unsigned char> raw_staging;  // kRawRing host-raw upload slots
   DeviceBuffer<const float*> frame_table;
   DeviceBuffer<float> frame_weights;
   DeviceBuffer<unsigned char> tile_flags;
@@ -180,6 +180,40 @@ struct gh_stack {
     if (caller_done) check(hipStreamWaitEvent(stream, caller_done, 0), "hipStreamWaitEvent(caller stream)");
   }
 
+  // Host raw frames go through a ring of device slots filled on a copy
+  // stream: the upload of frame k + 1 overlaps the calibration of frame k.
+  // Slot i is free again once the calibration that read it has run
+  // (ring_free[i], recorded on the calibrating stream).
+  static constexpr int kRawRing = 3;
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t ring_copied[kRawRing] = {};
+  hipEvent_t ring_free[kRawRing] = {};
+  int ring_next = 0;
+  size_t ring_slot_bytes = 0;
+
+  unsigned char* ring_slot(size_t frame_raw_bytes, int& slot) {
+    const size_t need = (frame_raw_bytes + 255) / 256 * 256;
+    if (!copy_stream) {
+      check(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking), "hipStreamCreate(raw upload)");
+      for (int i = 0; i < kRawRing; ++i) {
+        check(hipEventCreateWithFlags(&ring_copied[i], hipEventDisableTiming), "hipEventCreate(raw ring)");
+        check(hipEventCreateWithFlags(&ring_free[i], hipEventDisableTiming), "hipEventCreate(raw ring)");
+        check(hipEventRecord(ring_free[i], stream), "hipEventRecord(raw ring)");
+      }
+    }
+    if (need > ring_slot_bytes) {
+      // every slot must be idle before the ring is reallocated
+      for (int i = 0; i < kRawRing; ++i) check(hipEventSynchronize(ring_free[i]), "raw ring drain");
+      check(hipStreamSynchronize(copy_stream), "raw ring drain");
+      raw_staging.release();
+      raw_staging.ensure(need * kRawRing, "hipMalloc(raw upload ring)");
+      ring_slot_bytes = need;
+    }
+    slot = ring_next;
+    ring_next = (ring_next + 1) % kRawRing;
+    return raw_staging.ptr + static_cast<size_t>(slot) * ring_slot_bytes;
+  }
+
   void ensure_geometric() {
     if (geometric) return;
     check(hipMalloc(reinterpret_cast<void**>(&geometric), frame_bytes()), "hipMalloc(warp coverage)");
@@ -236,6 +270,12 @@ struct gh_stack {
     ln_sums.release();
     ln_counts.release();
     ln_coef.release();
+    if (copy_stream) (void)hipStreamSynchronize(copy_stream);
+    for (int i = 0; i < kRawRing; ++i) {
+      if (ring_copied[i]) (void)hipEventDestroy(ring_copied[i]);
+      if (ring_free[i]) (void)hipEventDestroy(ring_free[i]);
+    }
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
     if (caller_done) (void)hipEventDestroy(caller_done);
     if (stream) (void)hipStreamDestroy(stream);
   }

@@ -273,18 +273,108 @@ class ResidentCalibratedStack:
         policy=None,
     ) -> dict:
         """Raw FITS BITPIX=16/BZERO=32768 big-endian light frames
-        (native_bindings.cpp:6839; kernel calibration_kernels.cu:110-153)."""
+        (native_bindings.cpp:6839; kernel calibration_kernels.cu:110-153).
+        Frames are queued asynchronously (uploads overlap calibration);
+        release_callback(k) runs once frame k's host buffer has been consumed,
+        in order."""
+        import torch
+
         idx = _index_array(indices)
         start = time.perf_counter()
+        stream = torch.cuda.Stream()
+        pending = []
         for k, index in enumerate(idx):
             ptr, ref = _buffer(raw_lights[k], np.uint8, (self.pixels_per_frame * 2,), "raw_light")
             darks = dark_exposures[k] if isinstance(dark_exposures, (list, tuple, np.ndarray)) else dark_exposures
             self._calibrate([int(index)], ptr, self.pixels_per_frame * 2, N.GH_FMT_FITS_U16BE_BZERO,
-                            [light_exposures[k]], [darks], policy, False)
+                            [light_exposures[k]], [darks], policy, False, stream.cuda_stream)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+            pending.append((k, ev, ref))
+            while pending and pending[0][1].query():
+                done = pending.pop(0)
+                if release_callback is not None:
+                    release_callback(int(done[0]))
+        for k, ev, _ in pending:
+            ev.synchronize()
             if release_callback is not None:
                 release_callback(int(k))
+        stream.synchronize()
         return {"schema_version": 1, "frame_count": int(idx.size), "stream_count": int(stream_count),
                 "wave_frames": int(wave_frames), "total_s": time.perf_counter() - start}
+
+    def calibrate_frames_host(self, indices, raw, raw_format: str, light_exposures, dark_exposures, policy=None,
+                              frame_stride_bytes: int | None = None, accumulate_stats: bool = False,
+                              stream: int | None = None) -> None:
+        """Calibrate frames from one host buffer holding them back to back
+        (numpy array or CPU tensor; pinned memory gives the full link rate).
+        Uploads run through the device ring, overlapping calibration.  With a
+        stream the call is asynchronous and `raw` must outlive that stream's
+        work."""
+        fmt = {"float32": N.GH_FMT_F32, "uint16": N.GH_FMT_U16, "fits_u16be_bzero": N.GH_FMT_FITS_U16BE_BZERO}[raw_format]
+        sample = 4 if fmt == N.GH_FMT_F32 else 2
+        stride = int(frame_stride_bytes or self.pixels_per_frame * sample)
+        count = len(_index_array(indices))
+        if _is_torch_cuda(raw):
+            raise ValueError("calibrate_frames_host expects host memory; use calibrate_frames_device")
+        if type(raw).__module__.startswith("torch"):
+            if not raw.is_contiguous():
+                raise ValueError("raw host tensor must be contiguous")
+            nbytes, ptr = raw.numel() * raw.element_size(), ctypes.c_void_p(raw.data_ptr())
+        else:
+            arr = np.ascontiguousarray(raw)
+            if stream and arr is not raw:
+                raise ValueError("an asynchronous host calibration needs a contiguous raw buffer (no copy)")
+            nbytes, ptr, raw = arr.nbytes, ctypes.c_void_p(arr.ctypes.data), arr
+        if count and nbytes < (count - 1) * stride + self.pixels_per_frame * sample:
+            raise ValueError("raw host buffer is smaller than the frames it should hold")
+        self._calibrate(indices, ptr, stride, fmt, light_exposures, dark_exposures, policy, accumulate_stats, stream)
+
+    def calibrate_fits_files(self, indices, paths, light_exposures, dark_exposures, policy=None,
+                             ring_frames: int = 4) -> dict:
+        """Read BITPIX=16/BZERO=32768 FITS lights straight into a ring of
+        pinned host buffers and calibrate them as they arrive: the read of
+        file k + 1 overlaps the upload and calibration of file k (the
+        reference's pinned ring, resident_cuda.py:1159-1897)."""
+        import torch
+
+        from . import fits as F
+
+        idx = _index_array(indices)
+        paths = list(paths)
+        if len(paths) != idx.size:
+            raise ValueError("paths must match indices")
+        lights = list(light_exposures)
+        darks = (list(dark_exposures) if isinstance(dark_exposures, (list, tuple, np.ndarray))
+                 else [dark_exposures] * idx.size)
+        specs = [F.simple_fits_image_spec(p) for p in paths]
+        for spec in specs:
+            elig = F.u16_gpu_eligibility(spec, (self.height, self.width))
+            if not elig["eligible"]:
+                raise ValueError(f"{spec.path}: not a raw u16 light for this stack ({elig['reason']})")
+        nbytes = self.pixels_per_frame * 2
+        slots = max(1, min(int(ring_frames), int(idx.size)))
+        ring = [torch.empty(nbytes, dtype=torch.uint8, pin_memory=True) for _ in range(slots)]
+        done = [None] * slots
+        stream = torch.cuda.Stream()
+        t0 = time.perf_counter()
+        read_s = 0.0
+        for k in range(idx.size):
+            slot = k % slots
+            if done[slot] is not None:
+                done[slot].synchronize()
+            t = time.perf_counter()
+            F.read_u16be_raw_into(specs[k], ring[slot].numpy())
+            read_s += time.perf_counter() - t
+            self._calibrate([int(idx[k])], ctypes.c_void_p(ring[slot].data_ptr()), nbytes,
+                            N.GH_FMT_FITS_U16BE_BZERO, [lights[k]], [darks[k]], policy, False, stream.cuda_stream)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+            done[slot] = ev
+        stream.synchronize()
+        total = time.perf_counter() - t0
+        return {"schema_version": 1, "frame_count": int(idx.size), "ring_frames": slots,
+                "bytes_read": int(nbytes * idx.size), "file_read_s": read_s, "total_s": total}
 
     def calibrate_frames_device(self, indices, raw, raw_format: str, light_exposures, dark_exposures, policy=None,
                                 frame_stride_bytes: int | None = None, accumulate_stats: bool = False,

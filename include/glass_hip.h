@@ -149,7 +149,14 @@ int gh_download_tile(gh_stack* stack, const int64_t* indices, int64_t count, int
  * (native_bindings.cpp:5979-8778; kernels cpp/cuda/calibration_kernels.cu:5-186):
  * frame k of the batch is read from `raw` + k*raw_frame_stride_bytes (host or
  * device, format GH_FMT_*) and written, calibrated, into slot indices[k].
- * dark_scales[k] multiplies the dark (1.0 when unscaled). */
+ * dark_scales[k] multiplies the dark (1.0 when unscaled).  Host raw frames are
+ * uploaded through a ring of device slots on a copy stream, so the upload of
+ * frame k + 1 overlaps the calibration of frame k (pinned host memory,
+ * e.g. hipHostMalloc, gives the full link rate; the reference's pinned ring
+ * and lane streams, resident_cuda.py:1159-1897, native_bindings.cpp:15185-15242).
+ * A NULL stream makes the call synchronous; on a caller stream the call is
+ * asynchronous and a host `raw` buffer must stay valid until that stream's
+ * work has completed. */
 int gh_calibrate_frames(gh_stack* stack, const int64_t* indices, int64_t count, const void* raw,
                         int64_t raw_frame_stride_bytes, int32_t format, const float* dark_scales,
                         const gh_calib_policy* policy, void* stream);

@@ -15,6 +15,7 @@ Run in a container that has the reference mounted:
 from __future__ import annotations
 
 import json
+import shutil
 import sys
 from pathlib import Path
 
@@ -58,6 +59,11 @@ def main(reference_root: str) -> None:
     out["policy_json"] = np.frombuffer(json.dumps(policy, sort_keys=True).encode(), dtype=np.uint8)
     out["tile_size"] = np.asarray(8, dtype=np.int64)
     np.savez_compressed(HERE / "s2_gate_314.npz", **out)
+    # Two of the reference's FITS files, byte for byte, for the FITS reader
+    # and writer tests (a raw light and a calibrated light).
+    shutil.copyfile(data_dir / "light" / "light_000.fits", HERE / "s2_gate_314_light_000.fits")
+    shutil.copyfile(run_dir / "calib_cache" / "calibrated" / "calibrated_F000010.fits",
+                    HERE / "s2_gate_314_calibrated_F000010.fits")
     print("wrote", HERE / "s2_gate_314.npz", sorted(out))
 
 
