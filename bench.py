@@ -52,6 +52,7 @@ def parse_args():
     p.add_argument("--batch-jobs", type=int, default=64)
     p.add_argument("--batch-reps", type=int, default=3)
     p.add_argument("--no-local-norm", action="store_true", help="skip the local-normalisation leg")
+    p.add_argument("--no-host-raw", action="store_true", help="skip the host-raw (PCIe-inclusive) leg")
     p.add_argument("--local-norm-tile", type=int, default=512)
     p.add_argument("--copy-probe-mib", type=int, default=0,
                    help="issue one untimed device-to-device copy of this many MiB before the timed steps "
@@ -355,6 +356,59 @@ _TRAFFIC_KERNELS = {"calibrate": "gh::calibrate_batch_kernel<2, false, 16>", "wa
                     "integrate": "gh::integrate_winsorized_kernel<208, true>"}
 
 
+def host_raw_leg(args, stack, light_raw, a):
+    """The plain step with the raw lights handed over in pinned host memory
+    instead of HBM: every frame crosses PCIe through the library's upload
+    ring while earlier frames calibrate and warp (8-frame chunks on one
+    stream), then the integration runs as in the timed steps.  This is the PCIe-inclusive rate (never `value`).  Two runs,
+    the second reported; wall clock around each, HIP events for the stages."""
+    import torch
+
+    f = stack.frame_count
+    raw_bytes = light_raw.numel() * light_raw.element_size()
+    t = time.perf_counter()
+    host = torch.empty(raw_bytes, dtype=torch.uint8, pin_memory=True)
+    host.copy_(light_raw.reshape(-1).view(torch.uint8))
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t
+    sh = a["stream"].cuda_stream
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    out = {}
+    for run in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev[0].record(a["stream"])
+        stack.reset_warp_coverage()
+        for _ in range(f - len(a["moving"])):
+            stack.accumulate_full_warp_coverage_frame()
+        # chunks of 8 frames: the warp of a chunk runs while the next
+        # chunk's frames are still crossing PCIe
+        fb = raw_bytes // f
+        moving = set(a["moving"])
+        for c0 in range(0, f, 8):
+            chunk = list(range(c0, min(f, c0 + 8)))
+            stack.calibrate_frames_host(chunk, host[c0 * fb:(chunk[-1] + 1) * fb], "fits_u16be_bzero",
+                                        [a["light_exp"]] * len(chunk), a["dark_exp"], a["policy"], stream=sh)
+            warp = [i for i in chunk if i in moving]
+            if warp:
+                stack.warp_frames(warp, a["mats"][warp], args.interpolation, "cpu", stream=sh)
+        ev[1].record(a["stream"])
+        stack.integrate_to_device(a["outs"], a["weights"], "winsorized_sigma", 3.0, 3.0, 3, 0.5, "uint16",
+                                  dq_geometric=True, dq_active_frames=a["n_active"], stream=sh)
+        ev[2].record(a["stream"])
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        upload_ms = ev[0].elapsed_time(ev[1])
+        out = {"ms": round(wall * 1e3, 2), "frame_mpix_s": round(f * stack.pixels_per_frame / 1e6 / wall, 1),
+               "upload_calibrate_warp_ms": round(upload_ms, 2),
+               "upload_gbps": round(raw_bytes / (upload_ms * 1e-3) / 1e9, 1),
+               "integrate_ms": round(ev[1].elapsed_time(ev[2]), 2), "raw_gb": round(raw_bytes / 1e9, 2),
+               "pinned_setup_s": round(setup_s, 2)}
+        log(f"host raw run {run}: {out}")
+    del host
+    return out
+
+
 def local_norm_leg(args, stack, weights):
     """Resident local normalisation of every active frame to frame 0 on the
     calibrated + warped stack left by the timed steps: the grid model
@@ -475,6 +529,9 @@ def main():
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
     stage_ms = {"calibrate": [], "warp": [], "integrate": []}
 
+    step_args = {"moving": moving, "mats": mats, "outs": outs, "weights": weights, "n_active": n_active,
+                 "light_exp": light_exp, "dark_exp": dark_exp, "policy": policy, "stream": stream}
+
     def step(timed):
         if timed:
             ev[0].record(stream)
@@ -574,6 +631,9 @@ def main():
             "avg_launch_ms": round(avg_launch_ms, 4),
         },
     }
+    if not args.no_host_raw:
+        log("host-raw leg: raw lights in pinned host memory, upload overlapped with calibration")
+        line["host_raw"] = host_raw_leg(args, stack, light_raw, step_args)
     if not args.no_local_norm:
         log("local-normalisation leg (grid and global models on the warped stack)")
         line["local_normalization"] = local_norm_leg(args, stack, weights)
