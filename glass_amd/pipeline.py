@@ -287,6 +287,53 @@ def run_resident_stack(
         stack.close()
 
 
+def write_resident_outputs(result: StackResult, output_dir, filter_name: str, frame_count: int | None = None,
+                           max_workers: int | None = None) -> dict:
+    """Write the output maps as simple FITS files the way the resident run
+    does (resident_cuda.py:13464-13662, writers :4798-4848): master and weight
+    float32, DQ int16, coverage / low / high counts rounded with np.rint to
+    int16 (int32 beyond 32767 frames); one worker thread per file."""
+    from concurrent.futures import ThreadPoolExecutor
+    from pathlib import Path
+
+    from . import fits as F
+
+    out = Path(output_dir)
+    out.mkdir(parents=True, exist_ok=True)
+    n = int(frame_count if frame_count is not None else len(result.weights))
+    count_dtype = np.int16 if n <= np.iinfo(np.int16).max else np.int32
+    filt = str(filter_name)
+    dtype_name = np.dtype(count_dtype).name
+    specs = [
+        ("master", out / f"resident_master_{filt}.fits", result.master, {"IMAGETYP": "master", "FILTER": filt},
+         np.float32, False),
+        ("weight", out / f"resident_weight_map_{filt}.fits", result.weight, {"IMAGETYP": "weight", "FILTER": filt},
+         np.float32, False),
+        ("dq", out / f"resident_dq_map_{filt}.fits", result.dq,
+         {"IMAGETYP": "dq_mask", "FILTER": filt, "DQSTAGE": "integration",
+          "DQFLAGS": "NO_DATA,WARP_EDGE,LOW_REJECTED,HIGH_REJECTED"}, np.int16, False),
+        ("coverage", out / f"resident_coverage_map_{filt}.fits", result.coverage,
+         {"IMAGETYP": "coverage", "FILTER": filt, "DTYPE": dtype_name}, count_dtype, True),
+        ("low_rejection", out / f"resident_low_rejection_map_{filt}.fits", result.low,
+         {"IMAGETYP": "rej_low", "FILTER": filt, "DTYPE": dtype_name}, count_dtype, True),
+        ("high_rejection", out / f"resident_high_rejection_map_{filt}.fits", result.high,
+         {"IMAGETYP": "rej_high", "FILTER": filt, "DTYPE": dtype_name}, count_dtype, True),
+    ]
+
+    def write(spec):
+        name, path, data, header, dtype, counts = spec
+        t = time.perf_counter()
+        F.write_fits_data(path, F.count_map_for_write(data, dtype) if counts else data, header, dtype=dtype)
+        return name, str(path), time.perf_counter() - t
+
+    t0 = time.perf_counter()
+    workers = max(1, min(len(specs), int(max_workers or len(specs))))
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        done = list(pool.map(write, specs))
+    return {"paths": {name: path for name, path, _ in done}, "breakdown_s": {name: s for name, _, s in done},
+            "total_s": time.perf_counter() - t0, "workers": workers}
+
+
 def dq_map(stack: ResidentCalibratedStack, weights, low_sigma, high_sigma, min_samples, max_reject_fraction,
            active_frames: int) -> np.ndarray:
     """Resident DQ map computed by the integration kernel itself (fused)."""

@@ -127,3 +127,27 @@ def test_raw_u16_layout_and_eligibility(tmp_path):
 def test_count_maps_round_to_nearest_even():
     got = F.count_map_for_write(np.array([0.5, 1.5, 2.5, 2.6, -0.5], np.float32))
     assert got.dtype == np.int16 and got.tolist() == [0, 2, 2, 3, 0]
+
+
+def test_write_resident_outputs(tmp_path):
+    from glass_amd.pipeline import StackResult, write_resident_outputs
+
+    rng = np.random.default_rng(1)
+    shape = (9, 12)
+    res = StackResult(master=rng.normal(100, 5, shape).astype(np.float32),
+                      weight=rng.uniform(0, 8, shape).astype(np.float32),
+                      coverage=np.full(shape, 7.0, np.float32), low=np.full(shape, 0.5, np.float32),
+                      high=np.full(shape, 1.5, np.float32), dq=np.full(shape, 0x341, np.uint32),
+                      geometric=np.full(shape, 8.0, np.float32), weights=np.ones(8, np.float32))
+    prof = write_resident_outputs(res, tmp_path, "H")
+    assert sorted(prof["paths"]) == ["coverage", "dq", "high_rejection", "low_rejection", "master", "weight"]
+    master = tmp_path / "resident_master_H.fits"
+    assert np.array_equal(F.read_image(master), res.master)
+    assert F.read_primary_header(master)[0]["IMAGETYP"] == "master"
+    cov = F.simple_fits_image_spec(tmp_path / "resident_coverage_map_H.fits")
+    assert cov.bitpix == 16 and F.read_primary_header(cov.path)[0]["DTYPE"] == "int16"
+    assert np.all(F.read_image(tmp_path / "resident_low_rejection_map_H.fits") == 0.0)   # rint(0.5) = 0
+    assert np.all(F.read_image(tmp_path / "resident_high_rejection_map_H.fits") == 2.0)  # rint(1.5) = 2
+    assert np.all(F.read_image(tmp_path / "resident_dq_map_H.fits") == 0x341)
+    assert F.read_primary_header(tmp_path / "resident_dq_map_H.fits")[0]["DQFLAGS"] == \
+        "NO_DATA,WARP_EDGE,LOW_REJECTED,HIGH_REJECTED"
