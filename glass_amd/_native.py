@@ -31,6 +31,8 @@ GH_WARP_AFFINE = 4
 
 GH_REJECT_NONE = 0
 GH_REJECT_WINSORIZED_SIGMA = 1
+GH_REJECT_SIGMA_CLIP = 2
+GH_CLIP_STACK = 0
 
 GH_COUNT_F32 = 0
 GH_COUNT_U16 = 1
@@ -73,6 +75,17 @@ class IntegrateOutputs(ctypes.Structure):
         ("dq", c_void_p),
         ("dq_use_geometric", c_int32),
         ("dq_active_frames", c_int32),
+    ]
+
+
+class ClipParams(ctypes.Structure):
+    _fields_ = [
+        ("sampler", c_int32),
+        ("rejection", c_int32),
+        ("winsorize", c_int32),
+        ("low_sigma", c_float),
+        ("high_sigma", c_float),
+        ("clamping_threshold", c_float),
     ]
 
 
@@ -155,6 +168,10 @@ _SIGNATURES = {
     "gh_warp_coverage_map": [c_void_p, c_void_p],
     "gh_integrate": [c_void_p, POINTER(c_float), POINTER(IntegrateParams), POINTER(IntegrateOutputs), c_void_p],
     "gh_frame_stats": [c_void_p, c_int64, c_int32, POINTER(FrameStatistics)],
+    "gh_integrate_clip": [
+        c_void_p, POINTER(c_float), POINTER(c_double), POINTER(ClipParams), c_void_p, c_void_p, c_void_p, c_void_p,
+        c_void_p, c_void_p, c_void_p,
+    ],
     "gh_frames_stats": [c_void_p, POINTER(c_int64), c_int64, POINTER(FrameStatistics)],
     "gh_pair_grid_stats": [
         c_void_p, c_int64, POINTER(c_int64), c_int64, c_int32, c_int32, POINTER(c_float), POINTER(c_float),

@@ -1,6 +1,7 @@
-// C ABI of the resident local normalisation (include/glass_hip.h):
-// pair-grid statistics and the in-place global / grid application
-// (native_bindings.cpp:10645-11290).
+// C ABI of the resident local normalisation (pair-grid statistics and the
+// in-place global / grid application, native_bindings.cpp:10645-11290), the
+// batched frame statistics and the mean / sigma-clip integrations that sample
+// the frames themselves (native_bindings.cpp:13227-13393, :14632-15115).
 
 #include <cmath>
 
@@ -177,6 +178,107 @@ int gh_apply_normalization(gh_stack* s, const int64_t* indices, int64_t count, c
     }
     s->mark_caller(st);
     if (!dev_s || !stream_ptr) check(hipStreamSynchronize(st), "apply normalization");
+  });
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int gh_integrate_clip(gh_stack* s, const float* weights, const double* inverses, const gh_clip_params* params,
+                      float* master, float* weight, float* coverage, float* low, float* high, float* geometric,
+                      void* stream_ptr) {
+  return guarded([&] {
+    if (!params) throw gh::ValueError("clip parameters pointer is NULL");
+    const gh_clip_params prm = *params;
+    if (prm.sampler != GH_CLIP_STACK && prm.sampler != GH_INTERP_BILINEAR && prm.sampler != GH_INTERP_LANCZOS3)
+      throw gh::ValueError("interpolation must be bilinear or lanczos3");
+    if (prm.rejection != GH_REJECT_NONE && prm.rejection != GH_REJECT_SIGMA_CLIP)
+      throw gh::ValueError("unsupported rejection for the clip integration");
+    if (prm.rejection == GH_REJECT_SIGMA_CLIP && !(prm.low_sigma > 0.0f && prm.high_sigma > 0.0f))
+      throw gh::ValueError("sigma thresholds must be positive");
+    if (s->loaded_count != s->frames)
+      throw std::runtime_error("all resident frames must be loaded before integration");
+    if (prm.sampler != GH_CLIP_STACK && !inverses) throw gh::ValueError("inverse matrices pointer is NULL");
+    if (s->npix >= (int64_t(1) << 31)) throw gh::ValueError("frames of 2^31 pixels or more are not supported");
+    // Frames of positive finite weight, in order (the kernels skip the rest).
+    std::vector<const float*> table;
+    std::vector<float> w;
+    std::vector<double> inv;
+    for (int64_t f = 0; f < s->frames; ++f) {
+      const float wf = weights ? weights[f] : 1.0f;
+      if (!(wf > 0.0f) || !std::isfinite(wf)) continue;
+      table.push_back(s->slot[f]);
+      w.push_back(wf);
+      if (prm.sampler != GH_CLIP_STACK) {
+        for (int e = 0; e < 9; ++e) {
+          if (!std::isfinite(inverses[f * 9 + e])) throw gh::ValueError("warp inverse matrix must be finite");
+          inv.push_back(inverses[f * 9 + e]);
+        }
+      }
+    }
+    hipStream_t st = s->pick(stream_ptr);
+    if (st == s->stream) s->wait_caller();
+    const size_t n = table.size();
+    // one allocation for the pointer table, weights and inverses
+    const size_t table_bytes = (n * sizeof(const float*) + 255) / 256 * 256;
+    const size_t w_bytes = (n * sizeof(float) + 255) / 256 * 256;
+    const size_t inv_bytes = inv.size() * sizeof(double);
+    s->clip_tables.ensure(table_bytes + w_bytes + inv_bytes + 8, "hipMalloc(clip tables)");
+    unsigned char* base = s->clip_tables.ptr;
+    // the tables may still be read by a clip integration queued earlier
+    check(hipStreamSynchronize(st), "clip tables");
+    if (n) {
+      check(hipMemcpy(base, table.data(), n * sizeof(const float*), hipMemcpyHostToDevice), "hipMemcpy(clip table)");
+      check(hipMemcpy(base + table_bytes, w.data(), n * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy(clip weights)");
+      if (inv_bytes)
+        check(hipMemcpy(base + table_bytes + w_bytes, inv.data(), inv_bytes, hipMemcpyHostToDevice),
+              "hipMemcpy(clip inverses)");
+    }
+    gh::ClipArgs a{};
+    a.frames = reinterpret_cast<const float* const*>(base);
+    a.weights = reinterpret_cast<const float*>(base + table_bytes);
+    a.inverses = inv_bytes ? reinterpret_cast<const double*>(base + table_bytes + w_bytes) : nullptr;
+    a.n = static_cast<int>(n);
+    a.width = static_cast<int>(s->width);
+    a.height = static_cast<int>(s->height);
+    a.npix = s->npix;
+    a.clamp = prm.clamping_threshold;
+    a.mode = prm.rejection == GH_REJECT_SIGMA_CLIP ? 1 : 0;
+    a.winsorize = prm.winsorize ? 1 : 0;
+    a.low_sigma = prm.low_sigma;
+    a.high_sigma = prm.high_sigma;
+    float* user[6] = {master, weight, coverage, low, high, geometric};
+    float* dev[6] = {};
+    // Host outputs go through device temporaries (the integration outputs'
+    // five, plus one for the geometric coverage) and are copied back.
+    for (int i = 0; i < 6; ++i) {
+      if (!user[i]) continue;
+      if (is_device_pointer(user[i])) {
+        dev[i] = user[i];
+        continue;
+      }
+      DeviceBuffer<float>& tmp = i < 5 ? s->out_f32[i] : s->clip_geometric;
+      tmp.ensure(static_cast<size_t>(s->npix), "hipMalloc(clip output)");
+      dev[i] = tmp.ptr;
+    }
+    a.master = dev[0];
+    a.weight = dev[1];
+    a.coverage = dev[2];
+    a.low = dev[3];
+    a.high = dev[4];
+    a.geometric = dev[5];
+    const int sampler = prm.sampler == GH_CLIP_STACK ? 0 : (prm.sampler == GH_INTERP_BILINEAR ? 1 : 2);
+    gh::launch_clip_integrate(a, sampler, st);
+    bool any_host = false;
+    for (int i = 0; i < 6; ++i) {
+      if (!user[i] || dev[i] == user[i]) continue;
+      any_host = true;
+      check(hipMemcpyAsync(user[i], dev[i], static_cast<size_t>(s->npix) * sizeof(float), hipMemcpyDeviceToHost, st),
+            "hipMemcpy(clip output)");
+    }
+    s->mark_caller(st);
+    if (any_host || !stream_ptr) check(hipStreamSynchronize(st), "clip integration");
   });
 }
 

@@ -192,6 +192,23 @@ struct VariantJob {
 };
 void launch_decide_variants(const VariantJob* jobs, int n_jobs, int* out, hipStream_t stream);
 
+// Mean / sigma-clip integration that samples the frames itself (resident
+// stack as stored, or warped on the fly through FP64 inverses).
+struct ClipArgs {
+  const float* const* frames;  // [n] device table, frames of positive finite weight in order
+  const double* inverses;      // [n][9] (samplers 1, 2)
+  const float* weights;        // [n]
+  int n, width, height;
+  long long npix;
+  float clamp;                 // Lanczos3 clamp (< 0: off)
+  int mode;                    // 0 weighted mean, 1 sigma clip
+  int winsorize;
+  float low_sigma, high_sigma;
+  float *master, *weight, *coverage, *low, *high, *geometric;  // nullable outputs
+};
+// sampler: 0 stack as stored, 1 bilinear, 2 Lanczos3 (resident tap rules)
+void launch_clip_integrate(const ClipArgs& a, int sampler, hipStream_t stream);
+
 // Resident local normalisation.  A batch names up to kLnBatch resident
 // frames (sources for the statistics, targets for the application) and their
 // non-finite flags (application only).

@@ -135,6 +135,8 @@ struct gh_stack {
   DeviceBuffer<double> ln_sums;             // local normalisation: per-tile FP64 sums
   DeviceBuffer<unsigned long long> ln_counts;  //   and paired-pixel counts
   DeviceBuffer<float> ln_coef;              //   host coefficients staged for the application
+  DeviceBuffer<unsigned char> clip_tables;  // clip integration: frame table, weights, inverses
+  DeviceBuffer<float> clip_geometric;       //   host geometric-coverage output staging
 
   hipStream_t pick(void* s) const { return s ? static_cast<hipStream_t>(s) : stream; }
 
@@ -270,6 +272,8 @@ struct gh_stack {
     ln_sums.release();
     ln_counts.release();
     ln_coef.release();
+    clip_tables.release();
+    clip_geometric.release();
     if (copy_stream) (void)hipStreamSynchronize(copy_stream);
     for (int i = 0; i < kRawRing; ++i) {
       if (ring_copied[i]) (void)hipEventDestroy(ring_copied[i]);

@@ -768,6 +768,210 @@ __global__ void snapshot_warp_flags_kernel(WarpBatch b, int count, unsigned* sna
   }
 }
 
+// ------------------------------------------------------------------------
+// Mean / sigma-clip integration that samples every frame itself: from the
+// resident stack as stored (SAMPLER 0) or through each frame's matrix with
+// bilinear (1) or Lanczos3 (2) taps, without writing a warped stack.  Per
+// pixel, over the frames of positive finite weight in order
+// (cpp/cuda/integration_kernels.cu:1743-1900, :3840-4123):
+//  * mean (mode 0): sum w*x and sum w over finite samples, master = sum / sum w;
+//  * sigma clip (mode 1): mean and population std of the finite samples,
+//    thresholds mean -/+ k*std; with winsorize, the samples clipped to them give
+//    a new mean and std and new thresholds; samples outside are rejected (low /
+//    high counts) and the weighted mean of the rest is the master.
+// All sums are float32 in frame order, as in the reference kernels.  Each
+// frame is sampled once (into registers or LDS, see STORE) instead of once per
+// pass as the reference does; beyond the stores' capacity every pass samples
+// again.
+// A sample outside the footprint is neither counted nor covered; one inside
+// it adds to the geometric coverage and counts only when finite.
+template <int SAMPLER>
+__device__ __forceinline__ float clip_sample(const ClipArgs& a, int k, int x, int y, bool& footprint) {
+  const float* frame = table_at(a.frames, k);
+  if (SAMPLER == 0) {
+    footprint = true;
+    return ld_global(frame, static_cast<long long>(y) * a.width + x);
+  }
+  const double* m = a.inverses + 9 * k;
+  double sx, sy;
+  footprint = source_coords(m, 0, x, y, sx, sy) && isfinite(sx) && isfinite(sy);
+  if (SAMPLER == 1) {
+    footprint = footprint && sx >= 0.0 && sx <= static_cast<double>(a.width - 1) && sy >= 0.0 &&
+                sy <= static_cast<double>(a.height - 1);
+    if (!footprint) return NAN;
+    const double fx0 = floor(sx), fy0 = floor(sy);
+    const int x0 = static_cast<int>(fx0), y0 = static_cast<int>(fy0);
+    const int x1 = min(x0 + 1, a.width - 1), y1 = min(y0 + 1, a.height - 1);
+    const float tx = static_cast<float>(sx - fx0), ty = static_cast<float>(sy - fy0);
+    const float a00 = ld_global(frame, static_cast<long long>(y0) * a.width + x0);
+    const float a01 = ld_global(frame, static_cast<long long>(y0) * a.width + x1);
+    const float a10 = ld_global(frame, static_cast<long long>(y1) * a.width + x0);
+    const float a11 = ld_global(frame, static_cast<long long>(y1) * a.width + x1);
+    const float top = a00 * (1.0f - tx) + a01 * tx;
+    const float bottom = a10 * (1.0f - tx) + a11 * tx;
+    return top * (1.0f - ty) + bottom * ty;
+  }
+  footprint = footprint && lanczos_footprint(sx, sy, a.width, a.height);
+  if (!footprint) return NAN;
+  WarpParams p{};
+  p.fill = NAN;
+  p.clamp = a.clamp;
+  p.skip_nonfinite = 1;
+  bool covered;
+  const float v = lanczos_direct(frame, a.width, sx, sy, p, 1.0e-6f, covered);
+  footprint = covered;
+  return v;
+}
+
+// Sample storage: STORE 0 registers (CAP compile-time slots, sampler 0),
+// 1 LDS (n slots per thread, dynamic shared memory, kClipLdsBlock threads: the
+// warped samplers' per-frame code is too large to unroll CAP times), 2 none
+// (every pass samples again).
+constexpr int kClipLdsBlock = 128;
+constexpr int kClipLdsFrames = 128;  // 64 KB of LDS per block
+
+template <int SAMPLER, int CAP, int STORE>
+__global__ __launch_bounds__(256) void clip_integrate_kernel(ClipArgs a) {
+  extern __shared__ float clip_lds[];
+  const int nthreads = STORE == 1 ? kClipLdsBlock : 256;
+  const long long p = static_cast<long long>(blockIdx.x) * nthreads + threadIdx.x;
+  const bool live = p < a.npix;
+  const long long pp = live ? p : 0;
+  const int y = static_cast<int>(pp / a.width), x = static_cast<int>(pp - static_cast<long long>(y) * a.width);
+  float v[CAP > 0 ? CAP : 1];
+  float geometric = 0.0f;
+  auto sample = [&](int k, float& out) -> bool {
+    bool fp;
+    out = clip_sample<SAMPLER>(a, k, x, y, fp);
+    return fp;
+  };
+  if constexpr (STORE == 0) {
+#pragma unroll
+    for (int k = 0; k < CAP; ++k) {
+      if (k < a.n) {
+        float s;
+        const bool fp = sample(k, s);
+        geometric += fp ? 1.0f : 0.0f;
+        v[k] = (fp && isfinite(s)) ? s : NAN;
+      }
+    }
+  } else {
+    for (int k = 0; k < a.n; ++k) {
+      float s;
+      const bool fp = sample(k, s);
+      geometric += fp ? 1.0f : 0.0f;
+      if constexpr (STORE == 1) clip_lds[k * kClipLdsBlock + threadIdx.x] = (fp && isfinite(s)) ? s : NAN;
+    }
+  }
+  if (!live) return;  // no barrier below: each thread reads only its own slots
+  // value k of the pass (NaN when it does not count)
+  auto value = [&](int k) -> float {
+    if constexpr (STORE == 0) {
+      return v[k];
+    } else if constexpr (STORE == 1) {
+      return clip_lds[k * kClipLdsBlock + threadIdx.x];
+    } else {
+      float s;
+      const bool fp = sample(k, s);
+      return (fp && isfinite(s)) ? s : NAN;
+    }
+  };
+  constexpr int kUnroll = STORE == 0 ? CAP : 1;
+  float master = 0.0f, wsum = 0.0f, coverage = 0.0f, low = 0.0f, high = 0.0f;
+  if (a.mode == 0) {
+    float sum = 0.0f;
+#pragma unroll kUnroll
+    for (int k = 0; k < (STORE == 0 ? CAP : a.n); ++k) {
+      if (k >= a.n) break;
+      const float s = value(k);
+      if (!isfinite(s)) continue;
+      const float w = ld_global(a.weights, k);
+      sum += s * w;
+      wsum += w;
+      coverage += 1.0f;
+    }
+    master = wsum > 0.0f ? sum / wsum : 0.0f;
+  } else {
+    float mean = 0.0f, count = 0.0f;
+#pragma unroll kUnroll
+    for (int k = 0; k < (STORE == 0 ? CAP : a.n); ++k) {
+      if (k >= a.n) break;
+      const float s = value(k);
+      if (isfinite(s)) {
+        mean += s;
+        count += 1.0f;
+      }
+    }
+    if (count > 0.0f) {
+      mean /= count;
+      float var = 0.0f;
+#pragma unroll kUnroll
+      for (int k = 0; k < (STORE == 0 ? CAP : a.n); ++k) {
+        if (k >= a.n) break;
+        const float s = value(k);
+        if (isfinite(s)) {
+          const float d = s - mean;
+          var += d * d;
+        }
+      }
+      float center = mean, scale = sqrtf(var / count);
+      float lo = center - a.low_sigma * scale, hi = center + a.high_sigma * scale;
+      if (a.winsorize) {
+        float wmean = 0.0f;
+#pragma unroll kUnroll
+        for (int k = 0; k < (STORE == 0 ? CAP : a.n); ++k) {
+          if (k >= a.n) break;
+          float s = value(k);
+          if (!isfinite(s)) continue;
+          s = s < lo ? lo : (s > hi ? hi : s);
+          wmean += s;
+        }
+        wmean /= count;
+        float wvar = 0.0f;
+#pragma unroll kUnroll
+        for (int k = 0; k < (STORE == 0 ? CAP : a.n); ++k) {
+          if (k >= a.n) break;
+          float s = value(k);
+          if (!isfinite(s)) continue;
+          s = s < lo ? lo : (s > hi ? hi : s);
+          const float d = s - wmean;
+          wvar += d * d;
+        }
+        center = wmean;
+        scale = sqrtf(wvar / count);
+        lo = center - a.low_sigma * scale;
+        hi = center + a.high_sigma * scale;
+      }
+      float sum = 0.0f;
+#pragma unroll kUnroll
+      for (int k = 0; k < (STORE == 0 ? CAP : a.n); ++k) {
+        if (k >= a.n) break;
+        const float s = value(k);
+        if (!isfinite(s)) continue;
+        if (s < lo) {
+          low += 1.0f;
+          continue;
+        }
+        if (s > hi) {
+          high += 1.0f;
+          continue;
+        }
+        const float w = ld_global(a.weights, k);
+        sum += s * w;
+        wsum += w;
+        coverage += 1.0f;
+      }
+      master = wsum > 0.0f ? sum / wsum : 0.0f;
+    }
+  }
+  if (a.master) st_global(a.master, p, master);
+  if (a.weight) st_global(a.weight, p, wsum);
+  if (a.coverage) st_global(a.coverage, p, coverage);
+  if (a.low) st_global(a.low, p, low);
+  if (a.high) st_global(a.high, p, high);
+  if (a.geometric) st_global(a.geometric, p, geometric);
+}
+
 }  // namespace
 
 void make_lanczos_lut(float* table) {
@@ -861,6 +1065,29 @@ void launch_warp(const WarpBatch& batch, int count, const WarpParams& p, hipStre
     else warp_lanczos3_kernel<false><<<grid, block, 0, stream>>>(batch, count, p);
   }
   check(hipGetLastError(), "warp kernel launch");
+}
+
+void launch_clip_integrate(const ClipArgs& a, int sampler, hipStream_t stream) {
+  const unsigned grid256 = static_cast<unsigned>((a.npix + 255) / 256);
+  const unsigned grid_lds = static_cast<unsigned>((a.npix + kClipLdsBlock - 1) / kClipLdsBlock);
+  const size_t lds = static_cast<size_t>(a.n) * kClipLdsBlock * sizeof(float);
+#define GH_CLIP_LDS(S) clip_integrate_kernel<S, 0, 1><<<grid_lds, kClipLdsBlock, lds, stream>>>(a)
+#define GH_CLIP_AGAIN(S) clip_integrate_kernel<S, 0, 2><<<grid256, 256, 0, stream>>>(a)
+  if (sampler == 0) {
+    if (a.n <= 16) clip_integrate_kernel<0, 16, 0><<<grid256, 256, 0, stream>>>(a);
+    else if (a.n <= 64) clip_integrate_kernel<0, 64, 0><<<grid256, 256, 0, stream>>>(a);
+    else if (a.n <= 128) clip_integrate_kernel<0, 128, 0><<<grid256, 256, 0, stream>>>(a);
+    else GH_CLIP_AGAIN(0);
+  } else if (sampler == 1) {
+    if (a.n <= kClipLdsFrames) GH_CLIP_LDS(1);
+    else GH_CLIP_AGAIN(1);
+  } else {
+    if (a.n <= kClipLdsFrames) GH_CLIP_LDS(2);
+    else GH_CLIP_AGAIN(2);
+  }
+#undef GH_CLIP_LDS
+#undef GH_CLIP_AGAIN
+  check(hipGetLastError(), "clip integrate launch");
 }
 
 }  // namespace gh

@@ -751,6 +751,98 @@ class ResidentCalibratedStack:
         out = self._integrate(weights, params, {"master", "weight"}, N.GH_COUNT_F32)
         return out["master"], out["weight"]
 
+    # ------------------------------------------------ sigma clip / fused warp
+    def _clip(self, weights, matrices, interpolation, clamping_threshold, rejection, low_sigma, high_sigma,
+              winsorize, download_mode, geometric: bool):
+        if download_mode not in ("full", "master_weight", "master_only"):
+            raise ValueError("download_mode must be full, master_weight, or master_only")
+        f = self.frame_count
+        w = None
+        if weights is not None:
+            w = np.ascontiguousarray(np.asarray(weights, dtype=np.float32))
+            if w.ndim != 1 or w.size != f:
+                raise ValueError("weights must have shape (frame_count,)")
+        inv = None
+        sampler = N.GH_CLIP_STACK
+        if matrices is not None:
+            if interpolation not in ("bilinear", "lanczos3"):
+                raise ValueError("interpolation must be bilinear or lanczos3")
+            mats = np.asarray(matrices, dtype=np.float64)
+            if mats.shape != (f, 3, 3):
+                raise ValueError("matrices must have shape (frame_count, 3, 3)")
+            inv = np.ascontiguousarray(np.stack([invert_affine(m) for m in mats]), dtype=np.float64)
+            sampler = N.GH_INTERP_LANCZOS3 if interpolation == "lanczos3" else N.GH_INTERP_BILINEAR
+        prm = N.ClipParams(sampler, rejection, int(bool(winsorize)), float(low_sigma), float(high_sigma),
+                           float(clamping_threshold))
+        shape = (self.height, self.width)
+        want_weight = download_mode != "master_only"
+        full = download_mode == "full"
+        master = np.empty(shape, np.float32)
+        maps = {"weight": np.empty(shape, np.float32) if want_weight else None}
+        for name in ("coverage", "low", "high"):
+            maps[name] = np.empty(shape, np.float32) if full else None
+        maps["geometric"] = np.empty(shape, np.float32) if (full and geometric) else None
+        if rejection == N.GH_REJECT_NONE:
+            maps["low"] = maps["high"] = None
+
+        def ptr(a):
+            return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+        t0 = time.perf_counter()
+        N.call("gh_integrate_clip", self._h,
+               None if w is None else w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+               None if inv is None else inv.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(prm),
+               ptr(master), ptr(maps["weight"]), ptr(maps["coverage"]), ptr(maps["low"]), ptr(maps["high"]),
+               ptr(maps["geometric"]), None)
+        return master, maps, time.perf_counter() - t0
+
+    def integrate_sigma_clip(self, weights=None, low_sigma=3.0, high_sigma=3.0, winsorize=True,
+                             download_mode="full"):
+        """integrate_sigma_clip (native_bindings.cpp:13227-13393) -> (master,
+        weight|None, coverage|None, low|None, high|None): mean / std sigma clip
+        with an optional winsorized re-estimate, float32 sums in frame order."""
+        if not (float(low_sigma) > 0.0 and float(high_sigma) > 0.0):
+            raise ValueError("sigma thresholds must be positive")
+        master, m, _ = self._clip(weights, None, "bilinear", -1.0, N.GH_REJECT_SIGMA_CLIP, low_sigma, high_sigma,
+                                  winsorize, download_mode, geometric=False)
+        return master, m["weight"], m["coverage"], m["low"], m["high"]
+
+    def integrate_matrix_warped_mean(self, matrices, weights=None, interpolation="bilinear", clamping_threshold=-1.0,
+                                     download_mode="full"):
+        """integrate_matrix_warped_mean (native_bindings.cpp:14632-14838) ->
+        (master, weight|None, coverage|None, geometric|None, timing): the
+        weighted mean of every frame resampled through its matrix on the fly;
+        the resident stack is not modified."""
+        master, m, call_s = self._clip(weights, matrices, interpolation, clamping_threshold, N.GH_REJECT_NONE, 3.0,
+                                       3.0, False, download_mode, geometric=True)
+        timing = {"schema_version": 1, "timing_model": "native_fused_matrix_warp_weighted_mean_one_sync",
+                  "interpolation": interpolation, "clamping_threshold": float(clamping_threshold),
+                  "rejection": "none", "frame_count": self.frame_count, "total_s": call_s,
+                  "download_mode": download_mode, "diagnostic_maps_downloaded": download_mode == "full",
+                  "weight_map_downloaded": download_mode != "master_only", "avoids_stack_scatter": True,
+                  "modifies_resident_stack": False}
+        return master, m["weight"], m["coverage"], m["geometric"], timing
+
+    def integrate_matrix_warped_sigma_clip(self, matrices, weights=None, interpolation="bilinear",
+                                           clamping_threshold=-1.0, low_sigma=3.0, high_sigma=3.0, winsorize=True,
+                                           download_mode="full"):
+        """integrate_matrix_warped_sigma_clip (native_bindings.cpp:14840-15115)
+        -> (master, weight|None, coverage|None, low|None, high|None,
+        geometric|None, timing)."""
+        if not (float(low_sigma) > 0.0 and float(high_sigma) > 0.0):
+            raise ValueError("sigma thresholds must be positive")
+        master, m, call_s = self._clip(weights, matrices, interpolation, clamping_threshold, N.GH_REJECT_SIGMA_CLIP,
+                                       low_sigma, high_sigma, winsorize, download_mode, geometric=True)
+        timing = {"schema_version": 1, "timing_model": "native_fused_matrix_warp_sigma_clip_one_sync",
+                  "interpolation": interpolation, "clamping_threshold": float(clamping_threshold),
+                  "rejection": "winsorized_sigma" if winsorize else "sigma_clip", "winsorize": bool(winsorize),
+                  "low_sigma": float(low_sigma), "high_sigma": float(high_sigma), "frame_count": self.frame_count,
+                  "total_s": call_s, "download_mode": download_mode,
+                  "diagnostic_maps_downloaded": download_mode == "full",
+                  "weight_map_downloaded": download_mode != "master_only", "avoids_stack_scatter": True,
+                  "modifies_resident_stack": False}
+        return master, m["weight"], m["coverage"], m["low"], m["high"], m["geometric"], timing
+
     def integrate_hardened_winsorized_sigma(self, weights=None, low_sigma=3.0, high_sigma=3.0, min_samples=3,
                                             max_reject_fraction=0.5, count_map_dtype="float32", profile=False,
                                             download_mode="full"):

@@ -57,6 +57,7 @@ typedef struct gh_stack gh_stack;
 /* Rejection modes of gh_integrate. */
 #define GH_REJECT_NONE 0
 #define GH_REJECT_WINSORIZED_SIGMA 1
+#define GH_REJECT_SIGMA_CLIP 2
 
 /* Count-map dtypes of gh_integrate. */
 #define GH_COUNT_F32 0
@@ -189,6 +190,35 @@ int gh_warp_coverage_map(gh_stack* stack, float* out);
  * or NULL (unit).  Zero / non-finite weights exclude the frame. */
 int gh_integrate(gh_stack* stack, const float* weights, const gh_integrate_params* params,
                  const gh_integrate_outputs* outputs, void* stream);
+
+/* Parameters of gh_integrate_clip. */
+typedef struct gh_clip_params {
+  int32_t sampler;            /* GH_CLIP_STACK, GH_INTERP_BILINEAR or GH_INTERP_LANCZOS3 */
+  int32_t rejection;          /* GH_REJECT_NONE (weighted mean) or GH_REJECT_SIGMA_CLIP */
+  int32_t winsorize;          /* sigma clip: one winsorized re-estimate of centre and scale */
+  float low_sigma, high_sigma;
+  float clamping_threshold;   /* Lanczos3 sampler: clamp to [min - t*r, max + t*r] of the taps; < 0 off */
+} gh_clip_params;
+
+#define GH_CLIP_STACK 0
+
+/* integrate_sigma_clip (native_bindings.cpp:13227-13393; kernel
+ * cpp/cuda/integration_kernels.cu:1743-1900) and the fused matrix-warped forms
+ * integrate_matrix_warped_mean / integrate_matrix_warped_sigma_clip
+ * (native_bindings.cpp:14632-15115; kernels integration_kernels.cu:3840-4123):
+ * per pixel over the frames of positive finite weight, in frame order, with
+ * float32 sums -- weighted mean of the finite samples, or the mean / population
+ * std sigma clip (optionally one winsorized re-estimate), rejected sample
+ * counts and the weighted mean of the rest.  The samples are the resident
+ * frames as stored (GH_CLIP_STACK) or the frames resampled on the fly through
+ * inverses[k] (FP64 3x3, output -> source, k over all frames) with the
+ * resident tap rules (non-finite Lanczos3 taps skipped); the stack is not
+ * modified.  Outputs (host or device, NULL = not produced) are f32 [H*W]:
+ * master, weight sum, kept count, low / high rejection counts and geometric
+ * coverage (samples inside the footprint).  Every frame must be loaded. */
+int gh_integrate_clip(gh_stack* stack, const float* weights, const double* inverses, const gh_clip_params* params,
+                      float* master, float* weight, float* coverage, float* low, float* high, float* geometric,
+                      void* stream);
 
 /* frame_global_stats(i) (native_bindings.cpp:10569-10643) plus exact median /
  * MAD (src/glass/engine/quality.py:126-200).  FP64. */
