@@ -53,6 +53,8 @@ def parse_args():
     p.add_argument("--batch-reps", type=int, default=3)
     p.add_argument("--no-local-norm", action="store_true", help="skip the local-normalisation leg")
     p.add_argument("--no-host-raw", action="store_true", help="skip the host-raw (PCIe-inclusive) leg")
+    p.add_argument("--fused-leg", action="store_true",
+                   help="also time the fused matrix-warped integrations on the calibrated stack")
     p.add_argument("--local-norm-tile", type=int, default=512)
     p.add_argument("--copy-probe-mib", type=int, default=0,
                    help="issue one untimed device-to-device copy of this many MiB before the timed steps "
@@ -356,6 +358,33 @@ _TRAFFIC_KERNELS = {"calibrate": "gh::calibrate_batch_kernel<2, false, 16>", "wa
                     "integrate": "gh::integrate_winsorized_kernel<208, true>"}
 
 
+def fused_leg(args, stack, light_raw, a):
+    """integrate_matrix_warped_mean / _sigma_clip over the calibrated,
+    unwarped stack (every frame resampled inside the integration, no warped
+    stack written), bilinear and Lanczos3; wall clock per call (each call is
+    synchronous), second of two runs."""
+    import torch
+
+    f = stack.frame_count
+    stack.calibrate_frames_device(list(range(f)), light_raw, "fits_u16be_bzero", [a["light_exp"]] * f, a["dark_exp"],
+                                  a["policy"])
+    torch.cuda.synchronize()
+    out = {}
+    for interp in ("bilinear", "lanczos3"):
+        for rej in ("mean", "sigma_clip"):
+            for _ in range(2):
+                t = time.perf_counter()
+                if rej == "mean":
+                    stack.integrate_matrix_warped_mean(a["mats"], a["weights"], interp, -1.0, "master_only")
+                else:
+                    stack.integrate_matrix_warped_sigma_clip(a["mats"], a["weights"], interp, -1.0, 3.0, 3.0, True,
+                                                             "master_only")
+                dt = time.perf_counter() - t
+            out[f"{interp}_{rej}_ms"] = round(dt * 1e3, 1)
+            log(f"fused {interp} {rej}: {dt * 1e3:.1f} ms")
+    return out
+
+
 def host_raw_leg(args, stack, light_raw, a):
     """The plain step with the raw lights handed over in pinned host memory
     instead of HBM: every frame crosses PCIe through the library's upload
@@ -631,6 +660,9 @@ def main():
             "avg_launch_ms": round(avg_launch_ms, 4),
         },
     }
+    if args.fused_leg:
+        log("fused matrix-warped integrations on the calibrated (unwarped) stack")
+        line["fused_matrix_warped"] = fused_leg(args, stack, light_raw, step_args)
     if not args.no_host_raw:
         log("host-raw leg: raw lights in pinned host memory, upload overlapped with calibration")
         line["host_raw"] = host_raw_leg(args, stack, light_raw, step_args)

@@ -824,16 +824,15 @@ __device__ __forceinline__ float clip_sample(const ClipArgs& a, int k, int x, in
 }
 
 // Sample storage: STORE 0 registers (CAP compile-time slots, sampler 0),
-// 1 LDS (n slots per thread, dynamic shared memory, kClipLdsBlock threads: the
+// 1 LDS (n slots per thread in dynamic shared memory, 128-thread blocks up to
+// 128 frames and 64-thread blocks up to 256, at most 64 KB per block: the
 // warped samplers' per-frame code is too large to unroll CAP times), 2 none
 // (every pass samples again).
-constexpr int kClipLdsBlock = 128;
-constexpr int kClipLdsFrames = 128;  // 64 KB of LDS per block
 
 template <int SAMPLER, int CAP, int STORE>
 __global__ __launch_bounds__(256) void clip_integrate_kernel(ClipArgs a) {
   extern __shared__ float clip_lds[];
-  const int nthreads = STORE == 1 ? kClipLdsBlock : 256;
+  const int nthreads = STORE == 1 ? static_cast<int>(blockDim.x) : 256;
   const long long p = static_cast<long long>(blockIdx.x) * nthreads + threadIdx.x;
   const bool live = p < a.npix;
   const long long pp = live ? p : 0;
@@ -860,7 +859,7 @@ __global__ __launch_bounds__(256) void clip_integrate_kernel(ClipArgs a) {
       float s;
       const bool fp = sample(k, s);
       geometric += fp ? 1.0f : 0.0f;
-      if constexpr (STORE == 1) clip_lds[k * kClipLdsBlock + threadIdx.x] = (fp && isfinite(s)) ? s : NAN;
+      if constexpr (STORE == 1) clip_lds[k * nthreads + threadIdx.x] = (fp && isfinite(s)) ? s : NAN;
     }
   }
   if (!live) return;  // no barrier below: each thread reads only its own slots
@@ -869,7 +868,7 @@ __global__ __launch_bounds__(256) void clip_integrate_kernel(ClipArgs a) {
     if constexpr (STORE == 0) {
       return v[k];
     } else if constexpr (STORE == 1) {
-      return clip_lds[k * kClipLdsBlock + threadIdx.x];
+      return clip_lds[k * nthreads + threadIdx.x];
     } else {
       float s;
       const bool fp = sample(k, s);
@@ -1069,9 +1068,10 @@ void launch_warp(const WarpBatch& batch, int count, const WarpParams& p, hipStre
 
 void launch_clip_integrate(const ClipArgs& a, int sampler, hipStream_t stream) {
   const unsigned grid256 = static_cast<unsigned>((a.npix + 255) / 256);
-  const unsigned grid_lds = static_cast<unsigned>((a.npix + kClipLdsBlock - 1) / kClipLdsBlock);
-  const size_t lds = static_cast<size_t>(a.n) * kClipLdsBlock * sizeof(float);
-#define GH_CLIP_LDS(S) clip_integrate_kernel<S, 0, 1><<<grid_lds, kClipLdsBlock, lds, stream>>>(a)
+  const int lds_block = a.n <= 128 ? 128 : 64;
+  const unsigned grid_lds = static_cast<unsigned>((a.npix + lds_block - 1) / lds_block);
+  const size_t lds = static_cast<size_t>(a.n) * lds_block * sizeof(float);
+#define GH_CLIP_LDS(S) clip_integrate_kernel<S, 0, 1><<<grid_lds, lds_block, lds, stream>>>(a)
 #define GH_CLIP_AGAIN(S) clip_integrate_kernel<S, 0, 2><<<grid256, 256, 0, stream>>>(a)
   if (sampler == 0) {
     if (a.n <= 16) clip_integrate_kernel<0, 16, 0><<<grid256, 256, 0, stream>>>(a);
@@ -1079,10 +1079,10 @@ void launch_clip_integrate(const ClipArgs& a, int sampler, hipStream_t stream) {
     else if (a.n <= 128) clip_integrate_kernel<0, 128, 0><<<grid256, 256, 0, stream>>>(a);
     else GH_CLIP_AGAIN(0);
   } else if (sampler == 1) {
-    if (a.n <= kClipLdsFrames) GH_CLIP_LDS(1);
+    if (a.n <= 256) GH_CLIP_LDS(1);
     else GH_CLIP_AGAIN(1);
   } else {
-    if (a.n <= kClipLdsFrames) GH_CLIP_LDS(2);
+    if (a.n <= 256) GH_CLIP_LDS(2);
     else GH_CLIP_AGAIN(2);
   }
 #undef GH_CLIP_LDS

@@ -74,7 +74,7 @@ def test_sigma_clip_stack_bit_exact(gpu, f, winsorize):
     assert np.array_equal(after, x, equal_nan=True)
 
 
-@pytest.mark.parametrize("f,rejection", [(5, "mean"), (9, "sigma_clip"), (140, "sigma_clip")])
+@pytest.mark.parametrize("f,rejection", [(5, "mean"), (9, "sigma_clip"), (140, "sigma_clip"), (260, "mean")])
 def test_matrix_warped_bilinear_bit_exact(gpu, f, rejection):
     x = _frames(f, 30, 38, seed=3 * f)
     mats = _mats(f, f)
