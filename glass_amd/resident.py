@@ -820,7 +820,10 @@ class ResidentCalibratedStack:
                   "rejection": "none", "frame_count": self.frame_count, "total_s": call_s,
                   "download_mode": download_mode, "diagnostic_maps_downloaded": download_mode == "full",
                   "weight_map_downloaded": download_mode != "master_only", "avoids_stack_scatter": True,
-                  "modifies_resident_stack": False}
+                  "modifies_resident_stack": False, "inverse_batch_bytes": self.frame_count * 9 * 8,
+                  "weights_bytes": self.frame_count * 4,
+                  "output_bytes": self.pixels_per_frame * 4 * (1 + (download_mode != "master_only")
+                                                               + 2 * (download_mode == "full"))}
         return master, m["weight"], m["coverage"], m["geometric"], timing
 
     def integrate_matrix_warped_sigma_clip(self, matrices, weights=None, interpolation="bilinear",
@@ -840,7 +843,10 @@ class ResidentCalibratedStack:
                   "total_s": call_s, "download_mode": download_mode,
                   "diagnostic_maps_downloaded": download_mode == "full",
                   "weight_map_downloaded": download_mode != "master_only", "avoids_stack_scatter": True,
-                  "modifies_resident_stack": False}
+                  "modifies_resident_stack": False, "inverse_batch_bytes": self.frame_count * 9 * 8,
+                  "weights_bytes": self.frame_count * 4,
+                  "output_bytes": self.pixels_per_frame * 4 * (1 + (download_mode != "master_only")
+                                                               + 4 * (download_mode == "full"))}
         return master, m["weight"], m["coverage"], m["low"], m["high"], m["geometric"], timing
 
     def integrate_hardened_winsorized_sigma(self, weights=None, low_sigma=3.0, high_sigma=3.0, min_samples=3,
