@@ -29,11 +29,17 @@ constexpr int kAdjBlock = 256;
 
 // One axis: normalised weights w[6] and dw/df for fractional offset f.
 // sin(pi d_k) = (-1)^k sin(pi f); sin(pi d_k / 3) = s3 ck - c3 sk with
-// (ck, sk) = (cos, sin)(pi (k-2) / 3); cos likewise.
+// (ck, sk) = (cos, sin)(pi (k-2) / 3); cos likewise.  One sincos of pi f / 3
+// gives sin / cos of pi f by the triple-angle identities, and the divisions
+// are hardware reciprocals (about 1 ulp): these weights feed derivatives and
+// the Gauss-Newton normal equations, pinned by finite differences.
+__device__ __forceinline__ float rcp_approx(float x) { return __builtin_amdgcn_rcpf(x); }
+
 __device__ __forceinline__ void lanczos_weights_derivs(float f, float (&w)[6], float (&dw)[6]) {
-  float s1, c1, s3, c3;
-  sincospif(f, &s1, &c1);
+  float s3, c3;
   sincospif(f * (1.0f / 3.0f), &s3, &c3);
+  const float s1 = s3 * (3.0f - 4.0f * s3 * s3);  // sin(3a) = 3 sin a - 4 sin^3 a
+  const float c1 = c3 * (4.0f * c3 * c3 - 3.0f);  // cos(3a) = 4 cos^3 a - 3 cos a
   constexpr float kHalfRoot3 = 0.86602540378443864676f;
   constexpr float kPi = 3.14159265358979323846f;
   const float ck[6] = {-0.5f, 0.5f, 1.0f, 0.5f, -0.5f, -1.0f};
@@ -51,14 +57,15 @@ __device__ __forceinline__ void lanczos_weights_derivs(float f, float (&w)[6], f
       l[k] = 1.0f;
       dl[k] = -(10.0f / 27.0f) * kPi * kPi * d;
     } else {
-      const float inv = 1.0f / (kPi * kPi * d * d);
+      const float rd = rcp_approx(d);
+      const float inv = (1.0f / (kPi * kPi)) * rd * rd;
       l[k] = 3.0f * sa * sb * inv;
-      dl[k] = 3.0f * inv * (kPi * ca * sb + (kPi / 3.0f) * sa * cb) - 2.0f * l[k] / d;
+      dl[k] = 3.0f * inv * (kPi * ca * sb + (kPi / 3.0f) * sa * cb) - 2.0f * l[k] * rd;
     }
     sum += l[k];
     dsum += dl[k];
   }
-  const float rs = 1.0f / sum;
+  const float rs = rcp_approx(sum);
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
     w[k] = l[k] * rs;
@@ -205,15 +212,20 @@ __global__ __launch_bounds__(kAdjBlock) void normal_equations_kernel(const float
     const double xc = static_cast<double>(x) - cx, yc = static_cast<double>(y) - cy;
     const double j[6] = {gx * xc, gx * yc, static_cast<double>(gx), gy * xc, gy * yc, static_cast<double>(gy)};
     const double r = static_cast<double>(v) - static_cast<double>(r0);
+    // fused multiply-adds: one rounding per term (the sums are pinned by
+    // tolerances, not bits)
     int t = 0;
 #pragma unroll
     for (int a = 0; a < 6; ++a) {
 #pragma unroll
-      for (int b = a; b < 6; ++b) acc[t++] += j[a] * j[b];
+      for (int b = a; b < 6; ++b) {
+        acc[t] = fma(j[a], j[b], acc[t]);
+        ++t;
+      }
     }
 #pragma unroll
-    for (int a = 0; a < 6; ++a) acc[21 + a] += j[a] * r;
-    acc[27] += r * r;
+    for (int a = 0; a < 6; ++a) acc[21 + a] = fma(j[a], r, acc[21 + a]);
+    acc[27] = fma(r, r, acc[27]);
     acc[28] += 1.0;
   }
   block_reduce_store<kNormalSums>(acc, partials);
