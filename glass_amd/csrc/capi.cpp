@@ -391,6 +391,7 @@ int gh_integrate(gh_stack* s, const float* weights, const gh_integrate_params* p
     const float* pad_ptr = n_active ? table.back() : s->slot[0];
     table.resize(padded, pad_ptr);
     w.resize(padded, 0.0f);
+    s->wait_tables();
     s->frame_table.ensure(padded, "hipMalloc(frame table)");
     s->frame_weights.ensure(padded, "hipMalloc(frame weights)");
     check(hipMemcpy(s->frame_table.ptr, table.data(), padded * sizeof(const float*), hipMemcpyHostToDevice),
@@ -471,6 +472,7 @@ int gh_integrate(gh_stack* s, const float* weights, const gh_integrate_params* p
     }
     a.dq_active_frames = o.dq_active_frames;
     gh::launch_integrate(a, st);
+    s->mark_tables(st);
     const void* dev_src[6] = {a.master, a.weight, a.coverage, a.low, a.high, a.dq};
     bool any_host = false;
     for (int i = 0; i < 6; ++i) {

@@ -226,8 +226,7 @@ int gh_integrate_clip(gh_stack* s, const float* weights, const double* inverses,
     const size_t inv_bytes = inv.size() * sizeof(double);
     s->clip_tables.ensure(table_bytes + w_bytes + inv_bytes + 8, "hipMalloc(clip tables)");
     unsigned char* base = s->clip_tables.ptr;
-    // the tables may still be read by a clip integration queued earlier
-    check(hipStreamSynchronize(st), "clip tables");
+    s->wait_tables();
     if (n) {
       check(hipMemcpy(base, table.data(), n * sizeof(const float*), hipMemcpyHostToDevice), "hipMemcpy(clip table)");
       check(hipMemcpy(base + table_bytes, w.data(), n * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy(clip weights)");
@@ -270,6 +269,7 @@ int gh_integrate_clip(gh_stack* s, const float* weights, const double* inverses,
     a.geometric = dev[5];
     const int sampler = prm.sampler == GH_CLIP_STACK ? 0 : (prm.sampler == GH_INTERP_BILINEAR ? 1 : 2);
     gh::launch_clip_integrate(a, sampler, st);
+    s->mark_tables(st);
     bool any_host = false;
     for (int i = 0; i < 6; ++i) {
       if (!user[i] || dev[i] == user[i]) continue;

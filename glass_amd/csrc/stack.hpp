@@ -182,6 +182,20 @@ struct gh_stack {
     if (caller_done) check(hipStreamWaitEvent(stream, caller_done, 0), "hipStreamWaitEvent(caller stream)");
   }
 
+  // Device tables that the host rewrites before each integration launch
+  // (frame pointers, weights, inverses, tile flags): a new upload first waits
+  // for the launch that last read them, whichever stream it ran on.
+  hipEvent_t tables_done = nullptr;
+
+  void wait_tables() {
+    if (tables_done) check(hipEventSynchronize(tables_done), "integration tables in use");
+  }
+
+  void mark_tables(hipStream_t st) {
+    if (!tables_done) check(hipEventCreateWithFlags(&tables_done, hipEventDisableTiming), "hipEventCreate");
+    check(hipEventRecord(tables_done, st), "hipEventRecord(integration tables)");
+  }
+
   // Host raw frames go through a ring of device slots filled on a copy
   // stream: the upload of frame k + 1 overlaps the calibration of frame k.
   // Slot i is free again once the calibration that read it has run
@@ -281,6 +295,7 @@ struct gh_stack {
     }
     if (copy_stream) (void)hipStreamDestroy(copy_stream);
     if (caller_done) (void)hipEventDestroy(caller_done);
+    if (tables_done) (void)hipEventDestroy(tables_done);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
