@@ -148,3 +148,29 @@ def test_clip_argument_errors(gpu):
         s.upload_calibrated_frame(0, x[0])
         with pytest.raises(RuntimeError, match="must be loaded"):
             s.integrate_sigma_clip()
+
+
+def test_integrations_on_two_streams_keep_their_tables(gpu):
+    # Two asynchronous integrations with different frame sets on two caller
+    # streams, queued back to back: the second upload of the frame table must
+    # not overwrite the one the first launch is still reading.
+    import torch
+
+    f, h, w = 40, 256, 320
+    x = _frames(f, h, w, seed=17, nan_frac=0.0)
+    wa = np.ones(f, np.float32)
+    wb = np.ones(f, np.float32)
+    wb[::2] = 0.0
+    sa, sb = torch.cuda.Stream(), torch.cuda.Stream()
+    with _stack_of(x) as s:
+        exp_a = s.integrate_hardened_winsorized_sigma(wa)[0]
+        exp_b = s.integrate_hardened_winsorized_sigma(wb)[0]
+        outs = [{"master": torch.empty((h, w), dtype=torch.float32, device="cuda")} for _ in range(2)]
+        torch.cuda.synchronize()
+        for _ in range(3):
+            s.integrate_to_device(outs[0], wa, dq_geometric=False, stream=sa.cuda_stream)
+            s.integrate_to_device(outs[1], wb, dq_geometric=False, stream=sb.cuda_stream)
+        torch.cuda.synchronize()
+        got_a, got_b = outs[0]["master"].cpu().numpy(), outs[1]["master"].cpu().numpy()
+    assert np.array_equal(got_a, exp_a)
+    assert np.array_equal(got_b, exp_b)
