@@ -243,6 +243,10 @@ int gh_integrate_clip(gh_stack* s, const float* weights, const double* inverses,
     a.height = static_cast<int>(s->height);
     a.npix = s->npix;
     a.clamp = prm.clamping_threshold;
+    // affine inverses skip the projective divide (x / 1.0 is x exactly)
+    a.affine = 1;
+    for (size_t k = 0; k < n && !inv.empty(); ++k)
+      if (!(inv[k * 9 + 6] == 0.0 && inv[k * 9 + 7] == 0.0 && inv[k * 9 + 8] == 1.0)) a.affine = 0;
     a.mode = prm.rejection == GH_REJECT_SIGMA_CLIP ? 1 : 0;
     a.winsorize = prm.winsorize ? 1 : 0;
     a.low_sigma = prm.low_sigma;

@@ -201,6 +201,7 @@ struct ClipArgs {
   int n, width, height;
   long long npix;
   float clamp;                 // Lanczos3 clamp (< 0: off)
+  int affine;                  // every inverse has last row (0, 0, 1): no projective divide
   int mode;                    // 0 weighted mean, 1 sigma clip
   int winsorize;
   float low_sigma, high_sigma;

@@ -71,7 +71,8 @@ __device__ __forceinline__ void lanczos_axis(float f, float thr, float (&w)[6]) 
   for (int i = 0; i < 6; ++i) {
     const float d = f - static_cast<float>(i - 2);
     const float num = sgn[i] * sf * (s3 * ck[i] - c3 * sk[i]);
-    w[i] = (fabsf(d) < thr) ? 1.0f : kScale * num / (d * d);
+    // hardware reciprocal (about 1 ulp) instead of a correctly rounded divide
+    w[i] = (fabsf(d) < thr) ? 1.0f : (kScale * num) * __builtin_amdgcn_rcpf(d * d);
   }
 }
 
@@ -794,7 +795,7 @@ __device__ __forceinline__ float clip_sample(const ClipArgs& a, int k, int x, in
   }
   const double* m = a.inverses + 9 * k;
   double sx, sy;
-  footprint = source_coords(m, 0, x, y, sx, sy) && isfinite(sx) && isfinite(sy);
+  footprint = source_coords(m, a.affine, x, y, sx, sy) && isfinite(sx) && isfinite(sy);
   if (SAMPLER == 1) {
     footprint = footprint && sx >= 0.0 && sx <= static_cast<double>(a.width - 1) && sy >= 0.0 &&
                 sy <= static_cast<double>(a.height - 1);
@@ -827,7 +828,8 @@ __device__ __forceinline__ float clip_sample(const ClipArgs& a, int k, int x, in
 // 1 LDS (n slots per thread in dynamic shared memory, 128-thread blocks up to
 // 128 frames and 64-thread blocks up to 256, at most 64 KB per block: the
 // warped samplers' per-frame code is too large to unroll CAP times), 2 none
-// (every pass samples again).
+// (every pass samples again; the single-pass weighted mean always, and the
+// sigma clip where LDS occupancy would cost more than sampling again).
 
 template <int SAMPLER, int CAP, int STORE>
 __global__ __launch_bounds__(256) void clip_integrate_kernel(ClipArgs a) {
@@ -1078,8 +1080,14 @@ void launch_clip_integrate(const ClipArgs& a, int sampler, hipStream_t stream) {
     else if (a.n <= 64) clip_integrate_kernel<0, 64, 0><<<grid256, 256, 0, stream>>>(a);
     else if (a.n <= 128) clip_integrate_kernel<0, 128, 0><<<grid256, 256, 0, stream>>>(a);
     else GH_CLIP_AGAIN(0);
+  } else if (a.mode == 0) {
+    // the weighted mean is a single pass: nothing to store
+    if (sampler == 1) GH_CLIP_AGAIN(1);
+    else GH_CLIP_AGAIN(2);
   } else if (sampler == 1) {
-    if (a.n <= 256) GH_CLIP_LDS(1);
+    // bilinear samples are cheap to take again (4 taps); past 128 frames the
+    // 64-thread LDS blocks lose more to occupancy than re-sampling costs
+    if (a.n <= 128) GH_CLIP_LDS(1);
     else GH_CLIP_AGAIN(1);
   } else {
     if (a.n <= 256) GH_CLIP_LDS(2);

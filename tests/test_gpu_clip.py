@@ -99,13 +99,14 @@ def test_matrix_warped_bilinear_bit_exact(gpu, f, rejection):
     assert np.array_equal(after, x, equal_nan=True)  # the stack is not modified
 
 
-@pytest.mark.parametrize("clamp", [0.3, -1.0])
-def test_matrix_warped_lanczos3(gpu, clamp):
-    f, h, w = 6, 40, 44
+@pytest.mark.parametrize("f,clamp", [(6, 0.3), (6, -1.0), (140, 0.3)])
+def test_matrix_warped_lanczos3(gpu, f, clamp):
+    # 6 frames: 128-thread LDS store; 140: 64-thread LDS store
+    h, w = 40, 44
     x = _frames(f, h, w, seed=77)
     x[2, 10:14, 12:15] = np.nan  # non-finite taps are skipped
     mats = _mats(f, 5)
-    weights = np.array([1, 0.5, 2, 1, 0, 1], np.float32)
+    weights = np.tile(np.array([1, 0.5, 2, 1, 0, 1], np.float32), f // 6 + 1)[:f]
     active = [k for k in range(f) if weights[k] > 0]
     with _stack_of(x) as s:
         master, weight, coverage, low, high, geometric, _ = s.integrate_matrix_warped_sigma_clip(
