@@ -53,6 +53,7 @@ def parse_args():
     p.add_argument("--batch-reps", type=int, default=3)
     p.add_argument("--no-local-norm", action="store_true", help="skip the local-normalisation leg")
     p.add_argument("--no-host-raw", action="store_true", help="skip the host-raw (PCIe-inclusive) leg")
+    p.add_argument("--no-registration", action="store_true", help="skip the star-registration leg")
     p.add_argument("--fused-leg", action="store_true",
                    help="also time the fused matrix-warped integrations on the calibrated stack")
     p.add_argument("--local-norm-tile", type=int, default=512)
@@ -358,6 +359,45 @@ _TRAFFIC_KERNELS = {"calibrate": "gh::calibrate_batch_kernel<2, false, 16>", "wa
                     "integrate": "gh::integrate_winsorized_kernel<208, true>"}
 
 
+def registration_leg(args, stack, light_raw, a, frames=9):
+    """Star detection (threshold 5 sigma, 500 stars) on the calibrated, not
+    yet warped first `frames` lights, then the affine star transform of each
+    moving frame against frame 0, compared with the generator's matrices
+    (largest corner displacement over the frame).  Wall clock per frame for
+    detection (on the resident frame, synchronous) and for the transform."""
+    import numpy as np
+    import torch
+
+    from glass_amd.registration import estimate_star_transform
+
+    f = min(frames, stack.frame_count)
+    idx = list(range(f))
+    stack.calibrate_frames_device(idx, light_raw, "fits_u16be_bzero", [a["light_exp"]] * f, a["dark_exp"],
+                                  a["policy"])
+    torch.cuda.synchronize()
+    stack.detect_stars(0)  # first call sizes the candidate buffers
+    t = time.perf_counter()
+    stars = [stack.detect_stars(i, 5.0, 500) for i in idx]
+    detect_s = (time.perf_counter() - t) / f
+    h, w = stack.height, stack.width
+    corners = np.array([[0, 0, 1], [w - 1, 0, 1], [0, h - 1, 1], [w - 1, h - 1, 1]], np.float64).T
+    errs = []
+    t = time.perf_counter()
+    regs = [estimate_star_transform(stars[0], stars[i], "affine") for i in idx[1:]]
+    fit_s = (time.perf_counter() - t) / max(1, f - 1)
+    for i, reg in zip(idx[1:], regs):
+        got = np.asarray(reg.matrix, np.float64).reshape(3, 3) @ corners
+        exp = np.asarray(a["mats"][i], np.float64) @ corners
+        errs.append(float(np.max(np.hypot(got[0] - exp[0], got[1] - exp[1]))))
+    out = {"frames": f, "detect_ms_per_frame": round(detect_s * 1e3, 2), "fit_ms_per_frame": round(fit_s * 1e3, 2),
+           "stars_per_frame": round(float(np.mean([len(x) for x in stars])), 1),
+           "matched_min": int(min(r.matched_stars for r in regs)) if regs else 0,
+           "corner_err_px_max": round(max(errs), 4) if errs else None,
+           "corner_err_px_median": round(float(np.median(errs)), 4) if errs else None}
+    log(f"registration: {out}")
+    return out
+
+
 def fused_leg(args, stack, light_raw, a):
     """integrate_matrix_warped_mean / _sigma_clip over the calibrated,
     unwarped stack (every frame resampled inside the integration, no warped
@@ -660,6 +700,9 @@ def main():
             "avg_launch_ms": round(avg_launch_ms, 4),
         },
     }
+    if not args.no_registration:
+        log("star-registration leg: detect stars and fit affine matrices on calibrated, unwarped lights")
+        line["registration"] = registration_leg(args, stack, light_raw, step_args)
     if args.fused_leg:
         log("fused matrix-warped integrations on the calibrated (unwarped) stack")
         line["fused_matrix_warped"] = fused_leg(args, stack, light_raw, step_args)
