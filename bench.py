@@ -13,7 +13,8 @@ frames from raw (f32 stack, 49.3 GB), warp the 199 non-reference frames,
 integrate the 193 positive-weight frames into the output maps.
 
 Prints one JSON line (rank 0).  --gpus > 1 is refused: the path runs on one
-GPU; see DESIGN.md for how it shards.
+GPU (it would shard by output row bands, each GPU receiving its band plus the
+warp halo of every raw frame).
 """
 
 from __future__ import annotations
@@ -543,8 +544,8 @@ def measured_traffic(args, stage):
 def main():
     args = parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs the resident stacking path on one GPU; "
-                                   "--gpus > 1 is not supported (see DESIGN.md for the sharding design)"}))
+        print(json.dumps({"error": "the resident stacking path runs on one GPU; --gpus > 1 is not supported "
+                                   "(more GPUs would shard by output row bands with a warp halo)"}))
         sys.exit(2)
     import numpy as np
     import torch
