@@ -25,6 +25,7 @@ GH_FMT_FITS_U16BE_BZERO = 2
 GH_INTERP_BILINEAR = 1
 GH_INTERP_LANCZOS3 = 3
 
+GH_TRANSLATE_INTEGER = 0
 GH_WARP_TRACK_COVERAGE = 1
 GH_WARP_SKIP_NONFINITE_TAPS = 2
 GH_WARP_AFFINE = 4
@@ -172,6 +173,7 @@ _SIGNATURES = {
         c_void_p, POINTER(c_float), POINTER(c_double), POINTER(ClipParams), c_void_p, c_void_p, c_void_p, c_void_p,
         c_void_p, c_void_p, c_void_p,
     ],
+    "gh_translate_frame": [c_void_p, c_int64, c_int32, c_double, c_double, c_float, c_void_p],
     "gh_frames_stats": [c_void_p, POINTER(c_int64), c_int64, POINTER(FrameStatistics)],
     "gh_pair_grid_stats": [
         c_void_p, c_int64, POINTER(c_int64), c_int64, c_int32, c_int32, POINTER(c_float), POINTER(c_float),

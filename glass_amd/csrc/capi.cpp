@@ -309,6 +309,37 @@ int gh_warp_frames(gh_stack* s, const int64_t* indices, int64_t count, const dou
   });
 }
 
+int gh_translate_frame(gh_stack* s, int64_t index, int32_t mode, double dx, double dy, float fill,
+                       void* stream_ptr) {
+  return guarded([&] {
+    s->require_loaded(index, "translation warp");
+    if (mode != GH_TRANSLATE_INTEGER && mode != GH_INTERP_BILINEAR)
+      throw gh::ValueError("translation mode must be integer or bilinear");
+    if (!std::isfinite(dx) || !std::isfinite(dy)) throw gh::ValueError("translation must be finite");
+    int idx = 0, idy = 0;
+    if (mode == GH_TRANSLATE_INTEGER) {
+      if (dx != std::floor(dx) || dy != std::floor(dy) || std::fabs(dx) > 2.0e9 || std::fabs(dy) > 2.0e9)
+        throw gh::ValueError("integer translation needs whole-pixel offsets");
+      idx = static_cast<int>(dx);
+      idy = static_cast<int>(dy);
+    }
+    hipStream_t st = s->pick(stream_ptr);
+    if (st == s->stream) s->wait_caller();
+    s->ensure_geometric();
+    s->ensure_spares(1);
+    unsigned* flag = s->nonfinite + index;
+    gh::launch_fill(reinterpret_cast<float*>(flag), 1, 0.0f, st);  // all-zero bits: flag cleared
+    gh::launch_translate(s->slot[index], s->spare[0], static_cast<int>(s->width), static_cast<int>(s->height),
+                         mode == GH_INTERP_BILINEAR, idx, idy, static_cast<float>(dx), static_cast<float>(dy), fill,
+                         s->geometric, flag, st);
+    std::swap(s->slot[index], s->spare[0]);
+    s->calib_stats_valid[index] = 0;
+    s->warp_coverage_frames += 1;
+    s->mark_caller(st);
+    if (!stream_ptr) check(hipStreamSynchronize(st), "translation warp");
+  });
+}
+
 int gh_reset_warp_coverage(gh_stack* s) {
   return guarded([&] {
     s->ensure_geometric();

@@ -192,6 +192,11 @@ struct VariantJob {
 };
 void launch_decide_variants(const VariantJob* jobs, int n_jobs, int* out, hipStream_t stream);
 
+// Integer or float32-bilinear translation of one frame into dst (fill
+// outside); coverage (nullable) += 1 inside; *flag |= any non-finite output.
+void launch_translate(const float* src, float* dst, int width, int height, bool bilinear, int idx, int idy, float fdx,
+                      float fdy, float fill, float* coverage, unsigned* flag, hipStream_t stream);
+
 // Mean / sigma-clip integration that samples the frames itself (resident
 // stack as stored, or warped on the fly through FP64 inverses).
 struct ClipArgs {

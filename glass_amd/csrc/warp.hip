@@ -973,6 +973,50 @@ __global__ __launch_bounds__(256) void clip_integrate_kernel(ClipArgs a) {
   if (a.geometric) st_global(a.geometric, p, geometric);
 }
 
+// ------------------------------------------------------------------------
+// Translations (cpp/cuda/warp_kernels.cu:103-190): the integer form copies
+// in[y - dy][x - dx] (fill outside); the bilinear form samples at
+// (x - dx, y - dy) computed in float32, valid 0 <= s <= dim - 1, the far
+// neighbour clamped to the edge.  Out of place (the host swaps slots); the
+// geometric coverage accumulator gains 1 where the pixel is inside.
+template <bool BILINEAR>
+__global__ __launch_bounds__(256) void translate_kernel(const float* src, float* dst, int width, int height, int idx,
+                                                        int idy, float fdx, float fdy, float fill, float* coverage,
+                                                        unsigned* flag) {
+  bool bad = false;
+  const long long n = static_cast<long long>(width) * height;
+  for (long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * 256) {
+    const int y = static_cast<int>(i / width), x = static_cast<int>(i - static_cast<long long>(y) * width);
+    float v = fill;
+    bool inside;
+    if (!BILINEAR) {
+      const int sx = x - idx, sy = y - idy;
+      inside = sx >= 0 && sx < width && sy >= 0 && sy < height;
+      if (inside) v = ld_global(src, static_cast<long long>(sy) * width + sx);
+    } else {
+      const float sx = static_cast<float>(x) - fdx, sy = static_cast<float>(y) - fdy;
+      inside = !(sx < 0.0f || sx > static_cast<float>(width - 1) || sy < 0.0f || sy > static_cast<float>(height - 1));
+      if (inside) {
+        const int x0 = static_cast<int>(floorf(sx)), y0 = static_cast<int>(floorf(sy));
+        const int x1 = x0 + 1 < width ? x0 + 1 : x0, y1 = y0 + 1 < height ? y0 + 1 : y0;
+        const float tx = sx - static_cast<float>(x0), ty = sy - static_cast<float>(y0);
+        const float a00 = ld_global(src, static_cast<long long>(y0) * width + x0);
+        const float a01 = ld_global(src, static_cast<long long>(y0) * width + x1);
+        const float a10 = ld_global(src, static_cast<long long>(y1) * width + x0);
+        const float a11 = ld_global(src, static_cast<long long>(y1) * width + x1);
+        const float top = a00 * (1.0f - tx) + a01 * tx;
+        const float bottom = a10 * (1.0f - tx) + a11 * tx;
+        v = top * (1.0f - ty) + bottom * ty;
+      }
+    }
+    st_global(dst, i, v);
+    bad |= !isfinite(v);
+    if (coverage && inside) st_global(coverage, i, ld_global(coverage, i) + 1.0f);
+  }
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
+}
+
 }  // namespace
 
 void make_lanczos_lut(float* table) {
@@ -1096,6 +1140,18 @@ void launch_clip_integrate(const ClipArgs& a, int sampler, hipStream_t stream) {
 #undef GH_CLIP_LDS
 #undef GH_CLIP_AGAIN
   check(hipGetLastError(), "clip integrate launch");
+}
+
+void launch_translate(const float* src, float* dst, int width, int height, bool bilinear, int idx, int idy, float fdx,
+                      float fdy, float fill, float* coverage, unsigned* flag, hipStream_t stream) {
+  const long long n = static_cast<long long>(width) * height;
+  const unsigned grid = static_cast<unsigned>(std::min<long long>((n + 255) / 256, 8192));
+  if (bilinear)
+    translate_kernel<true><<<grid, 256, 0, stream>>>(src, dst, width, height, idx, idy, fdx, fdy, fill, coverage, flag);
+  else
+    translate_kernel<false><<<grid, 256, 0, stream>>>(src, dst, width, height, idx, idy, fdx, fdy, fill, coverage,
+                                                      flag);
+  check(hipGetLastError(), "translation launch");
 }
 
 }  // namespace gh

@@ -457,6 +457,22 @@ class ResidentCalibratedStack:
         interp = N.GH_INTERP_LANCZOS3 if interpolation == "lanczos3" else N.GH_INTERP_BILINEAR
         return self._warp(indices, matrices, interp, fill, -1.0, track_coverage, False, stream)
 
+    def apply_translation_frame(self, index: int, dx: int, dy: int, fill: float = math.nan,
+                                stream: int | None = None) -> None:
+        """apply_translation_frame (native_bindings.cpp:8778): whole-pixel shift
+        in place, `fill` outside, geometric coverage accumulated."""
+        if int(dx) != dx or int(dy) != dy:
+            raise TypeError("apply_translation_frame takes integer offsets")
+        N.call("gh_translate_frame", self._h, int(index), N.GH_TRANSLATE_INTEGER, float(int(dx)), float(int(dy)),
+               float(fill), ctypes.c_void_p(stream) if stream else None)
+
+    def apply_translation_bilinear_frame(self, index: int, dx: float, dy: float, fill: float = math.nan,
+                                         stream: int | None = None) -> None:
+        """apply_translation_bilinear_frame (native_bindings.cpp:8808): bilinear
+        sample at (x - dx, y - dy) in float32, in place."""
+        N.call("gh_translate_frame", self._h, int(index), N.GH_INTERP_BILINEAR, float(np.float32(dx)),
+               float(np.float32(dy)), float(fill), ctypes.c_void_p(stream) if stream else None)
+
     def reset_warp_coverage(self) -> None:
         N.call("gh_reset_warp_coverage", self._h)
 

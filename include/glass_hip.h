@@ -50,6 +50,7 @@ typedef struct gh_stack gh_stack;
 #define GH_INTERP_LANCZOS3 3
 
 /* Warp flags. */
+#define GH_TRANSLATE_INTEGER 0
 #define GH_WARP_TRACK_COVERAGE 1u     /* add per-pixel footprint to the geometric coverage map */
 #define GH_WARP_SKIP_NONFINITE_TAPS 2u /* resident semantics: non-finite Lanczos taps skipped */
 #define GH_WARP_AFFINE 4u             /* last inverse row is exactly (0,0,1): no projective divide */
@@ -179,6 +180,17 @@ int gh_warp_frames(gh_stack* stack, const int64_t* indices, int64_t count, const
 
 /* reset_warp_coverage / accumulate_full_warp_coverage_frame / warp_coverage_map
  * (native_bindings.cpp:3051-3080).  `out` host or device, [H*W] f32. */
+/* apply_translation_frame(i, dx, dy, fill) (mode GH_TRANSLATE_INTEGER, whole
+ * pixels) and apply_translation_bilinear_frame(i, dx, dy, fill) (mode
+ * GH_INTERP_BILINEAR, dx / dy rounded to float32) (native_bindings.cpp:
+ * 8778-8833; kernels cpp/cuda/warp_kernels.cu:103-190): the frame becomes, in
+ * place, in[y - dy][x - dx] (integer) or its bilinear sample at
+ * (x - dx, y - dy) computed in float32 (valid 0 <= s <= dim - 1, far
+ * neighbour clamped), `fill` outside; the geometric coverage map gains 1 where
+ * the pixel is inside and counts one more warped frame. */
+int gh_translate_frame(gh_stack* stack, int64_t index, int32_t mode, double dx, double dy, float fill,
+                       void* stream);
+
 int gh_reset_warp_coverage(gh_stack* stack);
 int gh_accumulate_full_warp_coverage(gh_stack* stack);
 int gh_warp_coverage_map(gh_stack* stack, float* out);

@@ -212,3 +212,46 @@ def warp_lanczos3_source_map(image, source_map, fill=0.0, dtype=np.float64):
                 wsum += wgt
         out[ok] = acc / wsum
     return out.astype(dtype)
+
+
+def translate_integer(image, dx: int, dy: int, fill=np.nan):
+    """Whole-pixel translation (cpp/cuda/warp_kernels.cu:103-127):
+    out[y, x] = in[y - dy, x - dx] inside the frame, `fill` elsewhere.
+    Returns (out, coverage 0/1)."""
+    img = np.asarray(image, dtype=np.float32)
+    h, w = img.shape
+    out = np.full((h, w), fill, dtype=np.float32)
+    cov = np.zeros((h, w), dtype=np.float32)
+    ys, xs = np.indices((h, w))
+    sx, sy = xs - int(dx), ys - int(dy)
+    ok = (sx >= 0) & (sx < w) & (sy >= 0) & (sy < h)
+    out[ok] = img[sy[ok], sx[ok]]
+    cov[ok] = 1.0
+    return out, cov
+
+
+def translate_bilinear_f32(image, dx: float, dy: float, fill=np.nan):
+    """Bilinear translation with float32 coordinates
+    (cpp/cuda/warp_kernels.cu:147-176): s = float32(x) - float32(dx), valid
+    0 <= s <= dim - 1, far neighbour clamped to the edge, lerps in float32.
+    Returns (out, coverage 0/1)."""
+    img = np.asarray(image, dtype=np.float32)
+    h, w = img.shape
+    f32 = np.float32
+    ys, xs = np.indices((h, w))
+    sx = (xs.astype(f32) - f32(dx)).astype(f32)
+    sy = (ys.astype(f32) - f32(dy)).astype(f32)
+    ok = ~((sx < 0) | (sx > f32(w - 1)) | (sy < 0) | (sy > f32(h - 1)))
+    out = np.full((h, w), fill, dtype=f32)
+    cov = ok.astype(f32)
+    x0 = np.floor(sx[ok]).astype(np.int64)
+    y0 = np.floor(sy[ok]).astype(np.int64)
+    x1 = np.where(x0 + 1 < w, x0 + 1, x0)
+    y1 = np.where(y0 + 1 < h, y0 + 1, y0)
+    tx = (sx[ok] - x0.astype(f32)).astype(f32)
+    ty = (sy[ok] - y0.astype(f32)).astype(f32)
+    one = f32(1)
+    top = (img[y0, x0] * (one - tx) + img[y0, x1] * tx).astype(f32)
+    bottom = (img[y1, x0] * (one - tx) + img[y1, x1] * tx).astype(f32)
+    out[ok] = (top * (one - ty) + bottom * ty).astype(f32)
+    return out, cov

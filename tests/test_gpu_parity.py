@@ -609,3 +609,47 @@ def test_end_to_end_config1_stagewise(gpu):
     exp = oracle.weighted_integrate_stack(warped, weights=weights, rejection="winsorized_sigma")
     _compare(got[:5], exp)
     assert got[3].sum() + got[4].sum() > 0  # cosmic rays / trails are rejected
+
+
+# ------------------------------------------------------------ translations
+def test_translation_warps_bit_exact(gpu):
+    from oracle.warp import translate_bilinear_f32, translate_integer
+
+    rng = np.random.default_rng(6)
+    h, w = 37, 53
+    frames = rng.normal(100, 10, (3, h, w)).astype(np.float32)
+    frames[1, 5, 7] = np.nan
+    with _stack_of(frames) as s:
+        s.reset_warp_coverage()
+        s.apply_translation_frame(0, 3, -2)
+        s.apply_translation_bilinear_frame(1, 1.3, 0.45, fill=-1.0)
+        s.apply_translation_bilinear_frame(2, -2.0, 1.0)
+        got = s.download_frames_tile([0, 1, 2], 0, 0, w, h)
+        cov = s.warp_coverage_map()
+        assert s.warp_coverage_frame_count == 3
+        flags = [s.frame_has_nonfinite(i) for i in range(3)]
+    e0, c0 = translate_integer(frames[0], 3, -2)
+    e1, c1 = translate_bilinear_f32(frames[1], 1.3, 0.45, fill=-1.0)
+    e2, c2 = translate_bilinear_f32(frames[2], -2.0, 1.0)
+    assert np.array_equal(got[0], e0, equal_nan=True)
+    assert np.array_equal(got[1], e1, equal_nan=True)
+    assert np.array_equal(got[2], e2, equal_nan=True)
+    assert np.array_equal(cov, c0 + c1 + c2)
+    assert flags == [True, True, True]  # NaN fill / NaN source reach the output
+
+
+def test_reference_fixture_translation_uses_nan_coverage(gpu):
+    # tests/test_cuda_resident_stack.py:1481-1494
+    from oracle.warp import translate_integer
+
+    frame = np.arange(20, dtype=np.float32).reshape(4, 5)
+    with _stack_of([frame]) as s:
+        s.apply_translation_frame(0, 1, -1, np.nan)
+        master, weight_map = s.integrate_mean()
+        with pytest.raises(TypeError):
+            s.apply_translation_frame(0, 0.5, 0)
+    expected, _ = translate_integer(frame, 1, -1)
+    valid = np.isfinite(expected)
+    assert np.allclose(master[valid], expected[valid], rtol=1e-5, atol=1e-5)
+    assert np.all(master[~valid] == 0.0)
+    assert np.allclose(weight_map, valid.astype(np.float32))

@@ -385,3 +385,33 @@ def test_resident_dq_rules():
     assert dq[0, 2] == oracle.DQ_NO_DATA
     assert dq[0, 3] == oracle.DQ_NO_DATA | oracle.DQ_WARP_EDGE
     assert dq[0, 4] == oracle.DQ_WARP_EDGE | oracle.DQ_HIGH_REJECTED
+
+
+def test_translation_oracle_closed_forms():
+    from oracle.warp import translate_bilinear_f32, translate_integer
+
+    img = np.arange(20, dtype=np.float32).reshape(4, 5)
+    out, cov = translate_integer(img, 1, -1)
+    # out[y, x] = img[y + 1, x - 1]: column 0 and the last row are outside
+    assert np.array_equal(out[:3, 1:], img[1:, :4])
+    assert np.all(np.isnan(out[:, 0])) and np.all(np.isnan(out[3, :]))
+    assert cov.sum() == 12
+    # whole-pixel bilinear equals the integer shift on finite data
+    b, bcov = translate_bilinear_f32(img, 1.0, -1.0)
+    assert np.array_equal(b, out, equal_nan=True) and np.array_equal(bcov, cov)
+    # a linear ramp is reproduced exactly at half-pixel offsets (exact in f32)
+    yy, xx = np.indices((6, 9), dtype=np.float32)
+    ramp = 4 * xx + 2 * yy + 1
+    r, rcov = translate_bilinear_f32(ramp, 0.5, -1.5)
+    ok = rcov > 0
+    exp = 4 * (xx - 0.5) + 2 * (yy + 1.5) + 1
+    assert np.array_equal(r[ok], exp[ok])
+    # the sample position is computed in float32: far from the origin the
+    # fraction is the float32 one, not the exact one
+    wide = np.zeros((1, 20000), np.float32)
+    wide[0, 16385] = 1.0
+    w32, _ = translate_bilinear_f32(wide, 0.3, 0.0)
+    sx = np.float32(16385) - np.float32(0.3)
+    # output 16385 samples between 16384 (0) and 16385 (1): the value is the
+    # float32 fraction, 0.69921875 there, not 0.7
+    assert w32[0, 16385] == sx - np.float32(16384) == np.float32(0.69921875)
