@@ -107,9 +107,15 @@ class ResidentCalibratedStack:
         self._height = int(height)
         self._width = int(width)
         self._has = {"bias": False, "dark": False, "flat": False}
+        self._pending = []       # queued asynchronous host calibrations
+        self._fits_queue = None  # FITS completion queue between begin and finish
 
     # ------------------------------------------------------------ lifecycle
     def close(self) -> None:
+        if getattr(self, "_fits_queue", None) is not None:
+            self.cancel_fits_u16be_bzero_paths_completion_queue_read()
+        if getattr(self, "_pending", None):
+            self._finish()
         if getattr(self, "_h", None) is not None and self._h.value:
             N.call("gh_stack_destroy", self._h)
             self._h = None
@@ -302,6 +308,162 @@ class ResidentCalibratedStack:
         stream.synchronize()
         return {"schema_version": 1, "frame_count": int(idx.size), "stream_count": int(stream_count),
                 "wave_frames": int(wave_frames), "total_s": time.perf_counter() - start}
+
+    # ---- asynchronous / multistream forms of the reference's calibration API
+    def _queue_host(self, indices, lights, light_exposures, dark_exposures, policy, raw_format):
+        from .staging import calibrate_host_frames
+
+        idx = [int(i) for i in _index_array(indices)]
+        if len(lights) != len(idx):
+            raise ValueError("lights must match indices")
+        t0 = time.perf_counter()
+        keep, stream = calibrate_host_frames(self, idx, lights, list(light_exposures), dark_exposures, policy,
+                                             raw_format)
+        self._pending.append((keep, stream, idx, t0))
+        return idx, t0
+
+    def _finish(self) -> dict:
+        t = time.perf_counter()
+        frames = 0
+        t0 = t
+        for keep, stream, idx, start in self._pending:
+            stream.synchronize()
+            frames += len(idx)
+            t0 = min(t0, start)
+        pending = bool(self._pending)
+        self._pending = []
+        now = time.perf_counter()
+        return {"schema_version": 1, "pending": pending, "frame_count": frames, "wait_sync_s": now - t,
+                "total_s": now - t0 if pending else 0.0}
+
+    def calibrate_frame_pinned_async(self, index, light, light_exposure_s, dark_exposure_s=None, policy=None):
+        """calibrate_frame_pinned_async (native_bindings.cpp:6013): queued;
+        finish_pending_calibration_timed() waits for it."""
+        self._queue_host([index], [light], [light_exposure_s], dark_exposure_s, policy, "float32")
+
+    def calibrate_frame_pinned_async_timed(self, index, light, light_exposure_s, dark_exposure_s=None, policy=None):
+        _, t0 = self._queue_host([index], [light], [light_exposure_s], dark_exposure_s, policy, "float32")
+        return {"schema_version": 1, "mode": "pinned_async", "enqueue_s": time.perf_counter() - t0}
+
+    def calibrate_frame_host_async_timed(self, index, light, light_exposure_s, dark_exposure_s=None, policy=None):
+        """calibrate_frame_host_async_timed (native_bindings.cpp:6047)."""
+        _, t0 = self._queue_host([index], [light], [light_exposure_s], dark_exposure_s, policy, "float32")
+        return {"schema_version": 1, "mode": "host_async", "enqueue_s": time.perf_counter() - t0}
+
+    def calibrate_frames_host_async_timed(self, indices, lights, light_exposures, dark_exposures, policy=None):
+        """calibrate_frames_host_async_timed (native_bindings.cpp:6065):
+        every frame queued, then waited for."""
+        idx, t0 = self._queue_host(indices, lights, light_exposures, dark_exposures, policy, "float32")
+        out = self._finish()
+        out.update(mode="host_async_batch", frame_count=len(idx), total_s=time.perf_counter() - t0)
+        return out
+
+    def calibrate_frames_host_async_multistream_timed(self, indices, lights, light_exposures, dark_exposures,
+                                                      stream_count, policy=None):
+        """calibrate_frames_host_async_multistream_timed (native_bindings.cpp:6179):
+        the uploads already overlap calibration through the library's ring,
+        so `stream_count` does not change the result."""
+        out = self.calibrate_frames_host_async_timed(indices, lights, light_exposures, dark_exposures, policy)
+        out.update(mode="host_async_multistream", stream_count=int(stream_count))
+        return out
+
+    def calibrate_frames_host_async_multistream_h2d_release_timed(self, indices, lights, light_exposures,
+                                                                  dark_exposures, stream_count, policy=None):
+        """Queued like calibrate_frames_host_async_multistream_timed; returns
+        once the work is queued (the host buffers are held until
+        finish_pending_calibration_timed())."""
+        idx, t0 = self._queue_host(indices, lights, light_exposures, dark_exposures, policy, "float32")
+        return {"schema_version": 1, "mode": "host_async_multistream_h2d_release", "frame_count": len(idx),
+                "stream_count": int(stream_count), "enqueue_s": time.perf_counter() - t0}
+
+    def calibrate_frames_host_async_multistream_callback_release_timed(
+            self, indices, lights, light_exposures, dark_exposures, stream_count, wave_frames, release_callback,
+            policy=None):
+        """float32 form of the FITS callback-release entry: release_callback(k)
+        runs, in order, once frame k's host buffer has been consumed."""
+        import torch
+
+        idx = _index_array(indices)
+        start = time.perf_counter()
+        stream = torch.cuda.Stream()
+        darks = dark_exposures if isinstance(dark_exposures, (list, tuple, np.ndarray)) else [dark_exposures] * len(idx)
+        pending = []
+        for k, index in enumerate(idx):
+            ptr, ref = _buffer(lights[k], np.float32, (self.height, self.width), "light")
+            self._calibrate([int(index)], ptr, self.pixels_per_frame * 4, N.GH_FMT_F32, [light_exposures[k]],
+                            [darks[k]], policy, False, stream.cuda_stream)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+            pending.append((k, ev, ref))
+            while pending and pending[0][1].query():
+                done = pending.pop(0)
+                if release_callback is not None:
+                    release_callback(int(done[0]))
+        for k, ev, _ in pending:
+            ev.synchronize()
+            if release_callback is not None:
+                release_callback(int(k))
+        stream.synchronize()
+        return {"schema_version": 1, "frame_count": int(idx.size), "stream_count": int(stream_count),
+                "wave_frames": int(wave_frames), "total_s": time.perf_counter() - start}
+
+    def finish_pending_calibration_timed(self) -> dict:
+        """finish_pending_calibration_timed (native_bindings.cpp:6510)."""
+        return self._finish()
+
+    def calibrate_frames_fits_u16be_bzero_paths_multistream_timed(self, indices, paths, data_offsets, byte_counts,
+                                                                  light_exposures, dark_exposures, stream_count,
+                                                                  wave_frames, policy=None):
+        """FITS light paths with known data offsets: read into the pinned
+        ring and calibrated as they arrive."""
+        from .staging import FitsCompletionQueue
+
+        q = FitsCompletionQueue(self, _index_array(indices), paths, data_offsets, byte_counts,
+                                max(2, int(wave_frames)), 1)
+        out = q.calibrate(light_exposures, dark_exposures, policy)
+        out.update(stream_count=int(stream_count), wave_frames=int(wave_frames))
+        return out
+
+    def calibrate_frames_fits_u16be_bzero_paths_completion_queue_timed(
+            self, indices, paths, data_offsets, byte_counts, light_exposures, dark_exposures, stream_count,
+            queue_buffer_count, worker_count, policy=None):
+        """The FITS completion queue in one call: `worker_count` reader threads,
+        `queue_buffer_count` pinned buffers, calibration in completion order."""
+        self.begin_fits_u16be_bzero_paths_completion_queue_read_timed(
+            indices, paths, data_offsets, byte_counts, stream_count, queue_buffer_count, worker_count, policy)
+        return self.finish_fits_u16be_bzero_paths_completion_queue_calibration_timed(
+            light_exposures, dark_exposures, stream_count, policy)
+
+    def begin_fits_u16be_bzero_paths_completion_queue_read_timed(self, indices, paths, data_offsets, byte_counts,
+                                                                 stream_count, queue_buffer_count, worker_count,
+                                                                 policy=None):
+        """Start the reader threads of the FITS completion queue."""
+        from .staging import FitsCompletionQueue
+
+        if self._fits_queue is not None:
+            raise RuntimeError("a FITS completion queue read is already in progress")
+        t0 = time.perf_counter()
+        self._fits_queue = FitsCompletionQueue(self, _index_array(indices), paths, data_offsets, byte_counts,
+                                               int(queue_buffer_count), int(worker_count))
+        return {"schema_version": 1, "frame_count": len(self._fits_queue.indices),
+                "queue_buffer_count": len(self._fits_queue.buffers), "worker_count": int(worker_count),
+                "begin_s": time.perf_counter() - t0}
+
+    def finish_fits_u16be_bzero_paths_completion_queue_calibration_timed(self, light_exposures, dark_exposures,
+                                                                         stream_count, policy=None):
+        """Calibrate the queued reads in completion order and wait for them."""
+        if self._fits_queue is None:
+            raise RuntimeError("no FITS completion queue read is in progress")
+        q, self._fits_queue = self._fits_queue, None
+        out = q.calibrate(light_exposures, dark_exposures, policy)
+        out["stream_count"] = int(stream_count)
+        return out
+
+    def cancel_fits_u16be_bzero_paths_completion_queue_read(self) -> None:
+        """Stop the reader threads of a queue that will not be calibrated."""
+        if self._fits_queue is not None:
+            q, self._fits_queue = self._fits_queue, None
+            q.cancel()
 
     def calibrate_frames_host(self, indices, raw, raw_format: str, light_exposures, dark_exposures, policy=None,
                               frame_stride_bytes: int | None = None, accumulate_stats: bool = False,

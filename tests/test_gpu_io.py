@@ -106,3 +106,65 @@ def test_calibrate_fits_files_through_the_pinned_ring(gpu, tmp_path):
     with ResidentCalibratedStack(1, h + 1, w) as s:
         with pytest.raises(ValueError, match="shape_mismatch"):
             s.calibrate_fits_files([0], paths[:1], [45.0], 90.0, POLICY)
+
+
+def test_async_and_multistream_calibration_entries(gpu):
+    f, h, w = 6, 30, 44
+    rng = np.random.default_rng(21)
+    adu = rng.integers(0, 65536, (f, h, w), dtype=np.uint16)
+    lights = [a.astype(np.float32) for a in adu]
+    masters = _masters(h, w, 7)
+    exps = [20.0 + k for k in range(f)]
+    expected = [_expected(adu[k], masters, exps[k], 60.0) for k in range(f)]
+    released = []
+    with ResidentCalibratedStack(f, h, w) as s:
+        s.set_calibration_masters(*masters)
+        s.calibrate_frame_pinned_async(0, lights[0], exps[0], 60.0, POLICY)
+        s.calibrate_frame_host_async_timed(1, lights[1], exps[1], 60.0, POLICY)
+        done = s.finish_pending_calibration_timed()
+        assert done["pending"] is True and done["frame_count"] == 2
+        assert s.finish_pending_calibration_timed()["pending"] is False
+        s.calibrate_frames_host_async_multistream_timed([2, 3], lights[2:4], exps[2:4], [60.0, 60.0], 2, POLICY)
+        s.calibrate_frames_host_async_multistream_h2d_release_timed([4], lights[4:5], exps[4:5], [60.0], 2, POLICY)
+        s.finish_pending_calibration_timed()
+        s.calibrate_frames_host_async_multistream_callback_release_timed([5], lights[5:6], exps[5:6], [60.0], 2, 1,
+                                                                         released.append, POLICY)
+        got = s.download_frames_tile(list(range(f)), 0, 0, w, h)
+    assert released == [0]
+    for k in range(f):
+        assert np.array_equal(got[k], expected[k]), k
+
+
+@pytest.mark.parametrize("buffers,workers", [(2, 3), (8, 1)])
+def test_fits_completion_queue(gpu, tmp_path, buffers, workers):
+    f, h, w = 7, 26, 34
+    rng = np.random.default_rng(buffers * 10 + workers)
+    adu = rng.integers(0, 65536, (f, h, w), dtype=np.uint16)
+    paths, offsets = [], []
+    for k in range(f):
+        p = tmp_path / f"l{k}.fits"
+        F.write_u16be_bzero(p, adu[k])
+        spec = F.simple_fits_image_spec(p)
+        paths.append(str(p))
+        offsets.append(spec.data_offset)
+    masters = _masters(h, w, 8)
+    order = [6, 2, 4, 0, 1, 5, 3]
+    with ResidentCalibratedStack(f, h, w) as s:
+        s.set_calibration_masters(*masters)
+        prof = s.calibrate_frames_fits_u16be_bzero_paths_completion_queue_timed(
+            order, [paths[i] for i in order], [offsets[i] for i in order], [h * w * 2] * f, [30.0] * f, 60.0, 2,
+            buffers, workers, POLICY)
+        got = s.download_frames_tile(list(range(f)), 0, 0, w, h)
+        assert sorted(prof["completion_order"]) == list(range(f)) and prof["bytes_read"] == f * h * w * 2
+        # begin / cancel leaves the stack usable; paths form of the multistream entry
+        s.begin_fits_u16be_bzero_paths_completion_queue_read_timed([0], paths[:1], offsets[:1], [h * w * 2], 1, 2, 1)
+        s.cancel_fits_u16be_bzero_paths_completion_queue_read()
+        s.calibrate_frames_fits_u16be_bzero_paths_multistream_timed([1, 0], [paths[0], paths[1]], offsets[:2],
+                                                                   [h * w * 2] * 2, [30.0] * 2, [60.0] * 2, 2, 2,
+                                                                   POLICY)
+        swapped = s.download_frames_tile([0, 1], 0, 0, w, h)
+    for k in range(f):
+        # frame index `order[j]` holds file `order[j]`
+        assert np.array_equal(got[k], _expected(adu[k], masters, 30.0, 60.0)), k
+    assert np.array_equal(swapped[1], _expected(adu[0], masters, 30.0, 60.0))
+    assert np.array_equal(swapped[0], _expected(adu[1], masters, 30.0, 60.0))
