@@ -173,6 +173,10 @@ _SIGNATURES = {
         c_void_p, POINTER(c_float), POINTER(c_double), POINTER(ClipParams), c_void_p, c_void_p, c_void_p, c_void_p,
         c_void_p, c_void_p, c_void_p,
     ],
+    "gh_estimate_translation": [
+        c_void_p, c_int64, c_int64, c_int32, c_int32, c_int32, c_float, c_float, c_float, c_int32,
+        POINTER(c_float), POINTER(c_float), POINTER(c_float), POINTER(c_int64),
+    ],
     "gh_translate_frame": [c_void_p, c_int64, c_int32, c_double, c_double, c_float, c_void_p],
     "gh_frames_stats": [c_void_p, POINTER(c_int64), c_int64, POINTER(FrameStatistics)],
     "gh_pair_grid_stats": [

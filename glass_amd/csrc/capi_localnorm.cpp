@@ -287,3 +287,63 @@ int gh_integrate_clip(gh_stack* s, const float* weights, const double* inverses,
 }
 
 }  // extern "C"
+
+extern "C" {
+
+int gh_estimate_translation(gh_stack* s, int64_t reference_index, int64_t moving_index, int32_t subpixel,
+                            int32_t radius_x, int32_t radius_y, float center_dx, float center_dy, float step,
+                            int32_t sample_stride, float* dx, float* dy, float* score, int64_t* candidates) {
+  return guarded([&] {
+    const char* what = subpixel ? "resident subpixel translation search" : "resident translation search";
+    s->require_loaded(reference_index, what);
+    s->require_loaded(moving_index, what);
+    if (subpixel) {
+      if (radius_x < 0 || radius_y < 0) throw gh::ValueError("radius_steps must be non-negative");
+      if (!(step > 0.0f)) throw gh::ValueError("step must be positive");
+    } else if (radius_x < 0 || radius_y < 0) {
+      throw gh::ValueError("max shifts must be non-negative");
+    }
+    if (sample_stride <= 0) throw gh::ValueError("sample_stride must be positive");
+    if (!dx || !dy || !score) throw gh::ValueError("output pointers must not be NULL");
+    const int64_t n = (2 * static_cast<int64_t>(radius_x) + 1) * (2 * static_cast<int64_t>(radius_y) + 1);
+    if (n > (int64_t(1) << 24)) throw gh::ValueError("too many translation candidates");
+    gh::NccArgs a{};
+    a.reference = s->slot[reference_index];
+    a.moving = s->slot[moving_index];
+    a.width = static_cast<int>(s->width);
+    a.height = static_cast<int>(s->height);
+    a.stride = sample_stride;
+    a.radius_x = radius_x;
+    a.radius_y = radius_y;
+    a.center_dx = center_dx;
+    a.center_dy = center_dy;
+    a.step = step;
+    const int chunks = gh::ncc_chunks(a);
+    const size_t part = static_cast<size_t>(n) * static_cast<size_t>(chunks) * 6;
+    s->ncc_ws.ensure(part + static_cast<size_t>(n), "hipMalloc(translation search)");
+    s->wait_caller();
+    gh::launch_ncc_search(a, subpixel != 0, s->ncc_ws.ptr, s->ncc_ws.ptr + part, s->stream);
+    std::vector<double> scores(static_cast<size_t>(n));
+    check(hipMemcpyAsync(scores.data(), s->ncc_ws.ptr + part, static_cast<size_t>(n) * sizeof(double),
+                         hipMemcpyDeviceToHost, s->stream),
+          "hipMemcpy(translation scores)");
+    check(hipStreamSynchronize(s->stream), "translation search");
+    // first best candidate in row-major candidate order
+    int64_t best = 0;
+    for (int64_t i = 1; i < n; ++i)
+      if (scores[i] > scores[best]) best = i;
+    const int cw = 2 * radius_x + 1;
+    const int ox = static_cast<int>(best % cw) - radius_x, oy = static_cast<int>(best / cw) - radius_y;
+    if (subpixel) {
+      *dx = center_dx + static_cast<float>(ox) * step;
+      *dy = center_dy + static_cast<float>(oy) * step;
+    } else {
+      *dx = static_cast<float>(ox);
+      *dy = static_cast<float>(oy);
+    }
+    *score = static_cast<float>(scores[best]);
+    if (candidates) *candidates = n;
+  });
+}
+
+}  // extern "C"

@@ -192,6 +192,19 @@ struct VariantJob {
 };
 void launch_decide_variants(const VariantJob* jobs, int n_jobs, int* out, hipStream_t stream);
 
+// Normalised cross-correlation over a grid of candidate translations:
+// integer offsets (ox, oy) in [-radius, radius], or float32 offsets
+// center + (ox, oy) * step sampled bilinearly.
+struct NccArgs {
+  const float* reference;
+  const float* moving;
+  int width, height, stride;
+  int radius_x, radius_y;
+  float center_dx, center_dy, step;  // subpixel candidates only
+};
+int ncc_chunks(const NccArgs& a);  // row chunks per candidate (partials: candidates * chunks * 6 doubles)
+void launch_ncc_search(const NccArgs& a, bool subpixel, double* partials, double* scores, hipStream_t stream);
+
 // Integer or float32-bilinear translation of one frame into dst (fill
 // outside); coverage (nullable) += 1 inside; *flag |= any non-finite output.
 void launch_translate(const float* src, float* dst, int width, int height, bool bilinear, int idx, int idy, float fdx,

@@ -635,6 +635,36 @@ class ResidentCalibratedStack:
         N.call("gh_translate_frame", self._h, int(index), N.GH_INTERP_BILINEAR, float(np.float32(dx)),
                float(np.float32(dy)), float(fill), ctypes.c_void_p(stream) if stream else None)
 
+    def _ncc(self, reference_index, moving_index, subpixel, rx, ry, cdx, cdy, step, stride):
+        dx, dy, score, n = ctypes.c_float(), ctypes.c_float(), ctypes.c_float(), ctypes.c_int64()
+        N.call("gh_estimate_translation", self._h, int(reference_index), int(moving_index), int(subpixel), int(rx),
+               int(ry), float(cdx), float(cdy), float(step), int(stride), ctypes.byref(dx), ctypes.byref(dy),
+               ctypes.byref(score), ctypes.byref(n))
+        return dx.value, dy.value, score.value, int(n.value)
+
+    def estimate_translation_to_reference(self, reference_index: int, moving_index: int, max_shift_x: int,
+                                          max_shift_y: int, sample_stride: int = 1) -> dict:
+        """Integer translation (moving -> reference) maximising the normalised
+        cross-correlation (native_bindings.cpp:10418-10485)."""
+        dx, dy, score, n = self._ncc(reference_index, moving_index, 0, max_shift_x, max_shift_y, 0.0, 0.0, 1.0,
+                                     sample_stride)
+        return {"dx": int(dx), "dy": int(dy), "score": score, "search_count": n, "sample_stride": int(sample_stride),
+                "reference_index": int(reference_index), "moving_index": int(moving_index),
+                "model": "resident_translation_integer_ncc"}
+
+    def estimate_translation_subpixel_to_reference(self, reference_index: int, moving_index: int, center_dx: float,
+                                                   center_dy: float, radius_steps: int, step: float,
+                                                   sample_stride: int = 1) -> dict:
+        """Subpixel refinement of a translation on a (2r+1)^2 grid of float32
+        offsets around the centre (native_bindings.cpp:10487-10568)."""
+        dx, dy, score, n = self._ncc(reference_index, moving_index, 1, radius_steps, radius_steps, center_dx,
+                                     center_dy, step, sample_stride)
+        return {"dx": dx, "dy": dy, "score": score, "candidate_count": n, "center_dx": float(np.float32(center_dx)),
+                "center_dy": float(np.float32(center_dy)), "radius_steps": int(radius_steps),
+                "step": float(np.float32(step)), "sample_stride": int(sample_stride),
+                "reference_index": int(reference_index), "moving_index": int(moving_index),
+                "model": "resident_translation_subpixel_ncc"}
+
     def reset_warp_coverage(self) -> None:
         N.call("gh_reset_warp_coverage", self._h)
 

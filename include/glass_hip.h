@@ -191,6 +191,21 @@ int gh_warp_frames(gh_stack* stack, const int64_t* indices, int64_t count, const
 int gh_translate_frame(gh_stack* stack, int64_t index, int32_t mode, double dx, double dy, float fill,
                        void* stream);
 
+/* estimate_translation_to_reference(reference, moving, max_shift_x, max_shift_y,
+ * sample_stride) (subpixel = 0: integer offsets within +-radius) and
+ * estimate_translation_subpixel_to_reference(reference, moving, center_dx,
+ * center_dy, radius_steps, step, sample_stride) (subpixel = 1: offsets
+ * center + k * step, k within +-radius, sampled bilinearly in float32)
+ * (native_bindings.cpp:10418-10568; kernels
+ * cpp/cuda/registration_kernels.cu:10-264): the offset (dx, dy), moving ->
+ * reference, that maximises the normalised cross-correlation of the reference
+ * with the moving frame shifted by it, over the reference pixels sampled every
+ * `sample_stride` (pairs finite in both).  The first best candidate in
+ * row-major order wins.  FP64 sums; `score` rounded to float32. */
+int gh_estimate_translation(gh_stack* stack, int64_t reference_index, int64_t moving_index, int32_t subpixel,
+                            int32_t radius_x, int32_t radius_y, float center_dx, float center_dy, float step,
+                            int32_t sample_stride, float* dx, float* dy, float* score, int64_t* candidates);
+
 int gh_reset_warp_coverage(gh_stack* stack);
 int gh_accumulate_full_warp_coverage(gh_stack* stack);
 int gh_warp_coverage_map(gh_stack* stack, float* out);

@@ -322,3 +322,61 @@ def estimate_star_transform(reference, moving, transform_model="translation", mi
     return Registration(transform_model, [[float(v) for v in row] for row in best], len(best_pairs),
                         len(best_pairs), float(best_rms) if np.isfinite(best_rms) else float("nan"), status,
                         best_pairs)
+
+
+def ncc_translation_search(reference, moving, radius_x, radius_y, subpixel=False, center_dx=0.0, center_dy=0.0,
+                           step=1.0, sample_stride=1):
+    """Normalised cross-correlation translation search
+    (cpp/cuda/registration_kernels.cu:10-264): candidates in row-major order,
+    integer offsets or float32 offsets center + k * step sampled bilinearly in
+    float32; FP64 sums over sampled reference pixels whose pair is finite;
+    the first best score wins.  Returns (dx, dy, score as float32, scores)."""
+    ref = np.asarray(reference, dtype=np.float32)
+    mov = np.asarray(moving, dtype=np.float32)
+    h, w = ref.shape
+    ys, xs = np.meshgrid(np.arange(0, h, sample_stride), np.arange(0, w, sample_stride), indexing="ij")
+    r_all = ref[ys, xs].astype(np.float64)
+    f32 = np.float32
+    scores = []
+    offsets = []
+    for oy in range(-radius_y, radius_y + 1):
+        for ox in range(-radius_x, radius_x + 1):
+            if subpixel:
+                fdx = f32(f32(center_dx) + f32(ox) * f32(step))
+                fdy = f32(f32(center_dy) + f32(oy) * f32(step))
+                sx = (xs.astype(f32) - fdx).astype(f32)
+                sy = (ys.astype(f32) - fdy).astype(f32)
+                ok = ~((sx < 0) | (sx > f32(w - 1)) | (sy < 0) | (sy > f32(h - 1)))
+                x0 = np.floor(sx[ok]).astype(np.int64)
+                y0 = np.floor(sy[ok]).astype(np.int64)
+                x1 = np.where(x0 + 1 < w, x0 + 1, x0)
+                y1 = np.where(y0 + 1 < h, y0 + 1, y0)
+                tx = (sx[ok] - x0.astype(f32)).astype(f32)
+                ty = (sy[ok] - y0.astype(f32)).astype(f32)
+                one = f32(1)
+                top = (mov[y0, x0] * (one - tx) + mov[y0, x1] * tx).astype(f32)
+                bot = (mov[y1, x0] * (one - tx) + mov[y1, x1] * tx).astype(f32)
+                m = (top * (one - ty) + bot * ty).astype(f32)
+                offsets.append((float(fdx), float(fdy)))
+            else:
+                sx, sy = xs - ox, ys - oy
+                ok = (sx >= 0) & (sx < w) & (sy >= 0) & (sy < h)
+                m = mov[sy[ok], sx[ok]]
+                offsets.append((float(ox), float(oy)))
+            r = r_all[ok]
+            m = m.astype(np.float64)
+            fin = np.isfinite(r) & np.isfinite(m)
+            r, m = r[fin], m[fin]
+            n = float(r.size)
+            score = -float(np.finfo(np.float32).max)
+            if n > 1:
+                sr, sm = r.sum(), m.sum()
+                num = (r * m).sum() - sr * sm / n
+                vr = max((r * r).sum() - sr * sr / n, 0.0)
+                vm = max((m * m).sum() - sm * sm / n, 0.0)
+                den = np.sqrt(vr * vm)
+                if den > 0:
+                    score = num / den
+            scores.append(score)
+    best = int(np.argmax(np.asarray(scores)))  # first maximum
+    return offsets[best][0], offsets[best][1], float(np.float32(scores[best])), np.asarray(scores)

@@ -86,3 +86,60 @@ def test_detect_then_register_recovers_shift(gpu):
     assert r.status == "ok" and r.inliers >= 20
     assert r.matrix == o.matrix and sorted(r.pairs) == sorted(o.pairs)
     assert math.isclose(r.matrix[0][2], 5.0, abs_tol=0.05) and math.isclose(r.matrix[1][2], -3.0, abs_tol=0.05)
+
+
+def test_reference_fixture_ncc_translation_then_bilinear_shift(gpu):
+    # tests/test_cuda_resident_stack.py:1684-1737
+    from glass_amd import ResidentCalibratedStack
+    from oracle.registration import ncc_translation_search
+    from oracle.warp import translate_bilinear_f32
+
+    image = np.zeros((64, 64), dtype=np.float32)
+    yy, xx = np.indices(image.shape, dtype=np.float32)
+    for x, y, flux in [(12.25, 14.50, 150.0), (24.00, 20.75, 230.0), (41.50, 18.25, 180.0), (16.75, 42.00, 130.0),
+                       (49.25, 45.50, 210.0), (35.50, 33.25, 170.0)]:
+        image += flux * np.exp(-(((xx - x) ** 2 + (yy - y) ** 2) / (2.0 * 1.2**2)))
+    reference = (image + 10.0 + 0.01 * xx + 0.02 * yy).astype(np.float32)
+    moving, _ = translate_bilinear_f32(reference, -2.25, 1.5, fill=0.0)
+    with ResidentCalibratedStack(2, 64, 64) as s:
+        s.upload_calibrated_frame(0, reference)
+        s.upload_calibrated_frame(1, moving)
+        coarse = s.estimate_translation_to_reference(0, 1, 4, 4)
+        refined = s.estimate_translation_subpixel_to_reference(0, 1, float(coarse["dx"]), float(coarse["dy"]), 3, 0.25)
+        s.apply_translation_bilinear_frame(1, refined["dx"], refined["dy"], np.nan)
+        aligned, weight_map = s.integrate_mean(np.array([0.0, 1.0], dtype=np.float32))
+    valid = weight_map > 0.0
+    valid &= (xx >= 8) & (xx < 56) & (yy >= 8) & (yy < 56)
+    rms = float(np.sqrt(np.mean((aligned[valid] - reference[valid]) ** 2)))
+    assert coarse["model"] == "resident_translation_integer_ncc" and coarse["search_count"] == 81
+    assert refined["model"] == "resident_translation_subpixel_ncc" and refined["candidate_count"] == 49
+    assert abs(refined["dx"] - 2.25) <= 0.25 and abs(refined["dy"] + 1.5) <= 0.25
+    assert refined["score"] > 0.98 and rms < 4.0
+    # the same candidates and the same winner as the oracle
+    odx, ody, oscore, _ = ncc_translation_search(reference, moving, 4, 4)
+    assert (coarse["dx"], coarse["dy"]) == (odx, ody) and coarse["score"] == pytest.approx(oscore, abs=1e-6)
+    sdx, sdy, sscore, _ = ncc_translation_search(reference, moving, 3, 3, True, coarse["dx"], coarse["dy"], 0.25)
+    assert (refined["dx"], refined["dy"]) == (sdx, sdy) and refined["score"] == pytest.approx(sscore, abs=1e-6)
+
+
+@pytest.mark.parametrize("stride", [1, 3])
+def test_ncc_translation_matches_oracle_with_nan_and_stride(gpu, stride):
+    from glass_amd import ResidentCalibratedStack
+    from oracle.registration import ncc_translation_search
+    from oracle.warp import translate_integer
+
+    rng = np.random.default_rng(stride)
+    ref = rng.normal(50, 8, (70, 90)).astype(np.float32)
+    ref[10:14, 20:25] = np.nan
+    mov, _ = translate_integer(ref, 5, -4, fill=np.nan)
+    mov += rng.normal(0, 0.5, mov.shape).astype(np.float32)
+    with ResidentCalibratedStack(2, 70, 90) as s:
+        s.upload_calibrated_frame(0, ref)
+        s.upload_calibrated_frame(1, mov)
+        got = s.estimate_translation_to_reference(0, 1, 7, 6, stride)
+        sub = s.estimate_translation_subpixel_to_reference(0, 1, -5.0, 4.0, 2, 0.5, stride)
+    dx, dy, score, _ = ncc_translation_search(ref, mov, 7, 6, sample_stride=stride)
+    assert (got["dx"], got["dy"]) == (dx, dy) == (-5.0, 4.0)
+    assert got["score"] == pytest.approx(score, abs=1e-6)
+    sdx, sdy, sscore, _ = ncc_translation_search(ref, mov, 2, 2, True, -5.0, 4.0, 0.5, stride)
+    assert (sub["dx"], sub["dy"]) == (sdx, sdy) and sub["score"] == pytest.approx(sscore, abs=1e-6)

@@ -127,3 +127,35 @@ def test_library_transform_rejects_unknown_model():
 
     with pytest.raises(ValueError):
         R.estimate_star_transform([O.Star(1.0, 1.0, 1.0)], [O.Star(1.0, 1.0, 1.0)], "homography")
+
+
+def test_ncc_translation_search_closed_forms():
+    from oracle.registration import ncc_translation_search
+    from oracle.warp import translate_bilinear_f32, translate_integer
+
+    rng = np.random.default_rng(5)
+    ref = rng.normal(10, 3, (40, 48)).astype(np.float32)
+    mov, _ = translate_integer(ref, -3, 2)  # moving = reference shifted by (-3, 2)
+    dx, dy, score, scores = ncc_translation_search(ref, mov, 4, 4)
+    # the estimate maps moving onto reference: the applied shift undone
+    assert (dx, dy) == (3.0, -2.0) and score == pytest.approx(1.0, abs=1e-6)
+    # a score is the Pearson coefficient of the overlapping finite pairs
+    ys, xs = np.indices(ref.shape)
+    ok = (xs - 1 >= 0) & (ys + 2 < 40)
+    r = ref[ok]
+    m = mov[ys[ok] + 2, xs[ok] - 1]
+    fin = np.isfinite(m)
+    idx = (-2 + 4) * 9 + (1 + 4)  # candidate (ox=1, oy=-2)
+    assert scores[idx] == pytest.approx(np.corrcoef(r[fin], m[fin])[0, 1], rel=1e-9)
+    # subpixel grid around the integer estimate, the reference fixture's way
+    yy, xx = np.indices((64, 64), dtype=np.float32)
+    image = np.zeros((64, 64), np.float32)
+    for x, y, flux in [(12.25, 14.5, 150.0), (24.0, 20.75, 230.0), (41.5, 18.25, 180.0), (16.75, 42.0, 130.0),
+                       (49.25, 45.5, 210.0), (35.5, 33.25, 170.0)]:
+        image += flux * np.exp(-(((xx - x) ** 2 + (yy - y) ** 2) / (2.0 * 1.2 ** 2)))
+    reference = (image + 10.0 + 0.01 * xx + 0.02 * yy).astype(np.float32)
+    moving, _ = translate_bilinear_f32(reference, -2.25, 1.5, fill=0.0)
+    cdx, cdy, _, _ = ncc_translation_search(reference, moving, 4, 4)
+    sdx, sdy, sscore, sc = ncc_translation_search(reference, moving, 3, 3, True, cdx, cdy, 0.25)
+    assert len(sc) == 49 and sscore > 0.98
+    assert abs(sdx - 2.25) <= 0.25 and abs(sdy - (-1.5)) <= 0.25
