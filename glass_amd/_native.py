@@ -8,7 +8,7 @@ point raises, so nothing silently runs on the CPU.
 from __future__ import annotations
 
 import ctypes
-from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_int32, c_int64, c_size_t, c_uint32, c_uint64, c_void_p
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_int32, c_int64, c_size_t, c_uint8, c_uint32, c_uint64, c_void_p
 from pathlib import Path
 
 LIBRARY_PATH = Path(__file__).resolve().parent / "lib" / "libglass_hip.so"
@@ -172,6 +172,10 @@ _SIGNATURES = {
     "gh_integrate_clip": [
         c_void_p, POINTER(c_float), POINTER(c_double), POINTER(ClipParams), c_void_p, c_void_p, c_void_p, c_void_p,
         c_void_p, c_void_p, c_void_p,
+    ],
+    "gh_integrate_tile_local": [
+        c_void_p, POINTER(c_float), POINTER(c_uint8), POINTER(c_int32), POINTER(c_float), c_int64,
+        POINTER(ClipParams), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
     ],
     "gh_estimate_translation": [
         c_void_p, c_int64, c_int64, c_int32, c_int32, c_int32, c_float, c_float, c_float, c_int32,

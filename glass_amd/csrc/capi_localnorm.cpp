@@ -3,7 +3,9 @@
 // batched frame statistics and the mean / sigma-clip integrations that sample
 // the frames themselves (native_bindings.cpp:13227-13393, :14632-15115).
 
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
 
 #include "stack.hpp"
 
@@ -183,6 +185,180 @@ int gh_apply_normalization(gh_stack* s, const int64_t* indices, int64_t count, c
 
 }  // extern "C"
 
+namespace {
+
+// Tile-local weight policy of gh_integrate_tile_local.
+struct TilePolicy {
+  const uint8_t* target_mask;  // [frames]
+  const int32_t* extents;      // [tiles][4]
+  const float* multipliers;    // [tiles]
+  int64_t tiles;
+  int64_t max_area;            // filled in by validate_tiles
+};
+
+// Checks of native_bindings.cpp:13042-13096 (same order and messages).
+void validate_tiles(const gh_stack* s, TilePolicy& t) {
+  if (!t.target_mask) throw gh::ValueError("target_mask must have shape (frame_count,)");
+  int64_t targets = 0;
+  for (int64_t f = 0; f < s->frames; ++f) targets += t.target_mask[f] != 0;
+  if (targets == 0) throw gh::ValueError("target_mask must select at least one resident frame");
+  if (t.tiles <= 0 || t.tiles > INT32_MAX || !t.extents)
+    throw gh::ValueError("tile_extents must contain between 1 and INT_MAX tiles");
+  if (!t.multipliers) throw gh::ValueError("tile_multipliers must have shape (tile_count,)");
+  t.max_area = 0;
+  for (int64_t i = 0; i < t.tiles; ++i) {
+    const int32_t* e = t.extents + 4 * i;
+    if (e[0] < 0 || e[1] < 0 || e[2] <= e[0] || e[3] <= e[1] || e[2] > s->width || e[3] > s->height)
+      throw gh::ValueError("tile_extents must be positive half-open rectangles inside the image");
+    if (!std::isfinite(t.multipliers[i]) || t.multipliers[i] < 0.0f)
+      throw gh::ValueError("tile_multipliers must be finite non-negative values");
+    t.max_area = std::max<int64_t>(t.max_area, int64_t(e[2] - e[0]) * (e[3] - e[1]));
+  }
+  for (int64_t i = 0; i < t.tiles; ++i) {
+    const int32_t* l = t.extents + 4 * i;
+    for (int64_t j = i + 1; j < t.tiles; ++j) {
+      const int32_t* r = t.extents + 4 * j;
+      if (std::max(l[0], r[0]) < std::min(l[2], r[2]) && std::max(l[1], r[1]) < std::min(l[3], r[3]))
+        throw gh::ValueError("tile_extents must not overlap for deterministic tile-local integration");
+    }
+  }
+}
+
+size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
+void integrate_clip(gh_stack* s, const float* weights, const double* inverses, const gh_clip_params& prm,
+                    TilePolicy* tiles, float* const user[6], void* stream_ptr) {
+  if (prm.sampler != GH_CLIP_STACK && prm.sampler != GH_INTERP_BILINEAR && prm.sampler != GH_INTERP_LANCZOS3)
+    throw gh::ValueError("interpolation must be bilinear or lanczos3");
+  if (prm.rejection != GH_REJECT_NONE && prm.rejection != GH_REJECT_SIGMA_CLIP)
+    throw gh::ValueError("unsupported rejection for the clip integration");
+  if (s->loaded_count != s->frames)
+    throw std::runtime_error("all resident frames must be loaded before integration");
+  if (prm.rejection == GH_REJECT_SIGMA_CLIP && !(prm.low_sigma > 0.0f && prm.high_sigma > 0.0f))
+    throw gh::ValueError("sigma thresholds must be positive");
+  if (prm.sampler != GH_CLIP_STACK && !inverses) throw gh::ValueError("inverse matrices pointer is NULL");
+  if (s->npix >= (int64_t(1) << 31)) throw gh::ValueError("frames of 2^31 pixels or more are not supported");
+  if (tiles) {
+    if (prm.sampler != GH_CLIP_STACK) throw gh::ValueError("tile-local integration samples the stack as stored");
+    validate_tiles(s, *tiles);
+  }
+  // Frames of positive finite weight, in order (the kernels skip the rest;
+  // a tile multiplier >= 0 cannot bring a skipped frame back).
+  std::vector<const float*> table;
+  std::vector<float> w;
+  std::vector<double> inv;
+  std::vector<unsigned char> target;
+  for (int64_t f = 0; f < s->frames; ++f) {
+    const float wf = weights ? weights[f] : 1.0f;
+    if (!(wf > 0.0f) || !std::isfinite(wf)) continue;
+    table.push_back(s->slot[f]);
+    w.push_back(wf);
+    if (tiles) target.push_back(tiles->target_mask[f] != 0 ? 1 : 0);
+    if (prm.sampler != GH_CLIP_STACK) {
+      for (int e = 0; e < 9; ++e) {
+        if (!std::isfinite(inverses[f * 9 + e])) throw gh::ValueError("warp inverse matrix must be finite");
+        inv.push_back(inverses[f * 9 + e]);
+      }
+    }
+  }
+  hipStream_t st = s->pick(stream_ptr);
+  if (st == s->stream) s->wait_caller();
+  const size_t n = table.size();
+  // one allocation: pointer table, weights, inverses | target flags, tile
+  // extents, tile multipliers
+  const size_t table_bytes = round256(n * sizeof(const float*));
+  const size_t w_bytes = round256(n * sizeof(float));
+  const size_t inv_bytes = round256(inv.size() * sizeof(double));
+  const size_t target_bytes = round256(target.size());
+  const size_t ntiles = tiles ? static_cast<size_t>(tiles->tiles) : 0;
+  const size_t ext_bytes = round256(ntiles * 4 * sizeof(int32_t));
+  const size_t mult_bytes = round256(ntiles * sizeof(float));
+  s->clip_tables.ensure(table_bytes + w_bytes + inv_bytes + target_bytes + ext_bytes + mult_bytes + 8,
+                        "hipMalloc(clip tables)");
+  unsigned char* base = s->clip_tables.ptr;
+  unsigned char* const w_at = base + table_bytes;
+  unsigned char* const inv_at = w_at + w_bytes;
+  unsigned char* const target_at = inv_at + inv_bytes;
+  unsigned char* const ext_at = target_at + target_bytes;
+  unsigned char* const mult_at = ext_at + ext_bytes;
+  s->wait_tables();
+  if (n) {
+    check(hipMemcpy(base, table.data(), n * sizeof(const float*), hipMemcpyHostToDevice), "hipMemcpy(clip table)");
+    check(hipMemcpy(w_at, w.data(), n * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy(clip weights)");
+    if (!inv.empty())
+      check(hipMemcpy(inv_at, inv.data(), inv.size() * sizeof(double), hipMemcpyHostToDevice),
+            "hipMemcpy(clip inverses)");
+    if (!target.empty())
+      check(hipMemcpy(target_at, target.data(), n, hipMemcpyHostToDevice), "hipMemcpy(tile-local target mask)");
+  }
+  if (tiles) {
+    check(hipMemcpy(ext_at, tiles->extents, ntiles * 4 * sizeof(int32_t), hipMemcpyHostToDevice),
+          "hipMemcpy(tile-local extents)");
+    check(hipMemcpy(mult_at, tiles->multipliers, ntiles * sizeof(float), hipMemcpyHostToDevice),
+          "hipMemcpy(tile-local multipliers)");
+  }
+  gh::ClipArgs a{};
+  a.frames = reinterpret_cast<const float* const*>(base);
+  a.weights = reinterpret_cast<const float*>(w_at);
+  a.inverses = inv.empty() ? nullptr : reinterpret_cast<const double*>(inv_at);
+  a.n = static_cast<int>(n);
+  a.width = static_cast<int>(s->width);
+  a.height = static_cast<int>(s->height);
+  a.npix = s->npix;
+  a.clamp = prm.clamping_threshold;
+  // affine inverses skip the projective divide (x / 1.0 is x exactly)
+  a.affine = 1;
+  for (size_t k = 0; k < n && !inv.empty(); ++k)
+    if (!(inv[k * 9 + 6] == 0.0 && inv[k * 9 + 7] == 0.0 && inv[k * 9 + 8] == 1.0)) a.affine = 0;
+  a.mode = prm.rejection == GH_REJECT_SIGMA_CLIP ? 1 : 0;
+  a.winsorize = prm.winsorize ? 1 : 0;
+  a.low_sigma = prm.low_sigma;
+  a.high_sigma = prm.high_sigma;
+  float* dev[6] = {};
+  // Host outputs go through device temporaries (the integration outputs'
+  // five, plus one for the geometric coverage) and are copied back.
+  for (int i = 0; i < 6; ++i) {
+    if (!user[i]) continue;
+    if (is_device_pointer(user[i])) {
+      dev[i] = user[i];
+      continue;
+    }
+    DeviceBuffer<float>& tmp = i < 5 ? s->out_f32[i] : s->clip_geometric;
+    tmp.ensure(static_cast<size_t>(s->npix), "hipMalloc(clip output)");
+    dev[i] = tmp.ptr;
+  }
+  a.master = dev[0];
+  a.weight = dev[1];
+  a.coverage = dev[2];
+  a.low = dev[3];
+  a.high = dev[4];
+  a.geometric = dev[5];
+  if (tiles) {
+    s->clip_tile_map.ensure(static_cast<size_t>(s->npix), "hipMalloc(tile-local multiplier map)");
+    gh::launch_tile_map(s->clip_tile_map.ptr, reinterpret_cast<const int*>(ext_at),
+                        reinterpret_cast<const float*>(mult_at), static_cast<int>(ntiles), tiles->max_area,
+                        a.width, a.npix, st);
+    a.tile_map = s->clip_tile_map.ptr;
+    a.target = target_at;
+    gh::launch_clip_integrate_tiles(a, st);
+  } else {
+    const int sampler = prm.sampler == GH_CLIP_STACK ? 0 : (prm.sampler == GH_INTERP_BILINEAR ? 1 : 2);
+    gh::launch_clip_integrate(a, sampler, st);
+  }
+  s->mark_tables(st);
+  bool any_host = false;
+  for (int i = 0; i < 6; ++i) {
+    if (!user[i] || dev[i] == user[i]) continue;
+    any_host = true;
+    check(hipMemcpyAsync(user[i], dev[i], static_cast<size_t>(s->npix) * sizeof(float), hipMemcpyDeviceToHost, st),
+          "hipMemcpy(clip output)");
+  }
+  s->mark_caller(st);
+  if (any_host || !stream_ptr) check(hipStreamSynchronize(st), "clip integration");
+}
+
+}  // namespace
+
 extern "C" {
 
 int gh_integrate_clip(gh_stack* s, const float* weights, const double* inverses, const gh_clip_params* params,
@@ -190,99 +366,20 @@ int gh_integrate_clip(gh_stack* s, const float* weights, const double* inverses,
                       void* stream_ptr) {
   return guarded([&] {
     if (!params) throw gh::ValueError("clip parameters pointer is NULL");
-    const gh_clip_params prm = *params;
-    if (prm.sampler != GH_CLIP_STACK && prm.sampler != GH_INTERP_BILINEAR && prm.sampler != GH_INTERP_LANCZOS3)
-      throw gh::ValueError("interpolation must be bilinear or lanczos3");
-    if (prm.rejection != GH_REJECT_NONE && prm.rejection != GH_REJECT_SIGMA_CLIP)
-      throw gh::ValueError("unsupported rejection for the clip integration");
-    if (prm.rejection == GH_REJECT_SIGMA_CLIP && !(prm.low_sigma > 0.0f && prm.high_sigma > 0.0f))
-      throw gh::ValueError("sigma thresholds must be positive");
-    if (s->loaded_count != s->frames)
-      throw std::runtime_error("all resident frames must be loaded before integration");
-    if (prm.sampler != GH_CLIP_STACK && !inverses) throw gh::ValueError("inverse matrices pointer is NULL");
-    if (s->npix >= (int64_t(1) << 31)) throw gh::ValueError("frames of 2^31 pixels or more are not supported");
-    // Frames of positive finite weight, in order (the kernels skip the rest).
-    std::vector<const float*> table;
-    std::vector<float> w;
-    std::vector<double> inv;
-    for (int64_t f = 0; f < s->frames; ++f) {
-      const float wf = weights ? weights[f] : 1.0f;
-      if (!(wf > 0.0f) || !std::isfinite(wf)) continue;
-      table.push_back(s->slot[f]);
-      w.push_back(wf);
-      if (prm.sampler != GH_CLIP_STACK) {
-        for (int e = 0; e < 9; ++e) {
-          if (!std::isfinite(inverses[f * 9 + e])) throw gh::ValueError("warp inverse matrix must be finite");
-          inv.push_back(inverses[f * 9 + e]);
-        }
-      }
-    }
-    hipStream_t st = s->pick(stream_ptr);
-    if (st == s->stream) s->wait_caller();
-    const size_t n = table.size();
-    // one allocation for the pointer table, weights and inverses
-    const size_t table_bytes = (n * sizeof(const float*) + 255) / 256 * 256;
-    const size_t w_bytes = (n * sizeof(float) + 255) / 256 * 256;
-    const size_t inv_bytes = inv.size() * sizeof(double);
-    s->clip_tables.ensure(table_bytes + w_bytes + inv_bytes + 8, "hipMalloc(clip tables)");
-    unsigned char* base = s->clip_tables.ptr;
-    s->wait_tables();
-    if (n) {
-      check(hipMemcpy(base, table.data(), n * sizeof(const float*), hipMemcpyHostToDevice), "hipMemcpy(clip table)");
-      check(hipMemcpy(base + table_bytes, w.data(), n * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy(clip weights)");
-      if (inv_bytes)
-        check(hipMemcpy(base + table_bytes + w_bytes, inv.data(), inv_bytes, hipMemcpyHostToDevice),
-              "hipMemcpy(clip inverses)");
-    }
-    gh::ClipArgs a{};
-    a.frames = reinterpret_cast<const float* const*>(base);
-    a.weights = reinterpret_cast<const float*>(base + table_bytes);
-    a.inverses = inv_bytes ? reinterpret_cast<const double*>(base + table_bytes + w_bytes) : nullptr;
-    a.n = static_cast<int>(n);
-    a.width = static_cast<int>(s->width);
-    a.height = static_cast<int>(s->height);
-    a.npix = s->npix;
-    a.clamp = prm.clamping_threshold;
-    // affine inverses skip the projective divide (x / 1.0 is x exactly)
-    a.affine = 1;
-    for (size_t k = 0; k < n && !inv.empty(); ++k)
-      if (!(inv[k * 9 + 6] == 0.0 && inv[k * 9 + 7] == 0.0 && inv[k * 9 + 8] == 1.0)) a.affine = 0;
-    a.mode = prm.rejection == GH_REJECT_SIGMA_CLIP ? 1 : 0;
-    a.winsorize = prm.winsorize ? 1 : 0;
-    a.low_sigma = prm.low_sigma;
-    a.high_sigma = prm.high_sigma;
-    float* user[6] = {master, weight, coverage, low, high, geometric};
-    float* dev[6] = {};
-    // Host outputs go through device temporaries (the integration outputs'
-    // five, plus one for the geometric coverage) and are copied back.
-    for (int i = 0; i < 6; ++i) {
-      if (!user[i]) continue;
-      if (is_device_pointer(user[i])) {
-        dev[i] = user[i];
-        continue;
-      }
-      DeviceBuffer<float>& tmp = i < 5 ? s->out_f32[i] : s->clip_geometric;
-      tmp.ensure(static_cast<size_t>(s->npix), "hipMalloc(clip output)");
-      dev[i] = tmp.ptr;
-    }
-    a.master = dev[0];
-    a.weight = dev[1];
-    a.coverage = dev[2];
-    a.low = dev[3];
-    a.high = dev[4];
-    a.geometric = dev[5];
-    const int sampler = prm.sampler == GH_CLIP_STACK ? 0 : (prm.sampler == GH_INTERP_BILINEAR ? 1 : 2);
-    gh::launch_clip_integrate(a, sampler, st);
-    s->mark_tables(st);
-    bool any_host = false;
-    for (int i = 0; i < 6; ++i) {
-      if (!user[i] || dev[i] == user[i]) continue;
-      any_host = true;
-      check(hipMemcpyAsync(user[i], dev[i], static_cast<size_t>(s->npix) * sizeof(float), hipMemcpyDeviceToHost, st),
-            "hipMemcpy(clip output)");
-    }
-    s->mark_caller(st);
-    if (any_host || !stream_ptr) check(hipStreamSynchronize(st), "clip integration");
+    float* const user[6] = {master, weight, coverage, low, high, geometric};
+    integrate_clip(s, weights, inverses, *params, nullptr, user, stream_ptr);
+  });
+}
+
+int gh_integrate_tile_local(gh_stack* s, const float* weights, const uint8_t* target_mask,
+                            const int32_t* tile_extents, const float* tile_multipliers, int64_t tile_count,
+                            const gh_clip_params* params, float* master, float* weight, float* coverage, float* low,
+                            float* high, void* stream_ptr) {
+  return guarded([&] {
+    if (!params) throw gh::ValueError("clip parameters pointer is NULL");
+    TilePolicy t{target_mask, tile_extents, tile_multipliers, tile_count, 0};
+    float* const user[6] = {master, weight, coverage, low, high, nullptr};
+    integrate_clip(s, weights, nullptr, *params, &t, user, stream_ptr);
   });
 }
 

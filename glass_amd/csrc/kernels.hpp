@@ -224,9 +224,18 @@ struct ClipArgs {
   int winsorize;
   float low_sigma, high_sigma;
   float *master, *weight, *coverage, *low, *high, *geometric;  // nullable outputs
+  const float* tile_map;          // tile-local form: per-pixel multiplier of the target entries
+  const unsigned char* target;    // tile-local form: [n], nonzero = weight scaled by tile_map
 };
 // sampler: 0 stack as stored, 1 bilinear, 2 Lanczos3 (resident tap rules)
 void launch_clip_integrate(const ClipArgs& a, int sampler, hipStream_t stream);
+// Tile-local form over the stack as stored (a.tile_map and a.target set).
+void launch_clip_integrate_tiles(const ClipArgs& a, hipStream_t stream);
+// map[p] = 1, then multipliers[t] over tile t's half-open rectangle
+// extents[4t..4t+3] = (x0, y0, x1, y1); tiles do not overlap; max_area is the
+// largest tile's pixel count.
+void launch_tile_map(float* map, const int* extents, const float* multipliers, int tiles, long long max_area,
+                     int width, long long npix, hipStream_t stream);
 
 // Resident local normalisation.  A batch names up to kLnBatch resident
 // frames (sources for the statistics, targets for the application) and their

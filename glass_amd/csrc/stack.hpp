@@ -137,6 +137,7 @@ struct gh_stack {
   DeviceBuffer<float> ln_coef;              //   host coefficients staged for the application
   DeviceBuffer<unsigned char> clip_tables;  // clip integration: frame table, weights, inverses
   DeviceBuffer<float> clip_geometric;       //   host geometric-coverage output staging
+  DeviceBuffer<float> clip_tile_map;        //   tile-local per-pixel multiplier
   DeviceBuffer<double> ncc_ws;              // translation search partials + scores
 
   hipStream_t pick(void* s) const { return s ? static_cast<hipStream_t>(s) : stream; }
@@ -289,6 +290,7 @@ struct gh_stack {
     ln_coef.release();
     clip_tables.release();
     clip_geometric.release();
+    clip_tile_map.release();
     ncc_ws.release();
     if (copy_stream) (void)hipStreamSynchronize(copy_stream);
     for (int i = 0; i < kRawRing; ++i) {

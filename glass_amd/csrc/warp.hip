@@ -831,7 +831,10 @@ __device__ __forceinline__ float clip_sample(const ClipArgs& a, int k, int x, in
 // (every pass samples again; the single-pass weighted mean always, and the
 // sigma clip where LDS occupancy would cost more than sampling again).
 
-template <int SAMPLER, int CAP, int STORE>
+// TILE: tile-local weights.  Table entry k marked in a.target has weight
+// weights[k] * tile_map[pixel] at the pixel; an entry whose weight there is
+// not positive and finite does not count at that pixel (a NaN sample).
+template <int SAMPLER, int CAP, int STORE, bool TILE = false>
 __global__ __launch_bounds__(256) void clip_integrate_kernel(ClipArgs a) {
   extern __shared__ float clip_lds[];
   const int nthreads = STORE == 1 ? static_cast<int>(blockDim.x) : 256;
@@ -841,9 +844,22 @@ __global__ __launch_bounds__(256) void clip_integrate_kernel(ClipArgs a) {
   const int y = static_cast<int>(pp / a.width), x = static_cast<int>(pp - static_cast<long long>(y) * a.width);
   float v[CAP > 0 ? CAP : 1];
   float geometric = 0.0f;
+  float tile_multiplier = 1.0f;
+  if constexpr (TILE) tile_multiplier = ld_global(a.tile_map, pp);
+  auto weight_of = [&](int k) -> float {
+    float w = ld_global(a.weights, k);
+    if constexpr (TILE) {
+      if (a.target[k] != 0) w *= tile_multiplier;
+    }
+    return w;
+  };
   auto sample = [&](int k, float& out) -> bool {
     bool fp;
     out = clip_sample<SAMPLER>(a, k, x, y, fp);
+    if constexpr (TILE) {
+      const float w = weight_of(k);
+      if (!(w > 0.0f) || !isfinite(w)) out = NAN;
+    }
     return fp;
   };
   if constexpr (STORE == 0) {
@@ -886,7 +902,7 @@ __global__ __launch_bounds__(256) void clip_integrate_kernel(ClipArgs a) {
       if (k >= a.n) break;
       const float s = value(k);
       if (!isfinite(s)) continue;
-      const float w = ld_global(a.weights, k);
+      const float w = weight_of(k);
       sum += s * w;
       wsum += w;
       coverage += 1.0f;
@@ -957,7 +973,7 @@ __global__ __launch_bounds__(256) void clip_integrate_kernel(ClipArgs a) {
           high += 1.0f;
           continue;
         }
-        const float w = ld_global(a.weights, k);
+        const float w = weight_of(k);
         sum += s * w;
         wsum += w;
         coverage += 1.0f;
@@ -1140,6 +1156,38 @@ void launch_clip_integrate(const ClipArgs& a, int sampler, hipStream_t stream) {
 #undef GH_CLIP_LDS
 #undef GH_CLIP_AGAIN
   check(hipGetLastError(), "clip integrate launch");
+}
+
+__global__ __launch_bounds__(256) void tile_map_kernel(float* map, const int* extents, const float* multipliers,
+                                                       int tiles, int width) {
+  for (int t = blockIdx.y; t < tiles; t += gridDim.y) {
+    const int x0 = extents[4 * t], y0 = extents[4 * t + 1], x1 = extents[4 * t + 2], y1 = extents[4 * t + 3];
+    const long long tw = x1 - x0, area = tw * (y1 - y0);
+    const float m = multipliers[t];
+    for (long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x; i < area;
+         i += static_cast<long long>(gridDim.x) * 256) {
+      const long long r = i / tw;
+      st_global(map, (y0 + r) * width + x0 + (i - r * tw), m);
+    }
+  }
+}
+
+void launch_tile_map(float* map, const int* extents, const float* multipliers, int tiles, long long max_area,
+                     int width, long long npix, hipStream_t stream) {
+  launch_fill(map, npix, 1.0f, stream);
+  const dim3 grid(static_cast<unsigned>(std::min<long long>((max_area + 255) / 256, 1024)),
+                  static_cast<unsigned>(std::min(tiles, 65535)));
+  tile_map_kernel<<<grid, 256, 0, stream>>>(map, extents, multipliers, tiles, width);
+  check(hipGetLastError(), "tile map launch");
+}
+
+void launch_clip_integrate_tiles(const ClipArgs& a, hipStream_t stream) {
+  const unsigned grid256 = static_cast<unsigned>((a.npix + 255) / 256);
+  if (a.n <= 16) clip_integrate_kernel<0, 16, 0, true><<<grid256, 256, 0, stream>>>(a);
+  else if (a.n <= 64) clip_integrate_kernel<0, 64, 0, true><<<grid256, 256, 0, stream>>>(a);
+  else if (a.n <= 128) clip_integrate_kernel<0, 128, 0, true><<<grid256, 256, 0, stream>>>(a);
+  else clip_integrate_kernel<0, 0, 2, true><<<grid256, 256, 0, stream>>>(a);
+  check(hipGetLastError(), "tile-local integrate launch");
 }
 
 void launch_translate(const float* src, float* dst, int width, int height, bool bilinear, int idx, int idy, float fdx,

@@ -1015,6 +1015,67 @@ class ResidentCalibratedStack:
                                   winsorize, download_mode, geometric=False)
         return master, m["weight"], m["coverage"], m["low"], m["high"]
 
+    def _tile_local(self, target_mask, tile_extents, tile_multipliers, weights, rejection, low_sigma, high_sigma,
+                    winsorize):
+        f = self.frame_count
+        if self.loaded_count != f:
+            raise RuntimeError("all resident frames must be loaded before integration")
+        target = np.ascontiguousarray(np.asarray(target_mask, dtype=np.uint8).reshape((f,)))
+        extents = np.asarray(tile_extents, dtype=np.int32)
+        if extents.ndim != 2 or extents.shape[1] != 4:
+            raise ValueError("tile_extents must have shape (tile_count, 4)")
+        extents = np.ascontiguousarray(extents)
+        multipliers = np.ascontiguousarray(np.asarray(tile_multipliers, dtype=np.float32))
+        if multipliers.ndim != 1 or multipliers.shape[0] != extents.shape[0]:
+            raise ValueError("tile_multipliers must have shape (tile_count,)")
+        w = None
+        if weights is not None:
+            w = np.ascontiguousarray(np.asarray(weights, dtype=np.float32).reshape((f,)))
+        prm = N.ClipParams(N.GH_CLIP_STACK, rejection, int(bool(winsorize)), float(low_sigma), float(high_sigma), -1.0)
+        shape = (self.height, self.width)
+        maps = [np.empty(shape, np.float32) for _ in range(2 if rejection == N.GH_REJECT_NONE else 5)]
+        ptrs = [ctypes.c_void_p(a.ctypes.data) for a in maps] + [None] * (5 - len(maps))
+        t0 = time.perf_counter()
+        N.call("gh_integrate_tile_local", self._h,
+               None if w is None else w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+               target.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+               extents.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+               multipliers.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), int(extents.shape[0]), ctypes.byref(prm),
+               *ptrs, None)
+        tiles = int(extents.shape[0])
+        timing = {"schema_version": 1, "frame_count": f, "target_frame_count": int(np.count_nonzero(target)),
+                  "tile_count": tiles, "total_s": time.perf_counter() - t0, "weights_bytes": f * 4,
+                  "target_mask_bytes": f, "tile_extents_bytes": tiles * 16, "tile_multipliers_bytes": tiles * 4,
+                  "output_bytes": self.pixels_per_frame * 4 * len(maps), "modifies_resident_stack": False}
+        return maps, timing
+
+    def integrate_tile_local_mean(self, target_mask, tile_extents, tile_multipliers, weights=None):
+        """integrate_tile_local_mean (native_bindings.cpp:12998-13225) ->
+        (master, weight_map, timing): the weighted mean of the stack where a
+        target frame's weight is multiplied by the multiplier of the tile that
+        holds the pixel (1 outside every tile)."""
+        maps, timing = self._tile_local(target_mask, tile_extents, tile_multipliers, weights, N.GH_REJECT_NONE,
+                                        3.0, 3.0, False)
+        timing.update(timing_model="native_resident_tile_local_weighted_mean_one_sync", rejection="none")
+        return maps[0], maps[1], timing
+
+    def integrate_tile_local_sigma_clip(self, target_mask, tile_extents, tile_multipliers, weights=None,
+                                        low_sigma=3.0, high_sigma=3.0, winsorize=True):
+        """integrate_tile_local_sigma_clip (native_bindings.cpp:14337-14630)
+        -> (master, weight, coverage, low, high, timing): integrate_sigma_clip
+        with the tile-local weights of integrate_tile_local_mean; a frame whose
+        weight at a pixel is not positive does not enter its statistics."""
+        if self.loaded_count != self.frame_count:
+            raise RuntimeError("all resident frames must be loaded before integration")
+        if not (float(low_sigma) > 0.0 and float(high_sigma) > 0.0):
+            raise ValueError("sigma thresholds must be positive")
+        maps, timing = self._tile_local(target_mask, tile_extents, tile_multipliers, weights,
+                                        N.GH_REJECT_SIGMA_CLIP, low_sigma, high_sigma, winsorize)
+        timing.update(timing_model="native_resident_tile_local_sigma_clip_one_sync",
+                      rejection="winsorized_sigma" if winsorize else "sigma_clip", low_sigma=float(low_sigma),
+                      high_sigma=float(high_sigma), winsorize=bool(winsorize))
+        return (*maps, timing)
+
     def integrate_matrix_warped_mean(self, matrices, weights=None, interpolation="bilinear", clamping_threshold=-1.0,
                                      download_mode="full"):
         """integrate_matrix_warped_mean (native_bindings.cpp:14632-14838) ->

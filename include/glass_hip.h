@@ -247,6 +247,22 @@ int gh_integrate_clip(gh_stack* stack, const float* weights, const double* inver
                       float* master, float* weight, float* coverage, float* low, float* high, float* geometric,
                       void* stream);
 
+/* integrate_tile_local_mean / integrate_tile_local_sigma_clip
+ * (native_bindings.cpp:12998-13225, :14337-14630; kernels
+ * cpp/cuda/integration_kernels.cu:1658-1741, :3622-3838): gh_integrate_clip
+ * over the stack as stored (params->sampler GH_CLIP_STACK) with a per-pixel
+ * weight: a frame f with target_mask[f] != 0 has weight
+ * weights[f] * tile_multipliers[t] at the pixels of tile t (1 outside every
+ * tile), and a frame whose weight at a pixel is not positive and finite does
+ * not enter that pixel's statistics.  tile_extents[t] = (x0, y0, x1, y1), a
+ * half-open rectangle inside the frame; tiles must not overlap; multipliers
+ * finite and >= 0; target_mask must select a frame.  Outputs as in
+ * gh_integrate_clip (the reference's mean returns master and weight only). */
+int gh_integrate_tile_local(gh_stack* stack, const float* weights, const uint8_t* target_mask,
+                            const int32_t* tile_extents, const float* tile_multipliers, int64_t tile_count,
+                            const gh_clip_params* params, float* master, float* weight, float* coverage, float* low,
+                            float* high, void* stream);
+
 /* frame_global_stats(i) (native_bindings.cpp:10569-10643) plus exact median /
  * MAD (src/glass/engine/quality.py:126-200).  FP64. */
 int gh_frame_stats(gh_stack* stack, int64_t index, int32_t exact_median, gh_frame_statistics* out);

@@ -24,7 +24,11 @@ multiply-add), as the library computes them.
 Pinned by (tests/test_oracle_sigma_clip.py): the reference's fixtures
 tests/test_cuda_resident_stack.py:2744-2866 with their values worked out by
 hand, an FP64 brute force per pixel, and the warp-then-integrate composition
-of tests/test_cuda_resident_stack.py:649-682.
+of tests/test_cuda_resident_stack.py:649-682.  The tile-local forms: the
+fixtures of tests/test_cuda_resident_stack.py:470-603 with hand-worked pixel
+values, and the invariants multiplier 1 = the plain integration, multiplier 0
+= the integration without the target frames, multiplier 2 on every frame =
+the same master with twice the weight.
 """
 
 from __future__ import annotations
@@ -42,7 +46,12 @@ def clip_integrate(samples, weights=None, rejection="sigma_clip", low_sigma=3.0,
     x = np.asarray(samples, dtype=np.float32)
     f = x.shape[0]
     w = np.ones(f, np.float32) if weights is None else np.asarray(weights, dtype=np.float32)
-    active = [k for k in range(f) if w[k] > 0.0 and np.isfinite(w[k])]
+    if w.ndim == 1:
+        active = [k for k in range(f) if w[k] > 0.0 and np.isfinite(w[k])]
+    else:
+        # per-pixel weights [F, H, W]: the caller has made every sample whose
+        # weight does not count NaN
+        active = list(range(f))
     shape = x.shape[1:]
     zeros = lambda: np.zeros(shape, np.float32)  # noqa: E731
     out = {k: zeros() for k in ("master", "weight", "coverage", "low", "high")}
@@ -106,6 +115,41 @@ def clip_integrate(samples, weights=None, rejection="sigma_clip", low_sigma=3.0,
     for key, val in (("master", master), ("weight", ws), ("coverage", cov), ("low", nlo), ("high", nhi)):
         out[key] = np.where(has, val, F32(0)).astype(F32)
     return out
+
+
+def tile_multiplier_map(shape, tile_extents, tile_multipliers):
+    """Per-pixel multiplier: that of the first tile (x0, y0, x1, y1), half
+    open, holding the pixel; 1 outside every tile
+    (cpp/cuda/integration_kernels.cu:1678-1690)."""
+    m = np.ones(shape, np.float32)
+    done = np.zeros(shape, bool)
+    for (x0, y0, x1, y1), mult in zip(np.asarray(tile_extents).reshape(-1, 4), np.asarray(tile_multipliers, F32)):
+        region = np.zeros(shape, bool)
+        region[int(y0):int(y1), int(x0):int(x1)] = True
+        region &= ~done
+        m[region] = mult
+        done |= region
+    return m
+
+
+def tile_local_integrate(stack, weights, target_mask, tile_extents, tile_multipliers, rejection="mean",
+                         low_sigma=3.0, high_sigma=3.0, winsorize=True):
+    """integrate_tile_local_mean / integrate_tile_local_sigma_clip
+    (cpp/cuda/integration_kernels.cu:1658-1741, :3622-3795): the weight of
+    frame k at a pixel is weights[k], times the pixel's tile multiplier when
+    target_mask[k] is set (one float32 product); a sample counts only where
+    that weight is positive and finite (in the statistics as well as the
+    weighted sum).  Returns the maps of ``clip_integrate``."""
+    x = np.asarray(stack, dtype=np.float32)
+    f = x.shape[0]
+    w = np.ones(f, np.float32) if weights is None else np.asarray(weights, dtype=np.float32)
+    mult = tile_multiplier_map(x.shape[1:], tile_extents, tile_multipliers)
+    target = np.asarray(target_mask).reshape(f) != 0
+    eff = np.stack([(w[k] * mult).astype(F32) if target[k] else np.full(x.shape[1:], w[k], F32) for k in range(f)])
+    with np.errstate(invalid="ignore"):
+        counts = (eff > 0) & np.isfinite(eff)
+    samples = np.where(counts, x, np.float32(np.nan)).astype(F32)
+    return clip_integrate(samples, eff, rejection, low_sigma, high_sigma, winsorize)
 
 
 def matrix_warped_samples(frames, matrices, interpolation="bilinear", clamping_threshold=-1.0):

@@ -175,3 +175,101 @@ def test_integrations_on_two_streams_keep_their_tables(gpu):
         got_a, got_b = outs[0]["master"].cpu().numpy(), outs[1]["master"].cpu().numpy()
     assert np.array_equal(got_a, exp_a)
     assert np.array_equal(got_b, exp_b)
+
+
+def _tile_case(f, h, w, seed):
+    rng = np.random.default_rng(seed)
+    x = _frames(f, h, w, seed)
+    weights = rng.uniform(0.25, 2.0, f).astype(np.float32)
+    weights[::7] = 0.0
+    weights[3] = np.nan
+    target = (rng.random(f) < 0.5).astype(np.uint8)
+    target[1] = 1
+    # non-overlapping tiles over part of the frame, one of multiplier 0
+    ext = np.array([[0, 0, w // 3, h // 2], [w // 3, 0, w, h // 3], [w // 4, h // 2, w - 1, h],
+                    [0, h // 2, w // 4, h - 1]], np.int32)
+    mult = np.array([2.0, 0.0, 0.375, 1e38], np.float32)
+    return x, weights, target, ext, mult
+
+
+@pytest.mark.filterwarnings("ignore:overflow encountered")  # multiplier 1e38: inf sums on both sides
+@pytest.mark.parametrize("f", [5, 40, 100, 150])  # the register, 128-slot and re-read kernels
+@pytest.mark.parametrize("sigma", [False, True])
+def test_tile_local_matches_the_oracle_bit_exact(gpu, f, sigma):
+    h, w = 37, 53
+    x, weights, target, ext, mult = _tile_case(f, h, w, 100 + f)
+    with _stack_of(x) as s:
+        if sigma:
+            *maps, timing = s.integrate_tile_local_sigma_clip(target, ext, mult, weights, 2.5, 2.0, True)
+        else:
+            *maps, timing = s.integrate_tile_local_mean(target, ext, mult, weights)
+    want = SC.tile_local_integrate(x, weights, target, ext, mult, "sigma_clip" if sigma else "mean", 2.5, 2.0, True)
+    names = MAPS if sigma else ("master", "weight")
+    assert len(maps) == len(names) and timing["tile_count"] == 4
+    assert timing["target_frame_count"] == int(np.count_nonzero(target))
+    for name, got in zip(names, maps):
+        assert np.array_equal(got, want[name], equal_nan=True), name
+
+
+@pytest.mark.parametrize("sigma", [False, True])
+def test_tile_local_reference_fixtures(gpu, sigma):
+    yy, xx = np.indices((5, 6), dtype=np.float32)
+    if sigma:
+        frames = [10.0 + xx * 0.1, 12.0 + yy * 0.2, 50.0 + xx * 0.05]
+    else:
+        frames = [10.0 + xx, 20.0 + yy * 2.0, 30.0 + xx * 0.25 + yy * 0.5]
+    frames = np.stack(frames).astype(np.float32)
+    if sigma:
+        frames[2][0, 0] = 11.0
+    frames[1][2, 3] = np.nan
+    weights = np.array([1.0, 1.0, 0.5], np.float32)
+    target = np.array([0, 1, 0], np.uint8)
+    ext = np.array([[1, 1, 5, 4]], np.int32)
+    mult = np.array([2.0], np.float32)
+    with _stack_of(frames) as s:
+        if sigma:
+            *maps, timing = s.integrate_tile_local_sigma_clip(target, ext, mult, weights, 3.0, 1.0, True)
+            assert timing["timing_model"] == "native_resident_tile_local_sigma_clip_one_sync"
+            assert timing["rejection"] == "winsorized_sigma"
+        else:
+            *maps, timing = s.integrate_tile_local_mean(target, ext, mult, weights)
+            plain_master, plain_weight = s.integrate_mean(weights)
+            assert timing["timing_model"] == "native_resident_tile_local_weighted_mean_one_sync"
+            assert not np.allclose(maps[0][1:4, 1:5], plain_master[1:4, 1:5])
+            assert maps[1][0, 0] == plain_weight[0, 0]
+    assert timing["target_frame_count"] == 1 and timing["tile_count"] == 1
+    assert timing["modifies_resident_stack"] is False
+    want = SC.tile_local_integrate(frames, weights, target, ext, mult, "sigma_clip" if sigma else "mean", 3.0, 1.0,
+                                   True)
+    for name, got in zip(MAPS, maps):
+        assert np.array_equal(got, want[name]), name
+
+
+def test_tile_local_rejects_bad_policies(gpu):
+    x = _frames(3, 6, 8, 1)
+    w = np.ones(3, np.float32)
+    with _stack_of(x) as s:
+        t = np.array([0, 1, 0], np.uint8)
+        one = np.array([1.0], np.float32)
+        with pytest.raises(ValueError, match="select at least one"):
+            s.integrate_tile_local_mean(np.zeros(3, np.uint8), [[0, 0, 2, 2]], one, w)
+        with pytest.raises(ValueError, match=r"shape \(tile_count, 4\)"):
+            s.integrate_tile_local_mean(t, [0, 0, 2, 2], one, w)
+        with pytest.raises(ValueError, match="between 1 and INT_MAX"):
+            s.integrate_tile_local_mean(t, np.zeros((0, 4), np.int32), np.zeros(0, np.float32), w)
+        with pytest.raises(ValueError, match="tile_multipliers must have shape"):
+            s.integrate_tile_local_mean(t, [[0, 0, 2, 2]], [1.0, 2.0], w)
+        for bad in ([[0, 0, 9, 2]], [[2, 0, 2, 2]], [[-1, 0, 2, 2]], [[0, 0, 2, 7]]):
+            with pytest.raises(ValueError, match="half-open rectangles"):
+                s.integrate_tile_local_mean(t, bad, one, w)
+        for bad in (-1.0, np.inf, np.nan):
+            with pytest.raises(ValueError, match="finite non-negative"):
+                s.integrate_tile_local_mean(t, [[0, 0, 2, 2]], [bad], w)
+        with pytest.raises(ValueError, match="must not overlap"):
+            s.integrate_tile_local_mean(t, [[0, 0, 3, 3], [2, 2, 4, 4]], [1.0, 1.0], w)
+        with pytest.raises(ValueError, match="sigma thresholds"):
+            s.integrate_tile_local_sigma_clip(t, [[0, 0, 2, 2]], one, w, 0.0, 3.0)
+        # touching tiles do not overlap; the stack is untouched by either call
+        before = s.download_frame(0)
+        s.integrate_tile_local_sigma_clip(t, [[0, 0, 3, 3], [3, 0, 8, 6]], [1.0, 0.5], w)
+        assert np.array_equal(s.download_frame(0), before, equal_nan=True)

@@ -114,3 +114,92 @@ def test_mean_matches_the_reference_warp_then_integrate_composition():
     np.testing.assert_allclose(out["weight"], wmap, rtol=1e-7)
     # the mean of the unwarped stack differs (the warp is applied)
     assert not np.allclose(out["master"], SC.clip_integrate(np.stack(frames), weights, "mean")["master"])
+
+
+def _tile_fixture(sigma: bool):
+    """Inputs of the reference's tile-local tests
+    (tests/test_cuda_resident_stack.py:470-603)."""
+    yy, xx = np.indices((5, 6), dtype=np.float32)
+    if sigma:
+        frames = [10.0 + xx * 0.1, 12.0 + yy * 0.2, 50.0 + xx * 0.05]
+    else:
+        frames = [10.0 + xx, 20.0 + yy * 2.0, 30.0 + xx * 0.25 + yy * 0.5]
+    frames = [f.astype(np.float32) for f in frames]
+    if sigma:
+        frames[2][0, 0] = 11.0
+    frames[1][2, 3] = np.nan
+    return (np.stack(frames), np.array([1.0, 1.0, 0.5], np.float32), np.array([0, 1, 0], np.uint8),
+            np.array([[1, 1, 5, 4]], np.int32), np.array([2.0], np.float32))
+
+
+def test_tile_local_mean_reference_fixture_by_hand():
+    stack, w, target, ext, mult = _tile_fixture(False)
+    out = SC.tile_local_integrate(stack, w, target, ext, mult, "mean")
+    # outside the tile: (10 + 20 + 0.5 * 30) / 2.5
+    assert out["master"][0, 0] == pytest.approx(18.0, rel=1e-6) and out["weight"][0, 0] == 2.5
+    # (2, 2) inside: frame 1 has weight 2; samples 12, 24, 31.5
+    assert out["master"][2, 2] == pytest.approx((12 + 2 * 24 + 0.5 * 31.5) / 3.5, rel=1e-6)
+    assert out["weight"][2, 2] == 3.5
+    # (2, 3) inside with frame 1 NaN: samples 13, 31.75
+    assert out["master"][2, 3] == pytest.approx((13 + 0.5 * 31.75) / 1.5, rel=1e-6)
+    assert out["weight"][2, 3] == 1.5
+    # row 4 is outside the tile (y1 = 4 is exclusive)
+    assert out["weight"][4, 2] == 2.5 and out["weight"][3, 4] == 3.5 and out["weight"][3, 5] == 2.5
+
+
+def test_tile_local_sigma_reference_fixture_by_hand():
+    stack, w, target, ext, mult = _tile_fixture(True)
+    out = SC.tile_local_integrate(stack, w, target, ext, mult, "sigma_clip", 3.0, 1.0, True)
+    # (0, 0): samples 10, 12, 11; mean 11, std sqrt(2/3); winsorized at
+    # 11 + 0.8165 the clipped mean 10.939 and std 0.743 put the high
+    # threshold at 11.68: 12 is rejected, 10 (w 1) and 11 (w 0.5) kept
+    assert out["master"][0, 0] == pytest.approx(15.5 / 1.5, rel=1e-6)
+    assert (out["weight"][0, 0], out["coverage"][0, 0], out["low"][0, 0], out["high"][0, 0]) == (1.5, 2, 0, 1)
+    # (1, 1) inside the tile: samples 10.1, 12.2, 50.05 with weights 1, 2, 0.5;
+    # the winsorized high threshold (about 36.4) rejects 50.05
+    assert out["master"][1, 1] == pytest.approx((10.1 + 2 * 12.2) / 3.0, rel=1e-6)
+    assert (out["weight"][1, 1], out["coverage"][1, 1], out["high"][1, 1]) == (3.0, 2, 1)
+    # (0, 1) outside: samples 10.1, 12.0, 50.05 with weights 1, 1, 0.5
+    assert out["master"][0, 1] == pytest.approx(11.05, rel=1e-6) and out["weight"][0, 1] == 2.0
+
+
+def _random_tiles(seed, f=7, h=9, w=11):
+    rng = np.random.default_rng(seed)
+    stack = rng.normal(100, 10, (f, h, w)).astype(np.float32)
+    stack[rng.random(stack.shape) < 0.05] = np.nan
+    stack[2, 4, 5] = 400.0
+    weights = rng.uniform(0.5, 2.0, f).astype(np.float32)
+    weights[1] = 0.0
+    target = (rng.random(f) < 0.5).astype(np.uint8)
+    target[0] = 1
+    ext = np.array([[0, 0, 4, 3], [4, 0, 11, 5], [2, 5, 9, 9]], np.int32)
+    return stack, weights, target, ext
+
+
+@pytest.mark.parametrize("rejection", ["mean", "sigma_clip"])
+def test_tile_local_invariants(rejection):
+    stack, weights, target, ext = _random_tiles(5)
+    args = (rejection, 2.5, 2.0, True)
+    plain = SC.clip_integrate(stack, weights, *args)
+    ones = SC.tile_local_integrate(stack, weights, target, ext, np.ones(3, np.float32), *args)
+    for key in plain:
+        assert np.array_equal(ones[key], plain[key]), key
+    # multiplier 0: the target frames drop out of the tile's statistics
+    zero = SC.tile_local_integrate(stack, weights, target, ext, np.array([1.0, 0.0, 1.0], np.float32), *args)
+    without = SC.clip_integrate(stack[target == 0], weights[target == 0], *args)
+    inside = np.zeros(stack.shape[1:], bool)
+    inside[0:5, 4:11] = True
+    for key in plain:
+        assert np.array_equal(zero[key][inside], without[key][inside]), key
+        assert np.array_equal(zero[key][~inside], plain[key][~inside]), key
+    # multiplier 2 on every frame: the same master, twice the weight
+    twice = SC.tile_local_integrate(stack, weights, np.ones(7, np.uint8), ext, np.full(3, 2.0, np.float32), *args)
+    inside_any = SC.tile_multiplier_map(stack.shape[1:], ext, [2, 2, 2]) == 2
+    assert np.array_equal(twice["master"], plain["master"])
+    assert np.array_equal(twice["weight"][inside_any], 2 * plain["weight"][inside_any])
+    assert np.array_equal(twice["weight"][~inside_any], plain["weight"][~inside_any])
+
+
+def test_tile_multiplier_map_first_tile_wins():
+    m = SC.tile_multiplier_map((4, 5), [[0, 0, 3, 2], [1, 1, 5, 4]], [3.0, 7.0])
+    assert m[0, 0] == 3 and m[1, 2] == 3 and m[1, 3] == 7 and m[3, 0] == 1 and m[3, 4] == 7
