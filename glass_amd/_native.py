@@ -34,6 +34,7 @@ GH_REJECT_NONE = 0
 GH_REJECT_WINSORIZED_SIGMA = 1
 GH_REJECT_SIGMA_CLIP = 2
 GH_CLIP_STACK = 0
+GH_CANDIDATES_ALL, GH_CANDIDATES_TOP, GH_CANDIDATES_TOP_NMS, GH_CANDIDATES_GRID_NMS = 0, 1, 2, 3
 
 GH_COUNT_F32 = 0
 GH_COUNT_U16 = 1
@@ -87,6 +88,32 @@ class ClipParams(ctypes.Structure):
         ("low_sigma", c_float),
         ("high_sigma", c_float),
         ("clamping_threshold", c_float),
+    ]
+
+
+class CandidateParams(ctypes.Structure):
+    _fields_ = [
+        ("mode", c_int32),
+        ("threshold", c_float),
+        ("max_candidates", c_int32),
+        ("max_output_candidates", c_int32),
+        ("grid_cols", c_int32),
+        ("grid_rows", c_int32),
+        ("candidates_per_cell", c_int32),
+        ("min_separation_px", c_float),
+        ("centroid_radius", c_int32),
+        ("centroid_global_mean_background", c_int32),
+    ]
+
+
+class CandidateResult(ctypes.Structure):
+    _fields_ = [
+        ("count", c_int64),
+        ("stored_count", c_int64),
+        ("selected_count", c_int64),
+        ("refined_count", c_int64),
+        ("centroid_background", c_float),
+        ("max_shift_px", c_float),
     ]
 
 
@@ -172,6 +199,9 @@ _SIGNATURES = {
     "gh_integrate_clip": [
         c_void_p, POINTER(c_float), POINTER(c_double), POINTER(ClipParams), c_void_p, c_void_p, c_void_p, c_void_p,
         c_void_p, c_void_p, c_void_p,
+    ],
+    "gh_star_candidates": [
+        c_void_p, c_int64, POINTER(CandidateParams), c_void_p, c_void_p, c_void_p, c_void_p, POINTER(CandidateResult),
     ],
     "gh_integrate_tile_local": [
         c_void_p, POINTER(c_float), POINTER(c_uint8), POINTER(c_int32), POINTER(c_float), c_int64,

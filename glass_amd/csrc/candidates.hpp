@@ -1,0 +1,45 @@
+// Star candidate catalogues (GPU ranking of 3x3 local maxima, per-cell top-k,
+// window centroids); the greedy separation filter runs on the host over the
+// ranked prefix.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+#include "common.hpp"
+
+namespace gh {
+
+constexpr int kCentroidMaxRadius = 8;
+constexpr int kCentroidMaxSamples = (2 * kCentroidMaxRadius + 1) * (2 * kCentroidMaxRadius + 1);
+
+// keys[slot] = rank key of every local maximum (slot < capacity); *count
+// counts all of them (must start at 0).
+void launch_local_max_keys(const float* frame, int height, int width, float threshold, unsigned long long* keys,
+                           unsigned capacity, unsigned* count, hipStream_t stream);
+size_t sort_keys_temp_bytes(unsigned n);
+// Ascending key sort = rank order; position_only sorts by pixel index (raster
+// order).
+void sort_keys(void* temp, size_t temp_bytes, const unsigned long long* in, unsigned long long* out, unsigned n,
+               bool position_only, hipStream_t stream);
+// (x, y, flux) of keys[0..n).
+void launch_decode_keys(const unsigned long long* keys, unsigned n, const float* frame, int width, float* xs, float* ys,
+                        float* fluxes, hipStream_t stream);
+
+struct GridSelectScratch {
+  unsigned *cell, *pos, *cell_sorted, *pos_sorted, *start;  // n, n, n, n, cells
+  unsigned char* keep;                                      // n
+  void* temp;
+  size_t temp_bytes;                                        // >= grid_select_temp_bytes
+};
+size_t grid_select_temp_bytes(unsigned n, unsigned cells);
+// out = the ranked keys that are among the first per_cell of their grid cell,
+// in rank order; *out_count their number.
+void grid_select(const unsigned long long* ranked, unsigned n, int width, int height, int cols, int rows,
+                 unsigned per_cell, const GridSelectScratch& w, unsigned long long* out, unsigned* out_count,
+                 hipStream_t stream);
+void launch_refine_centroids(const float* frame, int width, int height, float* xs, float* ys, const float* fluxes,
+                             unsigned char* status, int count, int radius, float background, hipStream_t stream);
+
+}  // namespace gh

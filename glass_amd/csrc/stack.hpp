@@ -138,6 +138,12 @@ struct gh_stack {
   DeviceBuffer<unsigned char> clip_tables;  // clip integration: frame table, weights, inverses
   DeviceBuffer<float> clip_geometric;       //   host geometric-coverage output staging
   DeviceBuffer<float> clip_tile_map;        //   tile-local per-pixel multiplier
+  // star candidate catalogues: rank keys, sorted keys, per-cell top-k, cell
+  // scratch, sort temporaries, decoded list, centroid status, counters
+  DeviceBuffer<unsigned long long> cand_keys, cand_sorted, cand_selected;
+  DeviceBuffer<unsigned> cand_u32, cand_count;
+  DeviceBuffer<unsigned char> cand_keep, cand_temp, cand_status;
+  DeviceBuffer<float> cand_xyf;
   DeviceBuffer<double> ncc_ws;              // translation search partials + scores
 
   hipStream_t pick(void* s) const { return s ? static_cast<hipStream_t>(s) : stream; }
@@ -291,6 +297,15 @@ struct gh_stack {
     clip_tables.release();
     clip_geometric.release();
     clip_tile_map.release();
+    cand_keys.release();
+    cand_sorted.release();
+    cand_selected.release();
+    cand_u32.release();
+    cand_count.release();
+    cand_keep.release();
+    cand_temp.release();
+    cand_status.release();
+    cand_xyf.release();
     ncc_ws.release();
     if (copy_stream) (void)hipStreamSynchronize(copy_stream);
     for (int i = 0; i < kRawRing; ++i) {

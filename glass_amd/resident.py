@@ -1015,6 +1015,131 @@ class ResidentCalibratedStack:
                                   winsorize, download_mode, geometric=False)
         return master, m["weight"], m["coverage"], m["low"], m["high"]
 
+    # ---- star candidate catalogues (native_bindings.cpp:11356-12122)
+    def _candidates(self, index, mode, threshold, max_candidates=0, max_output_candidates=0, grid=(0, 0, 0),
+                    min_separation_px=0.0, centroid_radius=0, centroid_global_mean_background=False):
+        prm = N.CandidateParams(mode, float(threshold), int(max_candidates), int(max_output_candidates), int(grid[0]),
+                                int(grid[1]), int(grid[2]), float(min_separation_px), int(centroid_radius or 0),
+                                int(bool(centroid_global_mean_background)))
+        cap = max(1, int(max_candidates) if mode in (N.GH_CANDIDATES_ALL, N.GH_CANDIDATES_TOP)
+                  else int(max_output_candidates))
+        xs, ys, fl = (np.zeros(cap, np.float32) for _ in range(3))
+        status = np.zeros(cap, np.uint8)
+        res = N.CandidateResult()
+        N.call("gh_star_candidates", self._h, int(index), ctypes.byref(prm), ctypes.c_void_p(xs.ctypes.data),
+               ctypes.c_void_p(ys.ctypes.data), ctypes.c_void_p(fl.ctypes.data), ctypes.c_void_p(status.ctypes.data),
+               ctypes.byref(res))
+        k = int(res.stored_count)
+        out = {"count": int(res.count), "stored_count": k}
+        if centroid_radius is not None and mode in (N.GH_CANDIDATES_TOP_NMS, N.GH_CANDIDATES_GRID_NMS):
+            on = int(centroid_radius) > 0
+            bg = float(res.centroid_background)
+            out["centroid_refine"] = {
+                "enabled": on, "radius": int(centroid_radius), "stored_count": k,
+                "refined_count": int(res.refined_count) if on else 0,
+                "failed_count": k - int(res.refined_count) if on else 0,
+                "max_shift_px": float(res.max_shift_px),
+                "mode": ("resident_gpu_global_mean_centroid" if centroid_global_mean_background
+                         else "resident_gpu_window_centroid") if on else "off",
+                "background_mode": ("global_mean" if centroid_global_mean_background else "local_median")
+                if on else "off",
+                "background": bg if np.isfinite(bg) else None,
+                "status": status[:k].copy() if on else None}
+        out.update(x=xs[:k].copy(), y=ys[:k].copy(), flux=fl[:k].copy())
+        return out, int(res.selected_count)
+
+    def star_candidates(self, index, threshold, max_candidates):
+        """star_candidates (native_bindings.cpp:11356-11418): the 3x3 local
+        maxima above `threshold`; the first max_candidates in raster order."""
+        if int(max_candidates) <= 0:
+            raise ValueError("max_candidates must be positive")
+        return self._candidates(index, N.GH_CANDIDATES_ALL, threshold, max_candidates)[0]
+
+    def star_top_candidates(self, index, threshold, max_candidates):
+        """star_top_candidates (native_bindings.cpp:11420-11487): the best
+        max_candidates local maxima by flux (descending), then y, then x."""
+        if int(max_candidates) <= 0:
+            raise ValueError("max_candidates must be positive")
+        return self._candidates(index, N.GH_CANDIDATES_TOP, threshold, max_candidates)[0]
+
+    def star_top_nms_candidates(self, index, threshold, scan_candidates, max_output_candidates, min_separation_px):
+        """star_top_nms_candidates (native_bindings.cpp:11489-11615): the best
+        scan_candidates, then greedy separation >= min_separation_px."""
+        out, _ = self._candidates(index, N.GH_CANDIDATES_TOP_NMS, threshold, scan_candidates, max_output_candidates,
+                                  min_separation_px=min_separation_px, centroid_radius=None)
+        out.update(scan_candidates=int(scan_candidates), max_output_candidates=int(max_output_candidates),
+                   min_separation_px=float(min_separation_px))
+        return {k: out[k] for k in ("count", "stored_count", "scan_candidates", "max_output_candidates",
+                                    "min_separation_px", "x", "y", "flux")}
+
+    def star_top_nms_candidates_centroid(self, index, threshold, scan_candidates, max_output_candidates,
+                                         min_separation_px, centroid_radius, centroid_global_mean_background=False):
+        """star_top_nms_candidates_centroid (native_bindings.cpp:11617-11806):
+        star_top_nms_candidates with window centroids of the kept positions."""
+        out, _ = self._candidates(index, N.GH_CANDIDATES_TOP_NMS, threshold, scan_candidates, max_output_candidates,
+                                  min_separation_px=min_separation_px, centroid_radius=centroid_radius,
+                                  centroid_global_mean_background=centroid_global_mean_background)
+        out.update(scan_candidates=int(scan_candidates), max_output_candidates=int(max_output_candidates),
+                   min_separation_px=float(min_separation_px))
+        return out
+
+    def _grid_candidates(self, index, threshold, grid_cols, grid_rows, candidates_per_cell, max_output_candidates,
+                         min_separation_px, deterministic, centroid_radius, centroid_global_mean_background):
+        out, selected = self._candidates(index, N.GH_CANDIDATES_GRID_NMS, threshold, 0, max_output_candidates,
+                                         (grid_cols, grid_rows, candidates_per_cell), min_separation_px,
+                                         centroid_radius, centroid_global_mean_background)
+        cells = int(grid_cols) * int(grid_rows)
+        out.update(grid_cols=int(grid_cols), grid_rows=int(grid_rows), cell_count=cells,
+                   candidates_per_cell=int(candidates_per_cell), grid_capacity=cells * int(candidates_per_cell),
+                   max_output_candidates=int(max_output_candidates), min_separation_px=float(min_separation_px),
+                   deterministic=bool(deterministic), cell_selected_count=selected,
+                   catalog_sort_mode="device_radix_rank_keys", catalog_topk_mode="stable_cell_rank_cut")
+        return out
+
+    def star_grid_top_nms_candidates(self, index, threshold, grid_cols, grid_rows, candidates_per_cell,
+                                     max_output_candidates, min_separation_px, deterministic=False):
+        """star_grid_top_nms_candidates[_deterministic]
+        (native_bindings.cpp:11808-12109): the best candidates_per_cell of each
+        grid cell, ranked together, then greedy separation.  The ranking here
+        is deterministic in both forms."""
+        return self._grid_candidates(index, threshold, grid_cols, grid_rows, candidates_per_cell,
+                                     max_output_candidates, min_separation_px, deterministic, 0, False)
+
+    def star_grid_top_nms_candidates_deterministic(self, index, threshold, grid_cols, grid_rows, candidates_per_cell,
+                                                   max_output_candidates, min_separation_px):
+        return self.star_grid_top_nms_candidates(index, threshold, grid_cols, grid_rows, candidates_per_cell,
+                                                 max_output_candidates, min_separation_px, deterministic=True)
+
+    def star_grid_top_nms_candidates_centroid(self, index, threshold, grid_cols, grid_rows, candidates_per_cell,
+                                              max_output_candidates, min_separation_px, centroid_radius,
+                                              centroid_global_mean_background=False, deterministic=False):
+        """star_grid_top_nms_candidates[_deterministic]_centroid
+        (native_bindings.cpp:12065-12122): the grid catalogue with window
+        centroids of the kept positions."""
+        return self._grid_candidates(index, threshold, grid_cols, grid_rows, candidates_per_cell,
+                                     max_output_candidates, min_separation_px, deterministic, centroid_radius,
+                                     centroid_global_mean_background)
+
+    def star_grid_top_nms_candidates_deterministic_centroid(self, index, threshold, grid_cols, grid_rows,
+                                                            candidates_per_cell, max_output_candidates,
+                                                            min_separation_px, centroid_radius,
+                                                            centroid_global_mean_background=False):
+        return self.star_grid_top_nms_candidates_centroid(index, threshold, grid_cols, grid_rows, candidates_per_cell,
+                                                          max_output_candidates, min_separation_px, centroid_radius,
+                                                          centroid_global_mean_background, deterministic=True)
+
+    def star_grid_top_nms_candidates_batch(self, indices, threshold, grid_cols, grid_rows, candidates_per_cell,
+                                           max_output_candidates, min_separation_px, deterministic=False):
+        """star_grid_top_nms_candidates_batch (native_bindings.cpp:12123-12644):
+        the per-frame catalogues of `indices`, each item tagged with its frame."""
+        out = []
+        for i in indices:
+            item = self.star_grid_top_nms_candidates(i, threshold, grid_cols, grid_rows, candidates_per_cell,
+                                                     max_output_candidates, min_separation_px, deterministic)
+            item["frame_index"] = int(i)
+            out.append(item)
+        return out
+
     def _tile_local(self, target_mask, tile_extents, tile_multipliers, weights, rejection, low_sigma, high_sigma,
                     winsorize):
         f = self.frame_count

@@ -263,6 +263,59 @@ int gh_integrate_tile_local(gh_stack* stack, const float* weights, const uint8_t
                             const gh_clip_params* params, float* master, float* weight, float* coverage, float* low,
                             float* high, void* stream);
 
+/* Star candidate catalogues of resident frame `index`
+ * (native_bindings.cpp:11356-12122; kernels
+ * cpp/cuda/star_detect_kernels.cu:41-742).  Candidates are interior pixels,
+ * finite, above `threshold` and not below any 8-neighbour; they rank by flux
+ * (descending), then y, then x.
+ *   GH_CANDIDATES_ALL       star_candidates: the first max_candidates in
+ *                           raster order (the reference's order is that of
+ *                           its atomic appends);
+ *   GH_CANDIDATES_TOP       star_top_candidates: the best max_candidates in
+ *                           rank order;
+ *   GH_CANDIDATES_TOP_NMS   star_top_nms_candidates[_centroid]: the best
+ *                           max_candidates (= scan_candidates), then greedily
+ *                           those at squared distance >= min_separation_px^2
+ *                           (float32) from every one kept, up to
+ *                           max_output_candidates;
+ *   GH_CANDIDATES_GRID_NMS  star_grid_top_nms_candidates[_deterministic]
+ *                           [_centroid]: the best candidates_per_cell of each
+ *                           cell (min(x * cols / W, cols - 1), same for y) in
+ *                           rank order, then the same greedy filter.
+ * centroid_radius > 0 moves each output position to its window centroid
+ * (radius clamped to 1..8; background the window's float32 median of finite
+ * samples, or the frame's finite mean with centroid_global_mean_background;
+ * FP64 weight sums) and writes centroid_status (1 = moved; may be NULL).
+ * x, y, flux hold max_candidates (ALL, TOP) or max_output_candidates entries.
+ * Synchronous. */
+#define GH_CANDIDATES_ALL 0
+#define GH_CANDIDATES_TOP 1
+#define GH_CANDIDATES_TOP_NMS 2
+#define GH_CANDIDATES_GRID_NMS 3
+
+typedef struct gh_candidate_params {
+  int32_t mode;
+  float threshold;
+  int32_t max_candidates;          /* ALL / TOP: catalogue size; TOP_NMS: scan_candidates */
+  int32_t max_output_candidates;   /* the NMS modes */
+  int32_t grid_cols, grid_rows, candidates_per_cell;
+  float min_separation_px;
+  int32_t centroid_radius;         /* 0: off */
+  int32_t centroid_global_mean_background;
+} gh_candidate_params;
+
+typedef struct gh_candidate_result {
+  int64_t count;                   /* local maxima above the threshold */
+  int64_t stored_count;            /* entries written */
+  int64_t selected_count;          /* ranked entries scanned by the separation filter */
+  int64_t refined_count;           /* centroids moved */
+  float centroid_background;       /* global-mean background used, else NaN */
+  float max_shift_px;              /* largest centroid move */
+} gh_candidate_result;
+
+int gh_star_candidates(gh_stack* stack, int64_t index, const gh_candidate_params* params, float* x, float* y,
+                       float* flux, uint8_t* centroid_status, gh_candidate_result* result);
+
 /* frame_global_stats(i) (native_bindings.cpp:10569-10643) plus exact median /
  * MAD (src/glass/engine/quality.py:126-200).  FP64. */
 int gh_frame_stats(gh_stack* stack, int64_t index, int32_t exact_median, gh_frame_statistics* out);
