@@ -117,6 +117,28 @@ class CandidateResult(ctypes.Structure):
     ]
 
 
+class StarTranslationParams(ctypes.Structure):
+    _fields_ = [
+        ("threshold", c_float),
+        ("max_candidates", c_int32),
+        ("tolerance_px", c_float),
+        ("max_abs_dx", c_float),
+        ("max_abs_dy", c_float),
+        ("prior_dx", c_float),
+        ("prior_dy", c_float),
+        ("prior_radius_px", c_float),
+        ("grid_cols", c_int32),
+        ("grid_rows", c_int32),
+    ]
+
+
+class StarTranslationResult(ctypes.Structure):
+    _fields_ = [(name, c_float) for name in ("dx", "dy", "refined_dx", "refined_dy", "rms_px", "max_abs_dx",
+                                             "max_abs_dy", "prior_radius_px")] + \
+        [(name, c_int64) for name in ("inliers", "mutual_inliers", "reference_count", "moving_count",
+                                      "reference_total_count", "moving_total_count")]
+
+
 class FrameStatistics(ctypes.Structure):
     _fields_ = [
         ("mean", c_double),
@@ -202,6 +224,9 @@ _SIGNATURES = {
     ],
     "gh_star_candidates": [
         c_void_p, c_int64, POINTER(CandidateParams), c_void_p, c_void_p, c_void_p, c_void_p, POINTER(CandidateResult),
+    ],
+    "gh_estimate_translation_from_stars": [
+        c_void_p, c_int64, c_int64, POINTER(StarTranslationParams), POINTER(StarTranslationResult),
     ],
     "gh_integrate_tile_local": [
         c_void_p, POINTER(c_float), POINTER(c_uint8), POINTER(c_int32), POINTER(c_float), c_int64,

@@ -187,6 +187,34 @@ __global__ __launch_bounds__(64) void refine_centroids_kernel(const float* __res
   status[i] = 1;
 }
 
+// catalog_offset_score_kernel (registration_kernels.cu:306-335): for each
+// finite offset, the number of finite offsets (itself included) within
+// float32 squared distance tol2; -1 for a non-finite one.  Offsets are staged
+// through LDS 256 at a time.
+__global__ __launch_bounds__(256) void offset_scores_kernel(const float* __restrict__ dx, const float* __restrict__ dy,
+                                                            int n, float tol2, int* scores) {
+  __shared__ float sx[256], sy[256];
+  const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+  const float mx = i < n ? dx[i] : NAN, my = i < n ? dy[i] : NAN;
+  int score = 0;
+  for (int base = 0; base < n; base += 256) {
+    const int j = base + static_cast<int>(threadIdx.x);
+    sx[threadIdx.x] = j < n ? dx[j] : NAN;
+    sy[threadIdx.x] = j < n ? dy[j] : NAN;
+    __syncthreads();
+    const int m = min(256, n - base);
+    for (int t = 0; t < m; ++t) {
+      const float ox = sx[t], oy = sy[t];
+      if (!isfinite(ox) || !isfinite(oy)) continue;
+      const float ex = ox - mx, ey = oy - my;
+      const float e2x = ex * ex, e2y = ey * ey;
+      score += (e2x + e2y <= tol2) ? 1 : 0;
+    }
+    __syncthreads();
+  }
+  if (i < n) scores[i] = (isfinite(mx) && isfinite(my)) ? score : -1;
+}
+
 unsigned blocks_for(long long n, unsigned threads) {
   return static_cast<unsigned>(std::max<long long>(1, (n + threads - 1) / threads));
 }
@@ -257,6 +285,12 @@ void grid_select(const unsigned long long* ranked, unsigned n, int width, int he
   check(hipGetLastError(), "cell cut launch");
   bytes = w.temp_bytes;
   check(hipcub::DeviceSelect::Flagged(w.temp, bytes, ranked, w.keep, out, out_count, n, stream), "cell select");
+}
+
+void launch_offset_scores(const float* dx, const float* dy, int n, float tol2, int* scores, hipStream_t stream) {
+  if (n <= 0) return;
+  offset_scores_kernel<<<blocks_for(n, 256), 256, 0, stream>>>(dx, dy, n, tol2, scores);
+  check(hipGetLastError(), "offset vote launch");
 }
 
 void launch_refine_centroids(const float* frame, int width, int height, float* xs, float* ys, const float* fluxes,

@@ -39,6 +39,9 @@ size_t grid_select_temp_bytes(unsigned n, unsigned cells);
 void grid_select(const unsigned long long* ranked, unsigned n, int width, int height, int cols, int rows,
                  unsigned per_cell, const GridSelectScratch& w, unsigned long long* out, unsigned* out_count,
                  hipStream_t stream);
+// scores[i] = #finite offsets within squared distance tol2 of offset i (-1
+// when offset i is not finite).
+void launch_offset_scores(const float* dx, const float* dy, int n, float tol2, int* scores, hipStream_t stream);
 void launch_refine_centroids(const float* frame, int width, int height, float* xs, float* ys, const float* fluxes,
                              unsigned char* status, int count, int radius, float background, hipStream_t stream);
 

@@ -1140,6 +1140,37 @@ class ResidentCalibratedStack:
             out.append(item)
         return out
 
+    def estimate_translation_from_stars_to_reference(self, reference_index, moving_index, threshold,
+                                                     max_candidates=64, tolerance_px=1.0, max_abs_dx=-1.0,
+                                                     max_abs_dy=None, prior_dx=0.0, prior_dy=0.0,
+                                                     prior_radius_px=-1.0, grid_cols=0, grid_rows=0):
+        """estimate_translation_from_stars_to_reference
+        (native_bindings.cpp:12645-12880): reference - moving offset voted by
+        the pair offsets of the two frames' star catalogues, refined by the
+        mutual nearest matches under it."""
+        prm = N.StarTranslationParams(float(threshold), int(max_candidates), float(tolerance_px), float(max_abs_dx),
+                                      -1.0 if max_abs_dy is None else float(max_abs_dy), float(prior_dx),
+                                      float(prior_dy), float(prior_radius_px), int(grid_cols), int(grid_rows))
+        res = N.StarTranslationResult()
+        N.call("gh_estimate_translation_from_stars", self._h, int(reference_index), int(moving_index),
+               ctypes.byref(prm), ctypes.byref(res))
+        grid = int(grid_cols) > 0 or int(grid_rows) > 0
+        rn, mn = int(res.reference_count), int(res.moving_count)
+        return {"dx": float(res.dx), "dy": float(res.dy), "inliers": int(res.inliers),
+                "refined_dx": float(res.refined_dx), "refined_dy": float(res.refined_dy),
+                "mutual_inliers": int(res.mutual_inliers), "rms_px": float(res.rms_px),
+                "candidate_count": rn * mn, "reference_count": rn, "moving_count": mn,
+                "reference_total_count": int(res.reference_total_count),
+                "moving_total_count": int(res.moving_total_count), "threshold": float(threshold),
+                "max_candidates": int(max_candidates),
+                "catalog_capacity": int(grid_cols) * int(grid_rows) if grid else int(max_candidates),
+                "candidate_selection": "grid_brightest_per_cell" if grid else "top_flux",
+                "grid_cols": int(grid_cols), "grid_rows": int(grid_rows), "tolerance_px": float(tolerance_px),
+                "max_abs_dx": float(res.max_abs_dx), "max_abs_dy": float(res.max_abs_dy),
+                "prior_dx": float(prior_dx), "prior_dy": float(prior_dy),
+                "prior_radius_px": float(res.prior_radius_px), "reference_index": int(reference_index),
+                "moving_index": int(moving_index), "model": "resident_star_catalog_pair_offset_translation"}
+
     def _tile_local(self, target_mask, tile_extents, tile_multipliers, weights, rejection, low_sigma, high_sigma,
                     winsorize):
         f = self.frame_count

@@ -316,6 +316,36 @@ typedef struct gh_candidate_result {
 int gh_star_candidates(gh_stack* stack, int64_t index, const gh_candidate_params* params, float* x, float* y,
                        float* flux, uint8_t* centroid_status, gh_candidate_result* result);
 
+/* estimate_translation_from_stars_to_reference (native_bindings.cpp:12645-
+ * 12880; kernels cpp/cuda/registration_kernels.cu:265-520, :2007-2110):
+ * reference - moving translation from star catalogues of both frames (the
+ * best max_candidates, or with grid_cols / grid_rows > 0 the brightest
+ * candidate of each cell in rank order, padded with (0, 0) entries to
+ * min(count, cells) like the reference's cell slots).  Every pair offset
+ * inside |dx| <= max_abs_dx, |dy| <= max_abs_dy (< 0: no limit; max_abs_dy
+ * < 0 takes max_abs_dx) and within prior_radius_px of the prior (< 0: none)
+ * votes; the offset with the most offsets within tolerance_px (float32
+ * squared distances; first in pair order) is (dx, dy).  Mutual nearest
+ * matches within tolerance under it give refined_dx / refined_dy (their mean
+ * offset; the best offset when there are none) and rms_px.  Throws when a
+ * catalogue is empty.  Synchronous. */
+typedef struct gh_star_translation_params {
+  float threshold;
+  int32_t max_candidates;
+  float tolerance_px, max_abs_dx, max_abs_dy, prior_dx, prior_dy, prior_radius_px;
+  int32_t grid_cols, grid_rows;
+} gh_star_translation_params;
+
+typedef struct gh_star_translation_result {
+  float dx, dy, refined_dx, refined_dy, rms_px;
+  float max_abs_dx, max_abs_dy, prior_radius_px;  /* after the defaults above */
+  int64_t inliers, mutual_inliers;
+  int64_t reference_count, moving_count, reference_total_count, moving_total_count;
+} gh_star_translation_result;
+
+int gh_estimate_translation_from_stars(gh_stack* stack, int64_t reference_index, int64_t moving_index,
+                                       const gh_star_translation_params* params, gh_star_translation_result* result);
+
 /* frame_global_stats(i) (native_bindings.cpp:10569-10643) plus exact median /
  * MAD (src/glass/engine/quality.py:126-200).  FP64. */
 int gh_frame_stats(gh_stack* stack, int64_t index, int32_t exact_median, gh_frame_statistics* out);

@@ -128,3 +128,71 @@ def test_candidate_errors_and_empty_frames(gpu):
     with ResidentCalibratedStack(2, 9, 9) as s:
         with pytest.raises(RuntimeError, match="loaded before star detection"):
             s.star_top_candidates(1, 1.0, 3)
+
+
+def _star_field():
+    image = np.zeros((96, 112), np.float32)
+    yy, xx = np.indices(image.shape, dtype=np.float32)
+    for x, y, flux in [(12, 17, 100.0), (30, 42, 220.0), (71, 15, 160.0), (88, 63, 180.0), (45, 79, 250.0),
+                       (19, 86, 130.0), (101, 33, 145.0)]:
+        image += flux * np.exp(-(((xx - x) ** 2 + (yy - y) ** 2) / (2.0 * 1.4 ** 2)))
+    image += 10.0 + 0.01 * xx + 0.02 * yy
+    return image.astype(np.float32)
+
+
+def _shift(data, dx, dy):
+    out = np.zeros_like(data)
+    h, w = data.shape
+    out[max(0, dy):min(h, h + dy), max(0, dx):min(w, w + dx)] = \
+        data[max(0, -dy):min(h, h - dy), max(0, -dx):min(w, w - dx)]
+    return out
+
+
+KEYS = ("dx", "dy", "inliers", "refined_dx", "refined_dy", "mutual_inliers", "rms_px")
+
+
+@pytest.mark.parametrize("grid", [False, True])
+def test_star_translation_reference_case(gpu, grid):
+    ref = _star_field()
+    mov = _shift(ref, 4, -3)
+    kw = {"grid_cols": 4, "grid_rows": 4} if grid else {}
+    with _stack(ref, mov) as s:
+        r = s.estimate_translation_from_stars_to_reference(0, 1, 30.0, 16, 0.25, 8.0, 8.0, **kw)
+    assert r["model"] == "resident_star_catalog_pair_offset_translation"
+    assert r["candidate_selection"] == ("grid_brightest_per_cell" if grid else "top_flux")
+    assert r["catalog_capacity"] == 16 and (r["dx"], r["dy"]) == (-4.0, 3.0) and r["mutual_inliers"] >= 6
+    assert abs(r["refined_dx"] + 4.0) < 1e-5 and abs(r["refined_dy"] - 3.0) < 1e-5
+    if grid:
+        _, rc = SCand.grid_brightest_catalogue(ref, 30.0, 4, 4)
+        _, mc = SCand.grid_brightest_catalogue(mov, 30.0, 4, 4)
+    else:
+        rc = [(x, y) for x, y, _ in SCand.top_candidates(ref, 30.0, 16)[1]]
+        mc = [(x, y) for x, y, _ in SCand.top_candidates(mov, 30.0, 16)[1]]
+    want = SCand.catalogue_translation(rc, mc, 0.25, 8.0, 8.0)
+    assert tuple(r[k] for k in KEYS) == want
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_star_translation_random_fields(gpu, seed):
+    ref = _field(seed, stars=40, noise=0.5)
+    mov = _shift(ref, -6, 2)
+    with _stack(ref, mov) as s:
+        a = s.estimate_translation_from_stars_to_reference(0, 1, 40.0, 48, 0.5)
+        b = s.estimate_translation_from_stars_to_reference(0, 1, 40.0, 48, 0.5, 10.0, None, 5.0, -2.0, 3.0)
+        c = s.estimate_translation_from_stars_to_reference(0, 1, 40.0, 48, 0.5, grid_cols=5, grid_rows=4)
+        d = s.estimate_translation_from_stars_to_reference(0, 1, 40.0, 48, 0.5, 0.5, 0.5)
+        with pytest.raises(ValueError, match="both be positive"):
+            s.estimate_translation_from_stars_to_reference(0, 1, 40.0, 48, 0.5, grid_cols=3)
+        with pytest.raises(RuntimeError, match="found no stars"):
+            s.estimate_translation_from_stars_to_reference(0, 1, 1e9, 48, 0.5)
+    rc = [(x, y) for x, y, _ in SCand.top_candidates(ref, 40.0, 48)[1]]
+    mc = [(x, y) for x, y, _ in SCand.top_candidates(mov, 40.0, 48)[1]]
+    assert tuple(a[k] for k in KEYS) == SCand.catalogue_translation(rc, mc, 0.5)
+    assert (a["dx"], a["dy"]) == (6.0, -2.0)
+    assert tuple(b[k] for k in KEYS) == SCand.catalogue_translation(rc, mc, 0.5, 10.0, 10.0, 5.0, -2.0, 3.0)
+    assert b["max_abs_dy"] == 10.0
+    _, rg = SCand.grid_brightest_catalogue(ref, 40.0, 5, 4)
+    _, mg = SCand.grid_brightest_catalogue(mov, 40.0, 5, 4)
+    assert tuple(c[k] for k in KEYS) == SCand.catalogue_translation(rg, mg, 0.5)
+    got_d, want_d = tuple(d[k] for k in KEYS), SCand.catalogue_translation(rc, mc, 0.5, 0.5, 0.5)
+    assert np.array_equal(np.array(got_d, float), np.array(want_d, float), equal_nan=True)

@@ -79,3 +79,61 @@ def test_centroid_closed_forms():
     # a finite override replaces the median; one at the peak leaves no weight
     assert SCand.refine_centroid(two, 10.0, 4.0, 3.0, 2, background=3.0)[2] is False
     assert SCand.refine_centroid(two, 10.0, 4.0, -1e31, 2)[2] is False
+
+
+def _star_field():
+    """tests/test_cuda_resident_stack.py:45-59 (the reference's star field)."""
+    image = np.zeros((96, 112), np.float32)
+    yy, xx = np.indices(image.shape, dtype=np.float32)
+    for x, y, flux in [(12, 17, 100.0), (30, 42, 220.0), (71, 15, 160.0), (88, 63, 180.0), (45, 79, 250.0),
+                       (19, 86, 130.0), (101, 33, 145.0)]:
+        image += flux * np.exp(-(((xx - x) ** 2 + (yy - y) ** 2) / (2.0 * 1.4 ** 2)))
+    image += 10.0 + 0.01 * xx + 0.02 * yy
+    return image.astype(np.float32)
+
+
+def _shift(data, dx, dy):
+    out = np.zeros_like(data)
+    h, w = data.shape
+    out[max(0, dy):min(h, h + dy), max(0, dx):min(w, w + dx)] = \
+        data[max(0, -dy):min(h, h - dy), max(0, -dx):min(w, w - dx)]
+    return out
+
+
+def test_catalogue_translation_reference_star_field():
+    ref = _star_field()
+    mov = _shift(ref, 4, -3)
+    _, rc = SCand.top_candidates(ref, 30.0, 16)
+    _, mc = SCand.top_candidates(mov, 30.0, 16)
+    dx, dy, inl, rdx, rdy, mutual, rms = SCand.catalogue_translation(
+        [(x, y) for x, y, _ in rc], [(x, y) for x, y, _ in mc], 0.25, 8.0, 8.0)
+    # the 7 mutual matches all offset by exactly (-4, 3); the kernel's mean is
+    # sum * (1 / 7) in float32: -28 * f32(1/7) rounds to -4, 21 * f32(1/7) to
+    # the float above 3, and the rms is that last-bit residual
+    assert (dx, dy, mutual, inl) == (-4.0, 3.0, 7, 7)
+    assert rdx == -4.0 and rdy == float(np.float32(21) * (np.float32(1) / np.float32(7))) and rms < 1e-6
+    _, rg = SCand.grid_brightest_catalogue(ref, 30.0, 4, 4)
+    _, mg = SCand.grid_brightest_catalogue(mov, 30.0, 4, 4)
+    assert len(rg) == len(mg) == 7
+    out = SCand.catalogue_translation(rg, mg, 0.25, 8.0, 8.0)
+    assert out[:3] == (-4.0, 3.0, 7) and out[5] == 7 and abs(out[4] - 3.0) < 1e-5
+
+
+def test_catalogue_translation_by_hand():
+    ref = [(10.0, 10.0), (30.0, 12.0), (50.0, 40.0), (5.0, 60.0)]
+    mov = [(8.0, 11.0), (28.0, 13.0), (48.0, 41.0), (70.0, 70.0)]   # shift (2, -1); one stray star
+    dx, dy, inl, rdx, rdy, mutual, rms = SCand.catalogue_translation(ref, mov, 0.5)
+    assert (dx, dy, inl, rdx, rdy, mutual, rms) == (2.0, -1.0, 3, 2.0, -1.0, 3, 0.0)
+    # |dy| <= 0.5 admits no pair: no offset, no refinement
+    dx, dy, inl, rdx, rdy, mutual, rms = SCand.catalogue_translation(ref, mov, 0.5, max_abs_dx=100.0, max_abs_dy=0.5)
+    assert np.isnan(dx) and np.isnan(rdx) and np.isnan(rms) and (inl, mutual) == (0, 0)
+    # a prior of radius 0.5 around (2, -1) keeps exactly the 3 true pairs
+    assert SCand.catalogue_translation(ref, mov, 0.5, prior_dx=2.0, prior_dy=-1.0, prior_radius_px=0.5)[:3] == \
+        (2.0, -1.0, 3)
+    # a prior far from every offset leaves none
+    assert np.isnan(SCand.catalogue_translation(ref, mov, 0.5, prior_dx=-90.0, prior_dy=0.0, prior_radius_px=3.0)[0])
+    # a jitter of 0.25 px on one star: the mean of the three mutual offsets
+    jit = [(8.25, 11.0)] + mov[1:]
+    out = SCand.catalogue_translation(ref, jit, 0.5)
+    assert out[5] == 3 and out[3] == np.float32((np.float32(1.75) + np.float32(2.0) + np.float32(2.0))
+                                                * (np.float32(1) / np.float32(3)))
