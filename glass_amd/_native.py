@@ -139,6 +139,18 @@ class StarTranslationResult(ctypes.Structure):
                                       "reference_total_count", "moving_total_count")]
 
 
+class AlignSearchParams(ctypes.Structure):
+    _fields_ = [("search_radius_px", c_float), ("coarse_step_px", c_float), ("fine_radius_px", c_float),
+                ("fine_step_px", c_float), ("coarse_sample_stride", c_int32), ("final_sample_stride", c_int32)]
+
+
+class AlignCandidate(ctypes.Structure):
+    _fields_ = [("matrix", c_float * 9), ("dx", c_float), ("dy", c_float), ("valid_pixels", c_int64),
+                ("sampled_pixels", c_int64), ("sample_stride", c_int32), ("rms", c_double),
+                ("mean_abs_diff", c_double), ("ncc", c_double), ("coarse_candidates", c_int64),
+                ("fine_candidates", c_int64)]
+
+
 class FrameStatistics(ctypes.Structure):
     _fields_ = [
         ("mean", c_double),
@@ -227,6 +239,10 @@ _SIGNATURES = {
     ],
     "gh_estimate_translation_from_stars": [
         c_void_p, c_int64, c_int64, POINTER(StarTranslationParams), POINTER(StarTranslationResult),
+    ],
+    "gh_refine_matrix_translation_candidates": [
+        c_void_p, c_int64, c_int64, POINTER(c_float), c_int64, POINTER(AlignSearchParams), POINTER(AlignCandidate),
+        POINTER(c_int64),
     ],
     "gh_integrate_tile_local": [
         c_void_p, POINTER(c_float), POINTER(c_uint8), POINTER(c_int32), POINTER(c_float), c_int64,

@@ -302,3 +302,83 @@ void launch_refine_centroids(const float* frame, int width, int height, float* x
 }
 
 }  // namespace gh
+
+namespace gh {
+namespace {
+
+// glass_matrix_alignment_metrics_candidates_f32_kernel
+// (warp_kernels.cu:1051-1172): per candidate, over the reference's pixels on
+// the stride grid, the moving frame sampled bilinearly through the float32
+// inverse (pixels whose source falls outside [0, W-1] x [0, H-1], or with a
+// non-finite reference or sample, are skipped) and seven FP64 sums (r, m, r^2,
+// m^2, r*m, (m-r)^2, |m-r|) plus the count.  The reference gives each
+// candidate one block; here a candidate's sample rows are split over
+// `chunks` blocks so that a handful of candidates still fills the chip, and
+// the (candidate, chunk) partials are added on the host in chunk order.
+__global__ __launch_bounds__(256) void alignment_partials_kernel(AlignArgs a, double* partials) {
+  __shared__ double red[4][8];
+  const int cand = static_cast<int>(blockIdx.x) / a.chunks, chunk = static_cast<int>(blockIdx.x) % a.chunks;
+  float inv[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) inv[k] = a.inverses[9 * cand + k];
+  const int r0 = chunk * a.rows_per_chunk, r1 = min(a.sample_h, r0 + a.rows_per_chunk);
+  const long long n = static_cast<long long>(max(r1 - r0, 0)) * a.sample_w;
+  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (long long i = threadIdx.x; i < n; i += 256) {
+    const int sr = r0 + static_cast<int>(i / a.sample_w), sc = static_cast<int>(i % a.sample_w);
+    const int x = sc * a.stride, y = sr * a.stride;
+    const float fx = static_cast<float>(x), fy = static_cast<float>(y);
+    const float denom = inv[6] * fx + inv[7] * fy + inv[8];
+    if (fabsf(denom) <= 1.0e-12f) continue;
+    const float sx = (inv[0] * fx + inv[1] * fy + inv[2]) / denom;
+    const float sy = (inv[3] * fx + inv[4] * fy + inv[5]) / denom;
+    if (!(sx >= 0.0f) || sx > static_cast<float>(a.width - 1) || !(sy >= 0.0f) ||
+        sy > static_cast<float>(a.height - 1))
+      continue;
+    const int x0 = static_cast<int>(floorf(sx)), y0 = static_cast<int>(floorf(sy));
+    const int x1 = x0 + 1 < a.width ? x0 + 1 : x0, y1 = y0 + 1 < a.height ? y0 + 1 : y0;
+    const float tx = sx - static_cast<float>(x0), ty = sy - static_cast<float>(y0);
+    const float* mv = a.moving;
+    const float v00 = mv[static_cast<long long>(y0) * a.width + x0], v10 = mv[static_cast<long long>(y0) * a.width + x1];
+    const float v01 = mv[static_cast<long long>(y1) * a.width + x0], v11 = mv[static_cast<long long>(y1) * a.width + x1];
+    const float top = v00 * (1.0f - tx) + v10 * tx;
+    const float bottom = v01 * (1.0f - tx) + v11 * tx;
+    const float warped = top * (1.0f - ty) + bottom * ty;
+    const float ref = a.reference[static_cast<long long>(y) * a.width + x];
+    if (!isfinite(ref) || !isfinite(warped)) continue;
+    const double r = ref, m = warped, d = m - r;
+    s[0] += r;
+    s[1] += m;
+    s[2] += r * r;
+    s[3] += m * m;
+    s[4] += r * m;
+    s[5] += d * d;
+    s[6] += fabs(d);
+    s[7] += 1.0;
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    for (int off = 32; off > 0; off >>= 1) s[k] += __shfl_down(s[k], off, 64);
+  }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) red[wave][k] = s[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    const int k = threadIdx.x;
+    partials[static_cast<long long>(blockIdx.x) * 8 + k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+  }
+}
+
+}  // namespace
+
+void launch_alignment_partials(const AlignArgs& a, double* partials, hipStream_t stream) {
+  if (a.candidates <= 0) return;
+  alignment_partials_kernel<<<static_cast<unsigned>(a.candidates) * static_cast<unsigned>(a.chunks), 256, 0,
+                              stream>>>(a, partials);
+  check(hipGetLastError(), "alignment metrics launch");
+}
+
+}  // namespace gh

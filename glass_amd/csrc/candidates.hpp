@@ -42,6 +42,16 @@ void grid_select(const unsigned long long* ranked, unsigned n, int width, int he
 // scores[i] = #finite offsets within squared distance tol2 of offset i (-1
 // when offset i is not finite).
 void launch_offset_scores(const float* dx, const float* dy, int n, float tol2, int* scores, hipStream_t stream);
+// Alignment metrics of translated matrix candidates: partials[(c * chunks +
+// k) * 8 + j], j = sums of r, m, r^2, m^2, r*m, (m-r)^2, |m-r| and the count
+// over sample rows [k * rows_per_chunk, ...) of candidate c.
+struct AlignArgs {
+  const float* reference;
+  const float* moving;
+  const float* inverses;  // [candidates][9] float32
+  int width, height, stride, sample_w, sample_h, rows_per_chunk, chunks, candidates;
+};
+void launch_alignment_partials(const AlignArgs& a, double* partials, hipStream_t stream);
 void launch_refine_centroids(const float* frame, int width, int height, float* xs, float* ys, const float* fluxes,
                              unsigned char* status, int count, int radius, float background, hipStream_t stream);
 

@@ -144,6 +144,8 @@ struct gh_stack {
   DeviceBuffer<unsigned> cand_u32, cand_count;
   DeviceBuffer<unsigned char> cand_keep, cand_temp, cand_status;
   DeviceBuffer<float> cand_xyf;
+  DeviceBuffer<float> align_inv;            // alignment-metric candidates: float32 inverses
+  DeviceBuffer<double> align_partials;      //   (candidate, row chunk) FP64 partial sums
   DeviceBuffer<double> ncc_ws;              // translation search partials + scores
 
   hipStream_t pick(void* s) const { return s ? static_cast<hipStream_t>(s) : stream; }
@@ -306,6 +308,8 @@ struct gh_stack {
     cand_temp.release();
     cand_status.release();
     cand_xyf.release();
+    align_inv.release();
+    align_partials.release();
     ncc_ws.release();
     if (copy_stream) (void)hipStreamSynchronize(copy_stream);
     for (int i = 0; i < kRawRing; ++i) {

@@ -1171,6 +1171,53 @@ class ResidentCalibratedStack:
                 "prior_radius_px": float(res.prior_radius_px), "reference_index": int(reference_index),
                 "moving_index": int(moving_index), "model": "resident_star_catalog_pair_offset_translation"}
 
+    def refine_matrix_translation_candidates_to_reference(self, reference_index, moving_index, matrices,
+                                                          search_radius_px=1.0, coarse_step_px=0.25,
+                                                          fine_radius_px=0.25, fine_step_px=0.0625,
+                                                          coarse_sample_stride=4, final_sample_stride=1):
+        """refine_matrix_translation_candidates_to_reference
+        (native_bindings.cpp:10079-10184): coarse then fine grids of
+        translation corrections for each seed matrix, scored by the bilinear
+        alignment metrics (rms, mean |diff|, ncc) against the reference."""
+        m = np.asarray(matrices, dtype=np.float32)
+        if m.shape == (3, 3):
+            m = m[None]
+        if m.ndim != 3 or m.shape[1:] != (3, 3):
+            raise ValueError("matrices must have shape (N, 3, 3) or (3, 3)")
+        if m.shape[0] <= 0:
+            raise ValueError("matrices must contain at least one matrix")
+        m = np.ascontiguousarray(m)
+        prm = N.AlignSearchParams(float(search_radius_px), float(coarse_step_px), float(fine_radius_px),
+                                  float(fine_step_px), int(coarse_sample_stride), int(final_sample_stride))
+        seeds = (N.AlignCandidate * m.shape[0])()
+        selected = ctypes.c_int64(-1)
+        N.call("gh_refine_matrix_translation_candidates", self._h, int(reference_index), int(moving_index),
+               m.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), int(m.shape[0]), ctypes.byref(prm), seeds,
+               ctypes.byref(selected))
+        model = "matrix_alignment_metrics_cuda_candidate_grid"
+
+        def metrics(c):
+            return {"valid_pixels": int(c.valid_pixels), "sampled_pixels": int(c.sampled_pixels),
+                    "sample_stride": int(c.sample_stride), "rms": float(c.rms),
+                    "mean_abs_diff": float(c.mean_abs_diff), "ncc": float(c.ncc), "model": model}
+
+        def matrix(c):
+            return np.asarray(list(c.matrix), np.float32).reshape(3, 3).astype(float).tolist()
+
+        seed_results = [{"seed_index": k, "matrix": matrix(c), "dx_correction": float(c.dx),
+                         "dy_correction": float(c.dy), "metrics": metrics(c),
+                         "coarse_candidates": int(c.coarse_candidates), "fine_candidates": int(c.fine_candidates)}
+                        for k, c in enumerate(seeds)]
+        best = seeds[selected.value]
+        return {"matrix": matrix(best), "dx_correction": float(best.dx), "dy_correction": float(best.dy),
+                "metrics": metrics(best), "selected_index": int(selected.value), "seed_count": int(m.shape[0]),
+                "seed_results": seed_results, "coarse_candidates_per_seed": int(seeds[0].coarse_candidates),
+                "search_radius_px": float(search_radius_px), "coarse_step_px": float(coarse_step_px),
+                "fine_radius_px": float(fine_radius_px), "fine_step_px": float(fine_step_px),
+                "coarse_sample_stride": int(coarse_sample_stride), "final_sample_stride": int(final_sample_stride),
+                "reference_index": int(reference_index), "moving_index": int(moving_index),
+                "model": "resident_cuda_matrix_metric_translation_multi_seed_refine_grid"}
+
     def _tile_local(self, target_mask, tile_extents, tile_multipliers, weights, rejection, low_sigma, high_sigma,
                     winsorize):
         f = self.frame_count

@@ -346,6 +346,37 @@ typedef struct gh_star_translation_result {
 int gh_estimate_translation_from_stars(gh_stack* stack, int64_t reference_index, int64_t moving_index,
                                        const gh_star_translation_params* params, gh_star_translation_result* result);
 
+/* refine_matrix_translation_candidates_to_reference (native_bindings.cpp:
+ * 10079-10184; helpers :1459-1700; kernel cpp/cuda/warp_kernels.cu:1051-1172):
+ * for each seed matrix (float32 [seed_count][9], output -> source), the
+ * corrections (dx, dy) of a float32 grid of radius search_radius_px and step
+ * coarse_step_px are added to the matrix translation and scored on the
+ * reference's pixels every coarse_sample_stride: the moving frame sampled
+ * bilinearly through the float32 inverse, FP64 sums giving rms, mean |diff|
+ * and ncc over the pixels inside the source frame with finite values.  The
+ * best (lowest rms, then highest ncc, then most pixels) is rescored on a fine
+ * grid around it (fine_radius_px > 0, stride final_sample_stride), or alone at
+ * the final stride.  seeds[k] receives each seed's best; *selected_index the
+ * best seed.  Synchronous. */
+typedef struct gh_align_search_params {
+  float search_radius_px, coarse_step_px, fine_radius_px, fine_step_px;
+  int32_t coarse_sample_stride, final_sample_stride;
+} gh_align_search_params;
+
+typedef struct gh_align_candidate {
+  float matrix[9];
+  float dx, dy;                    /* correction added to the seed's translation */
+  int64_t valid_pixels, sampled_pixels;
+  int32_t sample_stride;
+  double rms, mean_abs_diff, ncc;  /* NaN when undefined */
+  int64_t coarse_candidates, fine_candidates;
+} gh_align_candidate;
+
+int gh_refine_matrix_translation_candidates(gh_stack* stack, int64_t reference_index, int64_t moving_index,
+                                            const float* matrices, int64_t seed_count,
+                                            const gh_align_search_params* params, gh_align_candidate* seeds,
+                                            int64_t* selected_index);
+
 /* frame_global_stats(i) (native_bindings.cpp:10569-10643) plus exact median /
  * MAD (src/glass/engine/quality.py:126-200).  FP64. */
 int gh_frame_stats(gh_stack* stack, int64_t index, int32_t exact_median, gh_frame_statistics* out);

@@ -380,3 +380,128 @@ def ncc_translation_search(reference, moving, radius_x, radius_y, subpixel=False
             scores.append(score)
     best = int(np.argmax(np.asarray(scores)))  # first maximum
     return offsets[best][0], offsets[best][1], float(np.float32(scores[best])), np.asarray(scores)
+
+
+# ---- translated-matrix candidates scored by alignment metrics
+# (native_bindings.cpp:1459-1700, :10079-10184; warp_kernels.cu:1051-1172)
+
+def translation_offsets(cx, cy, radius, step):
+    """Float32 grid, dx outer (native_bindings.cpp:1489-1506)."""
+    f = np.float32
+    out = []
+    dx = f(f(cx) - f(radius))
+    while dx <= f(f(cx) + f(radius)) + f(f(step) * f(0.5)):
+        dy = f(f(cy) - f(radius))
+        while dy <= f(f(cy) + f(radius)) + f(f(step) * f(0.5)):
+            out.append((dx, dy))
+            dy = f(dy + f(step))
+        dx = f(dx + f(step))
+    return out or [(f(cx), f(cy))]
+
+
+def invert_matrix_f32(m):
+    """FP64 adjugate inverse rounded to float32 (native_bindings.cpp:1422-1457)."""
+    a, b, c, d, e, f_, g, h, i = (float(v) for v in np.asarray(m, np.float32).ravel())
+    cof = [e * i - f_ * h, -(d * i - f_ * g), d * h - e * g, -(b * i - c * h), a * i - c * g, -(a * h - b * g),
+           b * f_ - c * e, -(a * f_ - c * d), a * e - b * d]
+    det = a * cof[0] + b * cof[1] + c * cof[2]
+    k = 1.0 / det
+    return np.array([cof[0] * k, cof[3] * k, cof[6] * k, cof[1] * k, cof[4] * k, cof[7] * k, cof[2] * k,
+                     cof[5] * k, cof[8] * k], np.float64).astype(np.float32)
+
+
+def alignment_sums(reference, moving, inverse, stride):
+    """The seven FP64 sums and the count of the alignment-metric kernel
+    (float32 coordinates and bilinear taps, one rounding per operation)."""
+    f = np.float32
+    ref = np.asarray(reference, np.float32)
+    mov = np.asarray(moving, np.float32)
+    h, w = ref.shape
+    inv = np.asarray(inverse, np.float32)
+    ys, xs = np.meshgrid(np.arange(0, h, stride), np.arange(0, w, stride), indexing="ij")
+    fx, fy = xs.astype(np.float32), ys.astype(np.float32)
+    with np.errstate(all="ignore"):
+        den = (inv[6] * fx + inv[7] * fy) + inv[8]
+        sx = ((inv[0] * fx + inv[1] * fy) + inv[2]) / den
+        sy = ((inv[3] * fx + inv[4] * fy) + inv[5]) / den
+        ok = (np.abs(den) > f(1e-12)) & (sx >= 0) & (sx <= f(w - 1)) & (sy >= 0) & (sy <= f(h - 1))
+        x0 = np.floor(np.where(ok, sx, 0)).astype(np.int64)
+        y0 = np.floor(np.where(ok, sy, 0)).astype(np.int64)
+        x1 = np.where(x0 + 1 < w, x0 + 1, x0)
+        y1 = np.where(y0 + 1 < h, y0 + 1, y0)
+        tx = (np.where(ok, sx, 0) - x0.astype(np.float32)).astype(np.float32)
+        ty = (np.where(ok, sy, 0) - y0.astype(np.float32)).astype(np.float32)
+        top = mov[y0, x0] * (f(1) - tx) + mov[y0, x1] * tx
+        bottom = mov[y1, x0] * (f(1) - tx) + mov[y1, x1] * tx
+        warped = top * (f(1) - ty) + bottom * ty
+        r = ref[ys, xs]
+        ok &= np.isfinite(r) & np.isfinite(warped)
+    r = r[ok].astype(np.float64)
+    m = warped[ok].astype(np.float64)
+    d = m - r
+    return np.array([r.sum(), m.sum(), (r * r).sum(), (m * m).sum(), (r * m).sum(), (d * d).sum(),
+                     np.abs(d).sum(), float(ok.sum())])
+
+
+def alignment_metrics(s):
+    n = s[7]
+    out = {"valid_pixels": int(n), "rms": np.nan, "mean_abs_diff": np.nan, "ncc": np.nan}
+    if n == 0:
+        return out
+    out["rms"] = float(np.sqrt(s[5] / n))
+    out["mean_abs_diff"] = float(s[6] / n)
+    if n > 1:
+        num = s[4] - s[0] * s[1] / n
+        den = np.sqrt(max(s[2] - s[0] * s[0] / n, 0.0) * max(s[3] - s[1] * s[1] / n, 0.0))
+        if den > 0:
+            out["ncc"] = float(num / den)
+    return out
+
+
+def _better(c, b):
+    cr, br = np.isfinite(c["rms"]), np.isfinite(b["rms"])
+    if cr != br:
+        return cr
+    if cr and abs(c["rms"] - b["rms"]) > 1e-12:
+        return c["rms"] < b["rms"]
+    cn, bn = np.isfinite(c["ncc"]), np.isfinite(b["ncc"])
+    if cn != bn:
+        return cn
+    if cn and abs(c["ncc"] - b["ncc"]) > 1e-12:
+        return c["ncc"] > b["ncc"]
+    return c["valid_pixels"] > b["valid_pixels"]
+
+
+def _score(reference, moving, base, offsets, stride):
+    best = None
+    for dx, dy in offsets:
+        m = np.asarray(base, np.float32).reshape(9).copy()
+        m[2] = np.float32(m[2] + dx)
+        m[5] = np.float32(m[5] + dy)
+        c = alignment_metrics(alignment_sums(reference, moving, invert_matrix_f32(m), stride))
+        c.update(dx=float(dx), dy=float(dy), matrix=m.reshape(3, 3))
+        if best is None or _better(c, best):
+            best = c
+    return best
+
+
+def refine_matrix_translation_candidates(reference, moving, seeds, search_radius_px=1.0, coarse_step_px=0.25,
+                                         fine_radius_px=0.25, fine_step_px=0.0625, coarse_sample_stride=4,
+                                         final_sample_stride=1):
+    """(per-seed bests, selected index) of
+    refine_matrix_translation_candidates_to_reference."""
+    coarse = translation_offsets(0.0, 0.0, search_radius_px, coarse_step_px)
+    results, sel = [], -1
+    for base in np.asarray(seeds, np.float32).reshape(-1, 3, 3):
+        cb = _score(reference, moving, base, coarse, coarse_sample_stride)
+        best = cb
+        if fine_radius_px > 0:
+            best = _score(reference, moving, base, translation_offsets(cb["dx"], cb["dy"], fine_radius_px,
+                                                                       fine_step_px), final_sample_stride)
+        elif final_sample_stride != coarse_sample_stride:
+            best = _score(reference, moving, base, [(np.float32(cb["dx"]), np.float32(cb["dy"]))],
+                          final_sample_stride)
+        results.append(best)
+        if sel < 0 or _better(best, results[sel]):
+            sel = len(results) - 1
+    return results, sel

@@ -143,3 +143,64 @@ def test_ncc_translation_matches_oracle_with_nan_and_stride(gpu, stride):
     assert got["score"] == pytest.approx(score, abs=1e-6)
     sdx, sdy, sscore, _ = ncc_translation_search(ref, mov, 2, 2, True, -5.0, 4.0, 0.5, stride)
     assert (sub["dx"], sub["dy"]) == (sdx, sdy) and sub["score"] == pytest.approx(sscore, abs=1e-6)
+
+
+def _stack_of(*frames):
+    s = ResidentCalibratedStack(len(frames), frames[0].shape[0], frames[0].shape[1])
+    for i, f in enumerate(frames):
+        s.upload_calibrated_frame(i, f)
+    return s
+
+
+def _smooth_pair(h=180, w=230, seed=5):
+    rng = np.random.default_rng(seed)
+    yy, xx = np.indices((h, w), dtype=np.float64)
+    ref = (100 + 20 * np.sin(xx / 9.0) * np.cos(yy / 11.0) + 0.05 * xx + rng.normal(0, 0.5, (h, w)))
+    ref = ref.astype(np.float32)
+    ref[rng.random((h, w)) < 0.003] = np.nan
+    mov = np.full_like(ref, np.nan)
+    mov[2:, :-3] = ref[:-2, 3:]      # moving(x - 3, y + 2) = reference(x, y)
+    return ref, mov
+
+
+def test_matrix_translation_candidates_match_the_oracle(gpu):
+    ref, mov = _smooth_pair()
+    a = 0.002
+    seeds = np.stack([np.eye(3), [[np.cos(a), -np.sin(a), 3.5], [np.sin(a), np.cos(a), -1.25], [0, 0, 1]]])
+    with _stack_of(ref, mov) as s:
+        res = s.refine_matrix_translation_candidates_to_reference(0, 1, seeds, 4.0, 0.5, 0.5, 0.125, 3, 1)
+        single = s.refine_matrix_translation_candidates_to_reference(0, 1, np.eye(3), 4.0, 0.5, 0.0, 0.125, 3, 2)
+    want, sel = O.refine_matrix_translation_candidates(ref, mov, seeds, 4.0, 0.5, 0.5, 0.125, 3, 1)
+    assert res["selected_index"] == sel and res["seed_count"] == 2
+    assert res["model"] == "resident_cuda_matrix_metric_translation_multi_seed_refine_grid"
+    for got, exp in zip(res["seed_results"], want):
+        assert (got["dx_correction"], got["dy_correction"]) == (exp["dx"], exp["dy"])
+        assert np.array_equal(np.asarray(got["matrix"], np.float32), exp["matrix"])
+        m = got["metrics"]
+        assert m["valid_pixels"] == exp["valid_pixels"] and m["sample_stride"] == 1
+        for key in ("rms", "mean_abs_diff", "ncc"):
+            assert m[key] == pytest.approx(exp[key], rel=1e-9, abs=1e-12), key
+        assert got["coarse_candidates"] == 17 * 17 and got["fine_candidates"] == 9 * 9
+    # the identity seed recovers the shift exactly: correction (+3, -2)
+    best = res["seed_results"][0]
+    assert (best["dx_correction"], best["dy_correction"]) == (3.0, -2.0) and best["metrics"]["rms"] == 0.0
+    # no fine grid: the coarse best rescored alone at the final stride
+    ws, _ = O.refine_matrix_translation_candidates(ref, mov, [np.eye(3)], 4.0, 0.5, 0.0, 0.125, 3, 2)
+    assert single["seed_results"][0]["fine_candidates"] == 1
+    assert single["metrics"]["valid_pixels"] == ws[0]["valid_pixels"]
+    assert single["metrics"]["rms"] == pytest.approx(ws[0]["rms"], rel=1e-9, abs=1e-12)
+
+
+def test_matrix_translation_candidates_errors(gpu):
+    ref, mov = _smooth_pair(20, 24)
+    with _stack_of(ref, mov) as s:
+        with pytest.raises(ValueError, match="radii must be non-negative"):
+            s.refine_matrix_translation_candidates_to_reference(0, 1, np.eye(3), -1.0)
+        with pytest.raises(ValueError, match="steps must be positive"):
+            s.refine_matrix_translation_candidates_to_reference(0, 1, np.eye(3), 1.0, 0.0)
+        with pytest.raises(ValueError, match="strides must be positive"):
+            s.refine_matrix_translation_candidates_to_reference(0, 1, np.eye(3), coarse_sample_stride=0)
+        with pytest.raises(ValueError, match=r"shape \(N, 3, 3\)"):
+            s.refine_matrix_translation_candidates_to_reference(0, 1, np.eye(2))
+        with pytest.raises(ValueError, match="invertible"):
+            s.refine_matrix_translation_candidates_to_reference(0, 1, np.zeros((3, 3)))

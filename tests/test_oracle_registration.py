@@ -159,3 +159,27 @@ def test_ncc_translation_search_closed_forms():
     sdx, sdy, sscore, sc = ncc_translation_search(reference, moving, 3, 3, True, cdx, cdy, 0.25)
     assert len(sc) == 49 and sscore > 0.98
     assert abs(sdx - 2.25) <= 0.25 and abs(sdy - (-1.5)) <= 0.25
+
+
+def test_matrix_candidate_refinement_closed_forms():
+    """An integer shift of a smooth frame: the correction that undoes it scores
+    rms 0 and ncc 1 and wins the coarse and the fine grid; the float32 offset
+    grid of radius 1, step 0.25 has 81 points including the exact shift."""
+    from oracle import registration as R
+
+    yy, xx = np.indices((40, 52), dtype=np.float64)
+    ref = (100 + 20 * np.sin(xx / 5.0) * np.cos(yy / 7.0) + 0.3 * xx).astype(np.float32)
+    mov = np.full_like(ref, np.nan)
+    mov[:, 1:] = ref[:, :-1]         # moving(x + 1, y) = reference(x, y)
+    mov[1:, :] = mov[:-1, :].copy()  # and one row down: moving(x + 1, y + 1) = reference(x, y)
+    offs = R.translation_offsets(0.0, 0.0, 1.0, 0.25)
+    assert len(offs) == 81 and (np.float32(-1), np.float32(-1)) in offs
+    results, sel = R.refine_matrix_translation_candidates(ref, mov, [np.eye(3)], 1.0, 0.25, 0.25, 0.0625, 2, 1)
+    best = results[sel]
+    assert (best["dx"], best["dy"]) == (-1.0, -1.0)
+    assert best["rms"] == 0.0 and best["mean_abs_diff"] == 0.0 and abs(best["ncc"] - 1.0) < 1e-12
+    # the inverse of the winning matrix is the +1 / +1 translation, exactly
+    assert np.array_equal(R.invert_matrix_f32(best["matrix"]), np.array([1, 0, 1, 0, 1, 1, 0, 0, 1], np.float32))
+    # the sums: every valid pixel pairs reference(x, y) with itself
+    s = R.alignment_sums(ref, mov, R.invert_matrix_f32(best["matrix"]), 1)
+    assert s[0] == s[1] and s[5] == 0.0 and s[7] == (40 - 1) * (52 - 1)
