@@ -1140,6 +1140,25 @@ class ResidentCalibratedStack:
             out.append(item)
         return out
 
+    def refine_matrix_translation_candidates_batch_to_reference(self, reference_index, moving_indices, matrices,
+                                                                search_radius_px=1.0, coarse_step_px=0.25,
+                                                                fine_radius_px=0.25, fine_step_px=0.0625,
+                                                                coarse_sample_stride=4, final_sample_stride=1):
+        """refine_matrix_translation_candidates_batch_to_reference
+        (native_bindings.cpp:10186-10416): one seed matrix per moving frame,
+        each searched as refine_matrix_translation_candidates_to_reference
+        (the per-frame form the reference's wrapper uses without a batch
+        kernel, glass_cuda.py:3788-3801)."""
+        moving = [int(i) for i in list(moving_indices)]
+        if not moving:
+            return []
+        mats = np.asarray(matrices, dtype=np.float32)
+        if mats.ndim != 3 or mats.shape[1:] != (3, 3) or mats.shape[0] != len(moving):
+            raise ValueError("batch refine requires one seed matrix per moving frame")
+        return [self.refine_matrix_translation_candidates_to_reference(
+            reference_index, i, mats[k][None], search_radius_px, coarse_step_px, fine_radius_px, fine_step_px,
+            coarse_sample_stride, final_sample_stride) for k, i in enumerate(moving)]
+
     def estimate_translation_from_stars_to_reference(self, reference_index, moving_index, threshold,
                                                      max_candidates=64, tolerance_px=1.0, max_abs_dx=-1.0,
                                                      max_abs_dy=None, prior_dx=0.0, prior_dy=0.0,

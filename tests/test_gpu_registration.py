@@ -184,6 +184,15 @@ def test_matrix_translation_candidates_match_the_oracle(gpu):
     # the identity seed recovers the shift exactly: correction (+3, -2)
     best = res["seed_results"][0]
     assert (best["dx_correction"], best["dy_correction"]) == (3.0, -2.0) and best["metrics"]["rms"] == 0.0
+    with _stack_of(ref, mov, mov) as s3:
+        batch = s3.refine_matrix_translation_candidates_batch_to_reference(0, [1, 2], seeds, 4.0, 0.5, 0.5, 0.125,
+                                                                           3, 1)
+        assert s3.refine_matrix_translation_candidates_batch_to_reference(0, [], seeds[:0]) == []
+        with pytest.raises(ValueError, match="one seed matrix per moving frame"):
+            s3.refine_matrix_translation_candidates_batch_to_reference(0, [1], seeds)
+    assert [b["moving_index"] for b in batch] == [1, 2]
+    for b, exp in zip(batch, want):
+        assert (b["dx_correction"], b["dy_correction"]) == (exp["dx"], exp["dy"])
     # no fine grid: the coarse best rescored alone at the final stride
     ws, _ = O.refine_matrix_translation_candidates(ref, mov, [np.eye(3)], 4.0, 0.5, 0.0, 0.125, 3, 2)
     assert single["seed_results"][0]["fine_candidates"] == 1
