@@ -244,6 +244,10 @@ _SIGNATURES = {
         c_void_p, c_int64, c_int64, POINTER(c_float), c_int64, POINTER(AlignSearchParams), POINTER(AlignCandidate),
         POINTER(c_int64),
     ],
+    "gh_refine_matrix_translation_candidates_batch": [
+        c_void_p, c_int64, POINTER(c_int64), c_int64, POINTER(c_float), POINTER(AlignSearchParams),
+        POINTER(AlignCandidate),
+    ],
     "gh_integrate_tile_local": [
         c_void_p, POINTER(c_float), POINTER(c_uint8), POINTER(c_int32), POINTER(c_float), c_int64,
         POINTER(ClipParams), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,

@@ -315,6 +315,8 @@ namespace {
 // candidate one block; here a candidate's sample rows are split over
 // `chunks` blocks so that a handful of candidates still fills the chip, and
 // the (candidate, chunk) partials are added on the host in chunk order.
+// Each candidate names its moving frame, so the candidates of many frames
+// (the batch form) share one launch.
 __global__ __launch_bounds__(256) void alignment_partials_kernel(AlignArgs a, double* partials) {
   __shared__ double red[4][8];
   const int cand = static_cast<int>(blockIdx.x) / a.chunks, chunk = static_cast<int>(blockIdx.x) % a.chunks;
@@ -338,7 +340,7 @@ __global__ __launch_bounds__(256) void alignment_partials_kernel(AlignArgs a, do
     const int x0 = static_cast<int>(floorf(sx)), y0 = static_cast<int>(floorf(sy));
     const int x1 = x0 + 1 < a.width ? x0 + 1 : x0, y1 = y0 + 1 < a.height ? y0 + 1 : y0;
     const float tx = sx - static_cast<float>(x0), ty = sy - static_cast<float>(y0);
-    const float* mv = a.moving;
+    const float* mv = table_at(a.moving, cand);
     const float v00 = mv[static_cast<long long>(y0) * a.width + x0], v10 = mv[static_cast<long long>(y0) * a.width + x1];
     const float v01 = mv[static_cast<long long>(y1) * a.width + x0], v11 = mv[static_cast<long long>(y1) * a.width + x1];
     const float top = v00 * (1.0f - tx) + v10 * tx;

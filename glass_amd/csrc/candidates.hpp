@@ -47,8 +47,8 @@ void launch_offset_scores(const float* dx, const float* dy, int n, float tol2, i
 // over sample rows [k * rows_per_chunk, ...) of candidate c.
 struct AlignArgs {
   const float* reference;
-  const float* moving;
-  const float* inverses;  // [candidates][9] float32
+  const float* const* moving;  // [candidates] device table: the frame each candidate samples
+  const float* inverses;       // [candidates][9] float32
   int width, height, stride, sample_w, sample_h, rows_per_chunk, chunks, candidates;
 };
 void launch_alignment_partials(const AlignArgs& a, double* partials, hipStream_t stream);

@@ -146,6 +146,7 @@ struct gh_stack {
   DeviceBuffer<float> cand_xyf;
   DeviceBuffer<float> align_inv;            // alignment-metric candidates: float32 inverses
   DeviceBuffer<double> align_partials;      //   (candidate, row chunk) FP64 partial sums
+  DeviceBuffer<const float*> align_frames;  //   moving frame of each candidate
   DeviceBuffer<double> ncc_ws;              // translation search partials + scores
 
   hipStream_t pick(void* s) const { return s ? static_cast<hipStream_t>(s) : stream; }
@@ -310,6 +311,7 @@ struct gh_stack {
     cand_xyf.release();
     align_inv.release();
     align_partials.release();
+    align_frames.release();
     ncc_ws.release();
     if (copy_stream) (void)hipStreamSynchronize(copy_stream);
     for (int i = 0; i < kRawRing; ++i) {

@@ -1146,18 +1146,38 @@ class ResidentCalibratedStack:
                                                                 coarse_sample_stride=4, final_sample_stride=1):
         """refine_matrix_translation_candidates_batch_to_reference
         (native_bindings.cpp:10186-10416): one seed matrix per moving frame,
-        each searched as refine_matrix_translation_candidates_to_reference
-        (the per-frame form the reference's wrapper uses without a batch
-        kernel, glass_cuda.py:3788-3801)."""
+        each searched as refine_matrix_translation_candidates_to_reference;
+        the coarse candidates of all frames share one launch, as do the fine
+        ones."""
         moving = [int(i) for i in list(moving_indices)]
-        if not moving:
-            return []
         mats = np.asarray(matrices, dtype=np.float32)
-        if mats.ndim != 3 or mats.shape[1:] != (3, 3) or mats.shape[0] != len(moving):
+        if mats.ndim != 3 or mats.shape[1:] != (3, 3):
+            raise ValueError("matrices must have shape (N, 3, 3) or (3, 3)")
+        if moving and mats.shape[0] != len(moving):
             raise ValueError("batch refine requires one seed matrix per moving frame")
-        return [self.refine_matrix_translation_candidates_to_reference(
-            reference_index, i, mats[k][None], search_radius_px, coarse_step_px, fine_radius_px, fine_step_px,
-            coarse_sample_stride, final_sample_stride) for k, i in enumerate(moving)]
+        prm = N.AlignSearchParams(float(search_radius_px), float(coarse_step_px), float(fine_radius_px),
+                                  float(fine_step_px), int(coarse_sample_stride), int(final_sample_stride))
+        n = len(moving)
+        out = (N.AlignCandidate * max(n, 1))()
+        idx = np.ascontiguousarray(np.asarray(moving, dtype=np.int64).reshape(n))
+        mats = np.ascontiguousarray(mats)
+        N.call("gh_refine_matrix_translation_candidates_batch", self._h, int(reference_index),
+               idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), n,
+               mats.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(prm), out)
+        results = []
+        for k, i in enumerate(moving):
+            c = out[k]
+            results.append({
+                "reference_index": int(reference_index), "moving_index": i,
+                "matrix": np.asarray(list(c.matrix), np.float32).reshape(3, 3).astype(float).tolist(),
+                "dx_correction": float(c.dx), "dy_correction": float(c.dy),
+                "metrics": {"valid_pixels": int(c.valid_pixels), "sampled_pixels": int(c.sampled_pixels),
+                            "sample_stride": int(c.sample_stride), "rms": float(c.rms),
+                            "mean_abs_diff": float(c.mean_abs_diff), "ncc": float(c.ncc),
+                            "model": "matrix_alignment_metrics_cuda_candidate_grid"},
+                "coarse_candidates": int(c.coarse_candidates), "fine_candidates": int(c.fine_candidates),
+                "model": "resident_cuda_matrix_metric_translation_batch_refine_grid"})
+        return results
 
     def estimate_translation_from_stars_to_reference(self, reference_index, moving_index, threshold,
                                                      max_candidates=64, tolerance_px=1.0, max_abs_dx=-1.0,

@@ -377,6 +377,15 @@ int gh_refine_matrix_translation_candidates(gh_stack* stack, int64_t reference_i
                                             const gh_align_search_params* params, gh_align_candidate* seeds,
                                             int64_t* selected_index);
 
+/* refine_matrix_translation_candidates_batch_to_reference
+ * (native_bindings.cpp:10186-10416; kernel cpp/cuda/warp_kernels.cu:1198):
+ * moving frame moving_indices[k] searched from seed matrices[k] as above;
+ * out[k] its best.  The coarse candidates of every frame share one launch, as
+ * do the fine ones.  Synchronous. */
+int gh_refine_matrix_translation_candidates_batch(gh_stack* stack, int64_t reference_index,
+                                                  const int64_t* moving_indices, int64_t count, const float* matrices,
+                                                  const gh_align_search_params* params, gh_align_candidate* out);
+
 /* frame_global_stats(i) (native_bindings.cpp:10569-10643) plus exact median /
  * MAD (src/glass/engine/quality.py:126-200).  FP64. */
 int gh_frame_stats(gh_stack* stack, int64_t index, int32_t exact_median, gh_frame_statistics* out);

@@ -193,6 +193,9 @@ def test_matrix_translation_candidates_match_the_oracle(gpu):
     assert [b["moving_index"] for b in batch] == [1, 2]
     for b, exp in zip(batch, want):
         assert (b["dx_correction"], b["dy_correction"]) == (exp["dx"], exp["dy"])
+        assert b["metrics"]["valid_pixels"] == exp["valid_pixels"]
+        assert b["metrics"]["rms"] == pytest.approx(exp["rms"], rel=1e-9, abs=1e-12)
+        assert b["fine_candidates"] == 81 and b["coarse_candidates"] == 289
     # no fine grid: the coarse best rescored alone at the final stride
     ws, _ = O.refine_matrix_translation_candidates(ref, mov, [np.eye(3)], 4.0, 0.5, 0.0, 0.125, 3, 2)
     assert single["seed_results"][0]["fine_candidates"] == 1
