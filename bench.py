@@ -390,7 +390,25 @@ def registration_leg(args, stack, light_raw, a, frames=9):
         got = np.asarray(reg.matrix, np.float64).reshape(3, 3) @ corners
         exp = np.asarray(a["mats"][i], np.float64) @ corners
         errs.append(float(np.max(np.hypot(got[0] - exp[0], got[1] - exp[1]))))
+    # star catalogues on the same resident frames: grid top-k + separation
+    # filter (16 x 12 cells, 4 per cell, 400 kept) and the catalogue
+    # translation vote (top 64 by flux), threshold median + 5 * 1.4826 MAD
+    # of frame 0
+    st0 = stack.frame_global_stats(0, exact_median=True)
+    thr = float(st0["median"] + 5.0 * 1.4826 * st0["mad"])
+    cat = {"catalogue_threshold": round(thr, 3)}
+    if np.isfinite(thr):
+        stack.star_grid_top_nms_candidates(0, thr, 16, 12, 4, 400, 8.0)
+        t = time.perf_counter()
+        grids = [stack.star_grid_top_nms_candidates(i, thr, 16, 12, 4, 400, 8.0) for i in idx]
+        cat["grid_catalogue_ms_per_frame"] = round((time.perf_counter() - t) / f * 1e3, 2)
+        cat["grid_catalogue_local_maxima_median"] = int(np.median([g["count"] for g in grids]))
+        t = time.perf_counter()
+        tr = [stack.estimate_translation_from_stars_to_reference(0, i, thr, 64, 1.0) for i in idx[1:]]
+        cat["catalogue_translation_ms_per_frame"] = round((time.perf_counter() - t) / max(1, f - 1) * 1e3, 2)
+        cat["catalogue_translation_mutual_min"] = int(min(x["mutual_inliers"] for x in tr)) if tr else 0
     out = {"frames": f, "detect_ms_per_frame": round(detect_s * 1e3, 2), "fit_ms_per_frame": round(fit_s * 1e3, 2),
+           **cat,
            "stars_per_frame": round(float(np.mean([len(x) for x in stars])), 1),
            "matched_min": int(min(r.matched_stars for r in regs)) if regs else 0,
            "corner_err_px_max": round(max(errs), 4) if errs else None,
