@@ -199,6 +199,18 @@ class BatchJob(ctypes.Structure):
     ]
 
 
+class DeviceProperties(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 256),
+        ("arch", ctypes.c_char * 64),
+        ("memory_total_bytes", c_int64),
+        ("multiprocessor_count", c_int32),
+        ("major", c_int32),
+        ("minor", c_int32),
+        ("wavefront_size", c_int32),
+    ]
+
+
 GH_BATCH_USE_GRAPH = 1
 GH_BATCH_AUTOTUNE = 2
 
@@ -207,6 +219,9 @@ _SIGNATURES = {
     "gh_last_error": [c_char_p, c_size_t],
     "gh_version": [],
     "gh_device_count": [POINTER(c_int)],
+    "gh_device_info": [c_int, POINTER(DeviceProperties)],
+    "gh_host_alloc": [c_size_t, POINTER(c_void_p)],
+    "gh_host_free": [c_void_p],
     "gh_stack_create": [c_int64, c_int64, c_int64, POINTER(c_void_p)],
     "gh_stack_destroy": [c_void_p],
     "gh_stack_info": [c_void_p, POINTER(c_int64)],

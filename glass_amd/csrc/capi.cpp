@@ -30,6 +30,40 @@ int gh_device_count(int* count) {
   });
 }
 
+int gh_device_info(int device, gh_device_properties* out) {
+  return guarded([&] {
+    if (!out) throw gh::ValueError("device properties pointer is NULL");
+    int n = 0;
+    check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (device < 0 || device >= n) throw gh::IndexError("HIP device id is out of range");
+    hipDeviceProp_t props{};
+    check(hipGetDeviceProperties(&props, device), "hipGetDeviceProperties");
+    std::memset(out, 0, sizeof(*out));
+    std::strncpy(out->name, props.name, sizeof(out->name) - 1);
+    std::strncpy(out->arch, props.gcnArchName, sizeof(out->arch) - 1);
+    out->memory_total_bytes = static_cast<int64_t>(props.totalGlobalMem);
+    out->multiprocessor_count = props.multiProcessorCount;
+    out->major = props.major;
+    out->minor = props.minor;
+    out->wavefront_size = props.warpSize;
+  });
+}
+
+int gh_host_alloc(size_t bytes, void** out) {
+  return guarded([&] {
+    if (!out) throw gh::ValueError("output pointer is NULL");
+    *out = nullptr;
+    if (bytes == 0) throw gh::ValueError("pinned host allocation must be non-empty");
+    check(hipHostMalloc(out, bytes, hipHostMallocPortable), "hipHostMalloc");
+  });
+}
+
+int gh_host_free(void* ptr) {
+  return guarded([&] {
+    if (ptr) check(hipHostFree(ptr), "hipHostFree");
+  });
+}
+
 int gh_stack_create(int64_t frame_count, int64_t height, int64_t width, gh_stack** out) {
   return guarded([&] {
     if (!out) throw gh::ValueError("output handle pointer is NULL");

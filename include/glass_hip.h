@@ -121,6 +121,26 @@ int gh_version(void);
 /* Number of visible HIP devices (cudaGetDeviceCount, native_bindings.cpp:15572). */
 int gh_device_count(int* count);
 
+typedef struct gh_device_properties {
+  char name[256];             /* marketing name */
+  char arch[64];              /* gcnArchName, e.g. "gfx950:sramecc+:xnack-" */
+  int64_t memory_total_bytes;
+  int32_t multiprocessor_count; /* compute units */
+  int32_t major;
+  int32_t minor;
+  int32_t wavefront_size;
+} gh_device_properties;
+
+/* Properties of one device: replaces get_device_info / list_devices
+ * (device_info_dict, native_bindings.cpp:1932-1944, :15576-15594).
+ * GH_ERR_INDEX for an id outside [0, device count). */
+int gh_device_info(int device, gh_device_properties* out);
+
+/* Pinned (page-locked, portable) host memory: replaces host_pinned_empty_f32 /
+ * host_pinned_empty_u8 (native_bindings.cpp:1946-1990).  GH_ERR_VALUE for 0 bytes. */
+int gh_host_alloc(size_t bytes, void** out);
+int gh_host_free(void* ptr);
+
 /* ---- lifecycle: ResidentCalibratedStack(frame_count, height, width)
  *      (native_bindings.cpp:2917-2934, destructor :2939-2996) ----------------- */
 int gh_stack_create(int64_t frame_count, int64_t height, int64_t width, gh_stack** out);
