@@ -2,35 +2,39 @@
 caller buffers (pinned host memory for the upload ring) and the direct output
 writer.
 
-Follows the reference's bounded fast path (src/glass/io/fits_fast.py:44-165,
-:206-233, :379-437) and its direct primary-HDU writer
-(src/glass/io/fits_io.py:9-16, :95-151; count maps rounded with np.rint,
-resident_cuda.py:4798-4800):
-* the primary header is read in 2880-byte blocks of 80-character cards up to
-  END; the first occurrence of a keyword wins; values parse as quoted strings
-  (with doubled quotes), T/F, integers or floats (D exponents accepted);
-* a simple 2-D image (SIMPLE = T, NAXIS = 2, PCOUNT = 0, GCOUNT = 1) with a
-  known BITPIX, data right after the header;
-* the GPU decode path takes BITPIX = 16, BSCALE = 1, BZERO = 32768, no BLANK:
-  its data bytes are handed over undecoded;
-* written files hold SIMPLE/BITPIX/NAXIS/NAXIS1/NAXIS2, the caller's cards,
-  END, big-endian data and zero padding to a 2880-byte multiple, written to a
-  temporary name and renamed.
+The reader is written from the FITS standard (version 4.0, sections 3.1-3.3,
+4.1-4.2 and 5.2) for the files the reference's bounded fast path accepts
+(src/glass/io/fits_fast.py:44-165, :206-233): a primary HDU holding one 2-D
+image.  The header is a sequence of 2880-byte blocks of 80-byte keyword
+records, ended by the END record; the data begin at the next block boundary.
+Only the first record of a keyword counts.  Values follow the fixed/free
+format grammar of section 4.2: quoted strings ('' is a literal quote,
+trailing blanks are not significant), logicals T/F, integers and reals (E or D
+exponent); a '/' outside a string starts the comment.
+
+The writer (src/glass/io/fits_io.py:9-16, :95-151; count maps rounded with
+np.rint, resident_cuda.py:4798-4800) emits SIMPLE/BITPIX/NAXIS/NAXIS1/NAXIS2,
+the caller's records, END, big-endian data and zero padding to a 2880-byte
+multiple, through a temporary name and a rename.
 """
 
 from __future__ import annotations
 
 import os
-from dataclasses import dataclass
+import re
+from dataclasses import dataclass, field
 from pathlib import Path
 from typing import Any
 
 import numpy as np
 
-BLOCK = 2880
-CARD = 80
+BLOCK = 2880   # bytes per header / data block (standard section 3.1)
+CARD = 80      # bytes per keyword record
+_CARDS_PER_BLOCK = BLOCK // CARD
 
-_BITPIX_DTYPES = {8: "u1", 16: ">i2", 32: ">i4", 64: ">i8", -32: ">f4", -64: ">f8"}
+# BITPIX -> big-endian storage type (standard table 8).
+_STORAGE = {8: np.dtype("u1"), 16: np.dtype(">i2"), 32: np.dtype(">i4"), 64: np.dtype(">i8"),
+            -32: np.dtype(">f4"), -64: np.dtype(">f8")}
 _WRITE_DTYPES = {
     np.dtype(np.uint8): (8, ">u1"),
     np.dtype(np.int16): (16, ">i2"),
@@ -45,154 +49,165 @@ class FitsUnsupported(ValueError):
     """The file is not a simple image this reader handles."""
 
 
+# Value field of a keyword record (bytes 11-80), standard section 4.2.
+_VALUE = re.compile(
+    r"""\s*(?:
+        '(?P<string>(?:[^']|'')*)'
+      | (?P<logical>[TF])(?=\s|/|$)
+      | (?P<int>[+-]?\d+)(?=\s|/|$)
+      | (?P<real>[+-]?(?:\d+\.?\d*|\.\d+)(?:[EeDd][+-]?\d+)?)(?=\s|/|$)
+    )?\s*(?P<rest>.*)$""",
+    re.X,
+)
+
+
+def parse_card_value(card: str) -> Any:
+    """Value of one keyword record, or None for commentary records and
+    undefined values.  A value outside the grammar is returned as its text."""
+    if len(card) < 10 or card[8] != "=":
+        return None
+    m = _VALUE.match(card[10:])
+    if m.group("string") is not None:
+        return m.group("string").replace("''", "'").rstrip()
+    if m.group("logical") is not None:
+        return m.group("logical") == "T"
+    if m.group("int") is not None:
+        return int(m.group("int"))
+    if m.group("real") is not None:
+        return float(m.group("real").translate(str.maketrans("Dd", "Ee")))
+    rest = m.group("rest")
+    text = rest.split("/", 1)[0].strip() if not rest.startswith("/") else ""
+    return text or None
+
+
+@dataclass
+class FitsHeader:
+    """Keyword -> value of a primary header (first record of each keyword
+    wins) and the byte offset of the data that follow it."""
+
+    values: dict[str, Any] = field(default_factory=dict)
+    data_offset: int = 0
+
+    def number(self, key: str, default: float) -> float:
+        v = self.values.get(key, default)
+        return default if v is None or isinstance(v, (bool, str)) else v
+
+
+def _keyword_records(handle):
+    """Yield the 80-byte records of consecutive header blocks."""
+    while True:
+        block = handle.read(BLOCK)
+        if len(block) != BLOCK:
+            raise FitsUnsupported("FITS header ends inside a 2880-byte block (truncated file)"
+                                  if block else "FITS header has no END record")
+        for k in range(_CARDS_PER_BLOCK):
+            yield block[k * CARD:(k + 1) * CARD]
+
+
+def read_header(path) -> FitsHeader:
+    header = FitsHeader()
+    with open(path, "rb") as handle:
+        for count, record in enumerate(_keyword_records(handle), start=1):
+            if not record.isascii():
+                raise FitsUnsupported("FITS header record holds non-ASCII bytes")
+            card = record.decode("ascii")
+            key = card[:8].rstrip()
+            if key == "END":
+                blocks = -(-count // _CARDS_PER_BLOCK)
+                header.data_offset = blocks * BLOCK
+                return header
+            if key and key not in header.values:
+                value = parse_card_value(card)
+                if value is not None:
+                    header.values[key] = value
+    raise AssertionError("unreachable")
+
+
+def read_primary_header(path) -> tuple[dict[str, Any], int]:
+    """(keyword -> value, header bytes) of the primary HDU."""
+    h = read_header(path)
+    return h.values, h.data_offset
+
+
 @dataclass(frozen=True)
-class SimpleFitsImageSpec:
+class ImageLayout:
+    """Where and how a primary 2-D image is stored."""
+
     path: Path
+    shape: tuple[int, int]     # (NAXIS2, NAXIS1) = (height, width)
     bitpix: int
-    width: int
-    height: int
     data_offset: int
-    dtype: np.dtype
     bscale: float = 1.0
     bzero: float = 0.0
     blank: Any = None
 
     @property
-    def shape(self) -> tuple[int, int]:
-        return self.height, self.width
+    def height(self) -> int:
+        return self.shape[0]
+
+    @property
+    def width(self) -> int:
+        return self.shape[1]
+
+    @property
+    def dtype(self) -> np.dtype:
+        return _STORAGE[self.bitpix]
+
+    @property
+    def data_bytes(self) -> int:
+        return self.shape[0] * self.shape[1] * self.dtype.itemsize
 
 
-def _value_field(field: str) -> str:
-    """The value part of a card (before an unquoted '/')."""
-    out = []
-    quoted = False
-    i = 0
-    while i < len(field):
-        ch = field[i]
-        if ch == "'":
-            out.append(ch)
-            if quoted and i + 1 < len(field) and field[i + 1] == "'":
-                out.append("'")
-                i += 2
-                continue
-            quoted = not quoted
-        elif ch == "/" and not quoted:
-            break
-        else:
-            out.append(ch)
-        i += 1
-    return "".join(out).strip()
-
-
-def parse_card_value(card: str) -> Any:
-    if len(card) < 10 or card[8] != "=":
-        return None
-    raw = _value_field(card[10:])
-    if not raw:
-        return None
-    if raw[0] == "'":
-        chars = []
-        i = 1
-        while i < len(raw):
-            if raw[i] == "'":
-                if i + 1 < len(raw) and raw[i + 1] == "'":
-                    chars.append("'")
-                    i += 2
-                    continue
-                break
-            chars.append(raw[i])
-            i += 1
-        return "".join(chars).rstrip()
-    if raw == "T":
-        return True
-    if raw == "F":
-        return False
-    text = raw.replace("D", "E").replace("d", "e")
-    try:
-        if any(t in text for t in (".", "E", "e")):
-            return float(text)
-        return int(text)
-    except ValueError:
-        return raw
-
-
-def read_primary_header(path) -> tuple[dict[str, Any], int]:
-    """(first value of each keyword, bytes of header blocks)."""
-    header: dict[str, Any] = {}
-    consumed = 0
-    with open(path, "rb") as handle:
-        while True:
-            block = handle.read(BLOCK)
-            if not block:
-                break
-            if len(block) != BLOCK:
-                raise FitsUnsupported("truncated FITS header block")
-            consumed += BLOCK
-            for off in range(0, BLOCK, CARD):
-                try:
-                    card = block[off:off + CARD].decode("ascii", errors="strict")
-                except UnicodeDecodeError as exc:
-                    raise FitsUnsupported("FITS primary header is not ASCII") from exc
-                key = card[:8].strip()
-                if key == "END":
-                    return header, consumed
-                if key and key not in header:
-                    value = parse_card_value(card)
-                    if value is not None:
-                        header[key] = value
-    raise FitsUnsupported("FITS primary header END card not found")
-
-
-def simple_fits_image_spec(path) -> SimpleFitsImageSpec:
+def simple_fits_image_spec(path) -> ImageLayout:
+    """Probe a primary-HDU 2-D image; FitsUnsupported for anything else."""
     target = Path(path)
-    header, data_offset = read_primary_header(target)
-    if header.get("SIMPLE") is not True:
-        raise FitsUnsupported("not a simple primary FITS image")
-    if int(header.get("NAXIS", -1)) != 2:
-        raise FitsUnsupported("only 2D primary images are supported")
-    bitpix = int(header.get("BITPIX", 0))
-    if bitpix not in _BITPIX_DTYPES:
+    h = read_header(target)
+    v = h.values
+    naxis = v.get("NAXIS")
+    if v.get("SIMPLE") is not True or naxis is None:
+        raise FitsUnsupported("file does not start a conforming primary HDU (SIMPLE = T, NAXIS)")
+    if naxis != 2:
+        raise FitsUnsupported(f"primary HDU has NAXIS = {naxis}; only 2-D images are read")
+    bitpix = v.get("BITPIX")
+    if bitpix not in _STORAGE:
         raise FitsUnsupported(f"unsupported BITPIX={bitpix}")
-    width, height = int(header.get("NAXIS1", 0)), int(header.get("NAXIS2", 0))
-    if width <= 0 or height <= 0:
-        raise FitsUnsupported("invalid NAXIS1/NAXIS2 dimensions")
-    if int(header.get("PCOUNT", 0) or 0) != 0 or int(header.get("GCOUNT", 1) or 1) != 1:
-        raise FitsUnsupported("random-groups FITS images are not supported")
-    dtype = np.dtype(_BITPIX_DTYPES[bitpix])
-    if target.stat().st_size < data_offset + width * height * dtype.itemsize:
-        raise FitsUnsupported("FITS image data is truncated")
-    return SimpleFitsImageSpec(target, bitpix, width, height, data_offset, dtype,
-                               float(header.get("BSCALE", 1.0) or 1.0), float(header.get("BZERO", 0.0) or 0.0),
-                               header.get("BLANK"))
+    width, height = v.get("NAXIS1"), v.get("NAXIS2")
+    if not (isinstance(width, int) and isinstance(height, int) and width > 0 and height > 0):
+        raise FitsUnsupported(f"image axes NAXIS1={width}, NAXIS2={height} are not positive integers")
+    if v.get("GROUPS") is True or h.number("PCOUNT", 0) != 0 or h.number("GCOUNT", 1) != 1:
+        raise FitsUnsupported("random-groups records (GROUPS/PCOUNT/GCOUNT) are not images")
+    layout = ImageLayout(target, (height, width), bitpix, h.data_offset,
+                         float(h.number("BSCALE", 1.0) or 1.0), float(h.number("BZERO", 0.0) or 0.0),
+                         v.get("BLANK"))
+    if target.stat().st_size < layout.data_offset + layout.data_bytes:
+        raise FitsUnsupported("file is shorter than its header says (truncated image data)")
+    return layout
 
 
-def u16_gpu_eligibility(spec: SimpleFitsImageSpec, expected_shape=None) -> dict[str, Any]:
+def u16_gpu_eligibility(spec: ImageLayout, expected_shape=None) -> dict[str, Any]:
     """Whether the raw bytes can go to the GPU undecoded (BITPIX 16, BSCALE 1,
     BZERO 32768, no BLANK, expected shape)."""
-    reason = ""
-    if expected_shape is not None and spec.shape != tuple(expected_shape):
-        reason = "shape_mismatch"
-    elif spec.bitpix != 16:
-        reason = f"bitpix_not_16:{spec.bitpix}"
-    elif spec.bscale != 1.0:
-        reason = f"bscale_not_1:{spec.bscale:g}"
-    elif spec.bzero != 32768.0:
-        reason = f"bzero_not_32768:{spec.bzero:g}"
-    elif spec.blank is not None:
-        reason = "blank_present"
+    checks = (
+        (expected_shape is not None and spec.shape != tuple(expected_shape), "shape_mismatch"),
+        (spec.bitpix != 16, f"bitpix_not_16:{spec.bitpix}"),
+        (spec.bscale != 1.0, f"bscale_not_1:{spec.bscale:g}"),
+        (spec.bzero != 32768.0, f"bzero_not_32768:{spec.bzero:g}"),
+        (spec.blank is not None, "blank_present"),
+    )
+    reason = next((why for failed, why in checks if failed), "")
     return {"path": str(spec.path), "eligible": not reason, "reason": reason, "bitpix": spec.bitpix,
             "bscale": spec.bscale, "bzero": spec.bzero, "blank_present": spec.blank is not None,
             "shape": {"height": spec.height, "width": spec.width}, "data_offset": spec.data_offset}
 
 
-def read_u16be_raw_into(spec: SimpleFitsImageSpec, out) -> int:
+def read_u16be_raw_into(spec: ImageLayout, out) -> int:
     """Read the undecoded big-endian data bytes of an eligible image into the
     writable contiguous byte buffer `out` (numpy array or memoryview, e.g. a
     pinned host tensor's .numpy()).  Returns the bytes read."""
     elig = u16_gpu_eligibility(spec)
     if not elig["eligible"]:
         raise FitsUnsupported(f"not eligible for the raw u16 path: {elig['reason']}")
-    nbytes = spec.width * spec.height * 2
+    nbytes = spec.data_bytes
     view = memoryview(out).cast("B")
     if view.nbytes < nbytes:
         raise ValueError("output buffer is smaller than the image data")
@@ -202,7 +217,7 @@ def read_u16be_raw_into(spec: SimpleFitsImageSpec, out) -> int:
         while done < nbytes:
             got = os.preadv(fd, [view[done:nbytes]], spec.data_offset + done)
             if got <= 0:
-                raise FitsUnsupported("FITS image data is truncated")
+                raise FitsUnsupported("file is shorter than its header says (truncated image data)")
             done += got
     finally:
         os.close(fd)

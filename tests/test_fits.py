@@ -74,6 +74,40 @@ def test_card_grammar():
     assert F.parse_card_value("KEY     = 'a/b'") == "a/b"
     assert F.parse_card_value("COMMENT this is no value card") is None
     assert F.parse_card_value("KEY     =   / empty") is None
+    assert F.parse_card_value("KEY     =                    T/no blank before the comment") is True
+    assert F.parse_card_value("KEY     =                 .5e1") == 5.0
+    assert F.parse_card_value("KEY     = 'trailing   '") == "trailing"
+    assert F.parse_card_value("KEY     = '  leading'") == "  leading"
+    assert F.parse_card_value("KEY     = (1.0, 2.0) / complex stays text") == "(1.0, 2.0)"
+    assert F.parse_card_value("KEY     =                  +12") == 12
+
+
+def test_non_image_headers_are_refused(tmp_path):
+    def write(name, cards):
+        text = "".join(F._format_card(k, v) for k, v in cards) + "END".ljust(80)
+        path = tmp_path / name
+        path.write_bytes(text.encode().ljust(2880) + bytes(2880 * 2))
+        return path
+
+    cube = write("cube.fits", [("SIMPLE", True), ("BITPIX", 8), ("NAXIS", 3), ("NAXIS1", 2), ("NAXIS2", 2),
+                               ("NAXIS3", 2)])
+    with pytest.raises(F.FitsUnsupported, match="2-D"):
+        F.simple_fits_image_spec(cube)
+    groups = write("groups.fits", [("SIMPLE", True), ("BITPIX", 8), ("NAXIS", 2), ("NAXIS1", 2),
+                                   ("NAXIS2", 2), ("GROUPS", True), ("PCOUNT", 1), ("GCOUNT", 1)])
+    with pytest.raises(F.FitsUnsupported, match="random-groups"):
+        F.simple_fits_image_spec(groups)
+    not_simple = write("ns.fits", [("SIMPLE", False), ("BITPIX", 8), ("NAXIS", 2), ("NAXIS1", 2), ("NAXIS2", 2)])
+    with pytest.raises(F.FitsUnsupported, match="SIMPLE"):
+        F.simple_fits_image_spec(not_simple)
+    bad_bitpix = write("bp.fits", [("SIMPLE", True), ("BITPIX", 12), ("NAXIS", 2), ("NAXIS1", 2), ("NAXIS2", 2)])
+    with pytest.raises(F.FitsUnsupported, match="BITPIX"):
+        F.simple_fits_image_spec(bad_bitpix)
+    # 36 records fill the first block exactly: END opens the second, the data the third
+    many = [("SIMPLE", True), ("BITPIX", 8), ("NAXIS", 2), ("NAXIS1", 2), ("NAXIS2", 2)] + \
+        [(f"K{i}", i) for i in range(31)]
+    spec = F.simple_fits_image_spec(write("many.fits", many))
+    assert spec.data_offset == 2 * 2880 and spec.shape == (2, 2)
 
 
 def test_header_errors(tmp_path):
