@@ -434,7 +434,7 @@ int gh_integrate(gh_stack* s, const float* weights, const gh_integrate_params* p
       throw gh::ValueError("count_map_dtype must be float32 or uint16");
     hipStream_t st = s->pick(stream_ptr);
 
-    // Active frames: finite, positive weights (cpu/integration.py:143), frame order.
+    // Active frames: finite, positive weights (cpu/integration.py:67), frame order.
     std::vector<const float*> table;
     std::vector<float> w;
     bool unit = true;

@@ -1,7 +1,7 @@
 // Winsorized sigma-clip rejection + weighted-mean integration along the frame
 // axis for gfx950 (one thread per output pixel).
 //
-// Oracle: src/glass/cpu/integration.py:112-194 (weighted_integrate_stack) with
+// Oracle: src/glass/cpu/integration.py:36-118 (weighted_integrate_stack) with
 // src/glass/engine/rejection.py:31-366 (center_and_scale); the reference's
 // CPU-parity CUDA kernel is cpp/cuda/integration_kernels.cu:2292-2779.
 //
@@ -678,7 +678,7 @@ __global__ __launch_bounds__(kBlock) void integrate_winsorized_lds_kernel(Integr
 }
 
 // Rejection-free weighted mean for any frame count (integrate_mean,
-// cpp/cuda/integration_kernels.cu:1610-1656; oracle cpu/integration.py:182-194).
+// cpp/cuda/integration_kernels.cu:1610-1656; oracle cpu/integration.py:106-111).
 template <bool UNIT>
 __global__ __launch_bounds__(kBlock) void integrate_mean_kernel(IntegrateArgs a) {
   const long long p = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;

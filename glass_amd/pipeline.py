@@ -11,10 +11,11 @@ Semantics follow the CPU path, which is the oracle:
 * light calibration: src/glass/cpu/calibration.py:51-76;
 * warp: identity matrices skip resampling (resident_cuda.py:2751-2754), others
   are resampled on the reference grid; geometric coverage is accumulated;
-* weights: engine/integration.py:80-106 (none / variance_aware / simple_snr),
-  with zero weight for masked frames;
+* weights: none, variance_aware (engine/integration.py:80-106) or the resident
+  simple_snr rule (resident_cuda.py:1948-1955), zero weight for masked frames;
 * optional resident local normalisation (glass_amd.local_norm);
-* integration: cpu/integration.py:112-194; DQ: resident_cuda.py:3110-3297.
+* integration: cpu/integration.py:36-118 and the DQ map (resident_cuda.py:
+  3110-3297) in one launch.
 All arithmetic runs in libglass_hip.so; numpy only shapes host inputs/outputs.
 """
 
@@ -184,6 +185,25 @@ def variance_aware_weights(rms: Sequence[float], active: Sequence[bool] | None =
     return weights
 
 
+def simple_snr_weights(stats: Sequence[dict], active: Sequence[bool] | None = None,
+                       eps: float = 1.0e-6) -> np.ndarray:
+    """The resident ``simple_snr`` rule (``_simple_snr_weight_from_stats``,
+    resident_cuda.py:1948-1955, frame loop :12762-12779) on frame_global_stats
+    of the calibrated, unwarped frames: 0 without valid pixels, 1/eps when
+    std <= eps, else max(|mean|/std, eps); inactive frames keep weight 0 and
+    their statistics are not read."""
+    out = np.zeros(len(stats), dtype=np.float32)
+    for i, st in enumerate(stats):
+        if active is not None and not bool(active[i]):
+            continue
+        if int(st.get("valid_pixels") or 0) <= 0:
+            continue
+        std = float(st.get("std") or 0.0)
+        mean = abs(float(st.get("mean") or 0.0))
+        out[i] = 1.0 / eps if std <= eps else max(mean / std, eps)
+    return out
+
+
 def _is_identity(matrix) -> bool:
     return bool(np.allclose(np.asarray(matrix, dtype=np.float64), np.eye(3), atol=0.0, rtol=0.0))
 
@@ -265,6 +285,8 @@ def run_resident_stack(
                 weights = np.where(np.asarray(active, bool), weights, np.float32(0.0)).astype(np.float32)
         elif weighting == "variance_aware":
             weights = variance_aware_weights([s["std"] for s in stats], active)
+        elif weighting == "simple_snr":
+            weights = simple_snr_weights(stats, active)
         else:
             raise ValueError(f"unsupported weighting: {weighting}")
         timing_ln = 0.0
@@ -274,13 +296,11 @@ def run_resident_stack(
                                               local_normalization_tile_size)
             weights = ln["weights"]
             timing_ln = time.perf_counter() - t_ln
-        n_active = int(np.count_nonzero(weights > 0))
-        master, weight, coverage, low, high, prof = stack.integrate_hardened_winsorized_sigma_timed(
-            weights, low_sigma, high_sigma, min_samples, max_reject_fraction)
+        maps = integrate_with_dq(stack, weights, low_sigma, high_sigma, min_samples, max_reject_fraction)
         geometric = stack.warp_coverage_map()
         t_int = time.perf_counter()
-        dq = dq_map(stack, weights, low_sigma, high_sigma, min_samples, max_reject_fraction, n_active)
-        return StackResult(master, weight, coverage, low, high, dq, geometric, weights,
+        return StackResult(maps["master"], maps["weight"], maps["coverage"], maps["low"], maps["high"],
+                           maps["dq"], geometric, weights,
                            {"calibrate_s": t_cal - t0, "warp_s": t_warp - t_cal, "local_norm_s": timing_ln,
                             "integrate_s": t_int - t_warp - timing_ln})
     finally:
@@ -334,14 +354,23 @@ def write_resident_outputs(result: StackResult, output_dir, filter_name: str, fr
             "total_s": time.perf_counter() - t0, "workers": workers}
 
 
-def dq_map(stack: ResidentCalibratedStack, weights, low_sigma, high_sigma, min_samples, max_reject_fraction,
-           active_frames: int) -> np.ndarray:
-    """Resident DQ map computed by the integration kernel itself (fused)."""
+def integrate_with_dq(stack: ResidentCalibratedStack, weights, low_sigma=3.0, high_sigma=3.0, min_samples=3,
+                      max_reject_fraction=0.5) -> dict:
+    """One winsorized integration launch that returns the five maps of
+    ``integrate_hardened_winsorized_sigma`` and the resident DQ map
+    (resident_cuda.py:3110-3297, computed by the same kernel from its own
+    counts, the geometric warp coverage and the active frame count)."""
     from . import _native as N
-    import ctypes
 
+    if not (float(low_sigma) > 0 and float(high_sigma) > 0):
+        raise ValueError("low_sigma and high_sigma must be positive")
+    if int(min_samples) < 1:
+        raise ValueError("min_samples must be at least 1")
+    if not 0.0 <= float(max_reject_fraction) <= 1.0:
+        raise ValueError("max_reject_fraction must be between 0 and 1")
+    w = np.asarray(weights, dtype=np.float32)
+    active = int(np.count_nonzero(np.isfinite(w) & (w > 0)))
     params = N.IntegrateParams(N.GH_REJECT_WINSORIZED_SIGMA, float(low_sigma), float(high_sigma), int(min_samples),
                                float(max_reject_fraction), N.GH_COUNT_F32, -1, 0, 0)
-    out = stack._integrate(weights, params, set(), N.GH_COUNT_F32, dq=True, dq_geometric=True,
-                           dq_active_frames=active_frames)
-    return out["dq"]
+    return stack._integrate(w, params, {"master", "weight", "coverage", "low", "high"}, N.GH_COUNT_F32, dq=True,
+                            dq_geometric=True, dq_active_frames=active)

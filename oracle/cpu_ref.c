@@ -7,7 +7,7 @@
  *   calibrate:   src/glass/cpu/calibration.py:51-76 (f32 ops, no contraction)
  *   warp:        src/glass/engine/warp.py:155-228, :99-118, :121-152
  *                (FP64 coordinates and Lanczos3 weights, FP64 sums)
- *   integrate:   src/glass/cpu/integration.py:112-194 +
+ *   integrate:   src/glass/cpu/integration.py:36-118 +
  *                src/glass/engine/rejection.py:242-320 (per-pixel valid-count
  *                arithmetic; linear percentiles; FP64 sums)
  * Build: gcc -O3 -ffp-contract=off -fopenmp -shared -fPIC (oracle/cpu_ref.py).

@@ -1,6 +1,6 @@
 """Rejection + weighted-mean integration oracle (test infrastructure).
 
-* ``weighted_integrate_stack`` restates src/glass/cpu/integration.py:112-194 --
+* ``weighted_integrate_stack`` restates src/glass/cpu/integration.py:36-118 --
   the compact CPU baseline the reference resident tests compare against
   (tests/test_cuda_resident_stack.py:2871-2932).
 * ``stack_engine_combine`` restates the tile loop of CPUStackEngine.stack
@@ -33,7 +33,7 @@ def mean_integrate(frames, weights=None):
 
 
 def _guard(valid, low, high, min_samples, max_reject_fraction):
-    """Per-pixel rejection guard (cpu/integration.py:160-175)."""
+    """Per-pixel rejection guard (cpu/integration.py:85-99)."""
     rejected = low | high
     if np.any(rejected):
         total = np.sum(valid, axis=0)
@@ -61,7 +61,7 @@ def weighted_integrate_stack(
     min_samples=3,
     max_reject_fraction=0.5,
 ):
-    """src/glass/cpu/integration.py:112-194.
+    """src/glass/cpu/integration.py:36-118.
 
     Returns (master, weight_map, coverage_map, low_reject, high_reject), all f32.
     """

@@ -76,3 +76,24 @@ def test_variance_aware_weights_match_oracle():
     assert np.allclose(ours, np.asarray(ref, np.float32))
     masked = variance_aware_weights(rms, [True, False, True, True])
     assert masked[1] == 0.0 and masked[0] == ours[0]
+
+
+def test_simple_snr_weights_rule():
+    """resident_cuda.py:1948-1955: closed forms of the rule, and the driver's
+    vectorised form equals the oracle's per-frame function."""
+    import oracle
+    from glass_amd.pipeline import simple_snr_weights
+
+    cases = [
+        ({"valid_pixels": 0, "mean": 5.0, "std": 1.0}, 0.0),       # no pixels -> weight 0
+        ({"valid_pixels": 10, "mean": 5.0, "std": 0.0}, 1.0e6),    # std <= eps -> 1/eps
+        ({"valid_pixels": 10, "mean": 5.0, "std": 1.0e-6}, 1.0e6),
+        ({"valid_pixels": 10, "mean": -6.0, "std": 2.0}, 3.0),     # |mean| / std
+        ({"valid_pixels": 10, "mean": 1.0e-12, "std": 1.0}, 1.0e-6),  # floored at eps
+    ]
+    for stats, expected in cases:
+        assert oracle.simple_snr_weight(stats) == pytest.approx(expected, rel=1e-12)
+    got = simple_snr_weights([c[0] for c in cases])
+    assert np.array_equal(got, np.asarray([oracle.simple_snr_weight(c[0]) for c in cases], np.float32))
+    masked = simple_snr_weights([c[0] for c in cases], [True, True, False, True, True])
+    assert masked[2] == 0.0 and masked[3] == np.float32(3.0)

@@ -554,6 +554,40 @@ def test_end_to_end_config0_and_dq(gpu):
     assert np.array_equal(res.dq, edq)
 
 
+def test_end_to_end_simple_snr_weights_and_inactive_frame(gpu):
+    # The resident simple_snr rule (resident_cuda.py:1948-1955, :12762-12779)
+    # from frame_global_stats of the calibrated frames, one frame masked, a
+    # bilinear warp; the five maps and DQ come from one integration launch.
+    data = make_set(seed=5, n_lights=7, height=40, width=56, n_cal=3, n_stars=10)
+    pol = {"tile_size": 512}
+    mb = P.build_master_bias(data.bias, pol)
+    md = P.build_master_dark(data.dark, mb, pol)
+    mf, _ = P.build_master_flat(data.flat, mb, pol)
+    active = [True, True, True, False, True, True, True]
+    res = P.run_resident_stack(data.light, [data.light_exposure_s] * 7, {"bias": mb, "dark": md, "flat": mf},
+                               data.matrices, dark_exposure_s=data.dark_exposure_s, interpolation="bilinear",
+                               weighting="simple_snr", active=active)
+    cal = [oracle.calibrate_light(l, mb, md, mf, data.light_exposure_s, data.dark_exposure_s) for l in data.light]
+    ew = np.asarray([oracle.simple_snr_weight(oracle.frame_global_stats(c)) if a else 0.0
+                     for c, a in zip(cal, active)], np.float32)
+    assert np.allclose(res.weights, ew, rtol=1e-6, atol=0) and res.weights[3] == 0.0
+    warped, covs = [], []
+    for i, c in enumerate(cal):
+        if np.array_equal(data.matrices[i], np.eye(3)):
+            warped.append(c)
+            covs.append(np.ones_like(c))
+        else:
+            wimg, cov = oracle.warp_matrix(c, data.matrices[i], fill=0.0, interpolation="bilinear")
+            warped.append(wimg)
+            covs.append(cov)
+    exp = oracle.weighted_integrate_stack(np.stack(warped), weights=res.weights, coverage=np.stack(covs),
+                                          rejection="winsorized_sigma")
+    _compare((res.master, res.weight, res.coverage, res.low, res.high), exp)
+    edq = oracle.resident_dq_map(exp[0], exp[1], exp[2], exp[3], exp[4], np.sum(covs, axis=0).astype(np.float32),
+                                 active_frame_count=6)
+    assert np.array_equal(res.dq, edq)
+
+
 def test_end_to_end_config1_stagewise(gpu):
     # Config 1's recipe at a 256 x 256 crop (32 lights, 16 + 16 + 16 calibration
     # frames, raw u16 ADU, drift +-20 px / +-0.5 deg / scale 1 +- 5e-4, cosmic
