@@ -244,6 +244,8 @@ _SIGNATURES = {
     "gh_accumulate_full_warp_coverage": [c_void_p],
     "gh_warp_coverage_map": [c_void_p, c_void_p],
     "gh_integrate": [c_void_p, POINTER(c_float), POINTER(IntegrateParams), POINTER(IntegrateOutputs), c_void_p],
+    "gh_integrate_noalloc": [c_void_p, POINTER(c_float), POINTER(IntegrateParams), POINTER(IntegrateOutputs),
+                             c_void_p],
     "gh_frame_stats": [c_void_p, c_int64, c_int32, POINTER(FrameStatistics)],
     "gh_integrate_clip": [
         c_void_p, POINTER(c_float), POINTER(c_double), POINTER(ClipParams), c_void_p, c_void_p, c_void_p, c_void_p,

@@ -80,6 +80,11 @@ int gh_stack_create(int64_t frame_count, int64_t height, int64_t width, gh_stack
       s->width = width;
       s->npix = height * width;
       check(hipStreamCreate(&s->stream), "hipStreamCreate(resident stack)");
+      // Created here, not on first use, so recording them never creates a
+      // runtime object inside a call (the no-allocation entry point records
+      // caller_done).
+      check(hipEventCreateWithFlags(&s->caller_done, hipEventDisableTiming), "hipEventCreate(caller stream)");
+      check(hipEventCreateWithFlags(&s->tables_done, hipEventDisableTiming), "hipEventCreate(integration tables)");
       check(hipMalloc(reinterpret_cast<void**>(&s->pool), static_cast<size_t>(frame_count) * s->frame_bytes()),
             "hipMalloc(resident calibrated stack)");
       check(hipMalloc(reinterpret_cast<void**>(&s->nonfinite), static_cast<size_t>(frame_count) * sizeof(unsigned)),
@@ -414,9 +419,13 @@ int gh_warp_coverage_map(gh_stack* s, float* out) {
   });
 }
 
-int gh_integrate(gh_stack* s, const float* weights, const gh_integrate_params* params,
-                 const gh_integrate_outputs* outputs, void* stream_ptr) {
-  return guarded([&] {
+}  // extern "C"
+
+namespace ghi {
+
+void integrate_stack(gh_stack* s, const float* weights, const gh_integrate_params* params,
+                     const gh_integrate_outputs* outputs, void* stream_ptr, const IntegrationScratch* scratch) {
+    if (!s) throw gh::ValueError("stack handle is NULL");
     if (!params || !outputs) throw gh::ValueError("params and outputs must not be NULL");
     if (s->loaded_count != s->frames)
       throw std::runtime_error("all resident frames must be loaded before integration");
@@ -456,12 +465,26 @@ int gh_integrate(gh_stack* s, const float* weights, const gh_integrate_params* p
     const float* pad_ptr = n_active ? table.back() : s->slot[0];
     table.resize(padded, pad_ptr);
     w.resize(padded, 0.0f);
-    s->wait_tables();
-    s->frame_table.ensure(padded, "hipMalloc(frame table)");
-    s->frame_weights.ensure(padded, "hipMalloc(frame weights)");
-    check(hipMemcpy(s->frame_table.ptr, table.data(), padded * sizeof(const float*), hipMemcpyHostToDevice),
+    const float** d_table;
+    float* d_weights;
+    if (scratch) {
+      if (padded > scratch->table_entries)
+        throw gh::ValueError("frame table exceeds the no-allocation scratch");
+      // Every call on this device shares the scratch: the last launch that
+      // read it on this stream must have finished before it is rewritten.
+      check(hipStreamSynchronize(st), "synchronize(integration scratch)");
+      d_table = scratch->frame_table;
+      d_weights = scratch->weights;
+    } else {
+      s->wait_tables();
+      s->frame_table.ensure(padded, "hipMalloc(frame table)");
+      s->frame_weights.ensure(padded, "hipMalloc(frame weights)");
+      d_table = s->frame_table.ptr;
+      d_weights = s->frame_weights.ptr;
+    }
+    check(hipMemcpy(d_table, table.data(), padded * sizeof(const float*), hipMemcpyHostToDevice),
           "hipMemcpy(frame table)");
-    check(hipMemcpy(s->frame_weights.ptr, w.data(), padded * sizeof(float), hipMemcpyHostToDevice),
+    check(hipMemcpy(d_weights, w.data(), padded * sizeof(float), hipMemcpyHostToDevice),
           "hipMemcpy(frame weights)");
 
     // Statistics arithmetic (rejection.py:39-44): all-valid only when every
@@ -478,8 +501,8 @@ int gh_integrate(gh_stack* s, const float* weights, const gh_integrate_params* p
       }
     }
     gh::IntegrateArgs a{};
-    a.frame_ptr = s->frame_table.ptr;
-    a.frame_w = s->frame_weights.ptr;
+    a.frame_ptr = d_table;
+    a.frame_w = d_weights;
     a.n_active = n_active;
     a.npix = s->npix;
     a.width = static_cast<int>(s->width);
@@ -496,10 +519,18 @@ int gh_integrate(gh_stack* s, const float* weights, const gh_integrate_params* p
     } else if (prm.tile_size > 0 && all_weights_positive && any_nonfinite) {
       const int tiles_x = static_cast<int>((s->width + prm.tile_size - 1) / prm.tile_size);
       const int tiles_y = static_cast<int>((s->height + prm.tile_size - 1) / prm.tile_size);
-      s->tile_flags.ensure(static_cast<size_t>(tiles_x) * tiles_y, "hipMalloc(tile flags)");
-      gh::launch_tile_all_finite(s->frame_table.ptr, n_active, static_cast<int>(s->height),
-                                 static_cast<int>(s->width), prm.tile_size, s->tile_flags.ptr, st);
-      a.tile_flags = s->tile_flags.ptr;
+      const size_t tiles = static_cast<size_t>(tiles_x) * tiles_y;
+      unsigned char* d_tiles;
+      if (scratch) {
+        if (tiles > scratch->tile_entries) throw gh::ValueError("tile grid exceeds the no-allocation scratch");
+        d_tiles = scratch->tile_flags;
+      } else {
+        s->tile_flags.ensure(tiles, "hipMalloc(tile flags)");
+        d_tiles = s->tile_flags.ptr;
+      }
+      gh::launch_tile_all_finite(d_table, n_active, static_cast<int>(s->height),
+                                 static_cast<int>(s->width), prm.tile_size, d_tiles, st);
+      a.tile_flags = d_tiles;
       a.tile_size = prm.tile_size;
       a.tiles_x = tiles_x;
     } else {
@@ -515,6 +546,7 @@ int gh_integrate(gh_stack* s, const float* weights, const gh_integrate_params* p
     auto route = [&](void* user, int slot, size_t elem_bytes) -> void* {
       if (!user) return nullptr;
       if (is_device_pointer(user)) return user;
+      if (scratch) throw gh::ValueError("the no-allocation integration writes device outputs only");
       host_dst[slot] = user;
       host_bytes[slot] = elem_bytes * static_cast<size_t>(s->npix);
       if (slot < 5) {
@@ -537,7 +569,7 @@ int gh_integrate(gh_stack* s, const float* weights, const gh_integrate_params* p
     }
     a.dq_active_frames = o.dq_active_frames;
     gh::launch_integrate(a, st);
-    s->mark_tables(st);
+    if (!scratch) s->mark_tables(st);
     const void* dev_src[6] = {a.master, a.weight, a.coverage, a.low, a.high, a.dq};
     bool any_host = false;
     for (int i = 0; i < 6; ++i) {
@@ -548,7 +580,15 @@ int gh_integrate(gh_stack* s, const float* weights, const gh_integrate_params* p
     }
     s->mark_caller(st);
     if (any_host || !stream_ptr) check(hipStreamSynchronize(st), "integrate synchronize");
-  });
+}
+
+}  // namespace ghi
+
+extern "C" {
+
+int gh_integrate(gh_stack* s, const float* weights, const gh_integrate_params* params,
+                 const gh_integrate_outputs* outputs, void* stream_ptr) {
+  return guarded([&] { ghi::integrate_stack(s, weights, params, outputs, stream_ptr, nullptr); });
 }
 
 int gh_frame_stats(gh_stack* s, int64_t index, int32_t exact_median, gh_frame_statistics* out) {

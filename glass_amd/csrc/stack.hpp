@@ -182,15 +182,17 @@ struct gh_stack {
   // caller stream may touch the coverage map, so they wait on this event,
   // recorded on that caller stream after such work.
   hipEvent_t caller_done = nullptr;
+  bool caller_recorded = false;
 
   void mark_caller(hipStream_t st) {
     if (st == stream) return;
     if (!caller_done) check(hipEventCreateWithFlags(&caller_done, hipEventDisableTiming), "hipEventCreate");
     check(hipEventRecord(caller_done, st), "hipEventRecord(caller stream)");
+    caller_recorded = true;
   }
 
   void wait_caller() {
-    if (caller_done) check(hipStreamWaitEvent(stream, caller_done, 0), "hipStreamWaitEvent(caller stream)");
+    if (caller_recorded) check(hipStreamWaitEvent(stream, caller_done, 0), "hipStreamWaitEvent(caller stream)");
   }
 
   // Device tables that the host rewrites before each integration launch
@@ -198,13 +200,16 @@ struct gh_stack {
   // for the launch that last read them, whichever stream it ran on.
   hipEvent_t tables_done = nullptr;
 
+  bool tables_recorded = false;
+
   void wait_tables() {
-    if (tables_done) check(hipEventSynchronize(tables_done), "integration tables in use");
+    if (tables_recorded) check(hipEventSynchronize(tables_done), "integration tables in use");
   }
 
   void mark_tables(hipStream_t st) {
     if (!tables_done) check(hipEventCreateWithFlags(&tables_done, hipEventDisableTiming), "hipEventCreate");
     check(hipEventRecord(tables_done, st), "hipEventRecord(integration tables)");
+    tables_recorded = true;
   }
 
   // Host raw frames go through a ring of device slots filled on a copy
@@ -324,4 +329,24 @@ struct gh_stack {
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
+
+namespace ghi {
+
+// Device memory an integration launch reads besides the frames and outputs:
+// the frame-pointer table, the frame weights and the per-tile statistics
+// flags.  integrate_stack takes them from here when `scratch` is given
+// (and then accepts device outputs only), else from the stack's own buffers,
+// grown on demand.
+struct IntegrationScratch {
+  const float** frame_table;
+  float* weights;
+  size_t table_entries;
+  unsigned char* tile_flags;
+  size_t tile_entries;
+};
+
+void integrate_stack(gh_stack* s, const float* weights, const gh_integrate_params* params,
+                     const gh_integrate_outputs* outputs, void* stream_ptr, const IntegrationScratch* scratch);
+
+}  // namespace ghi
 

@@ -238,6 +238,18 @@ int gh_warp_coverage_map(gh_stack* stack, float* out);
 int gh_integrate(gh_stack* stack, const float* weights, const gh_integrate_params* params,
                  const gh_integrate_outputs* outputs, void* stream);
 
+/* gh_integrate without device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (serving loops, code that
+ * must not call hipMalloc).  The frame-pointer table, frame weights and
+ * per-tile statistics flags live in a fixed library-owned device array of
+ * 208 KiB per GPU (up to 16384 positive-weight frames and 16384 statistics
+ * tiles; larger requests are GH_ERR_VALUE).  Every output must be device
+ * memory or NULL (host outputs are GH_ERR_VALUE).  Results equal gh_integrate's.
+ * All calls on one GPU share that array: calls that overlap on one GPU (other
+ * streams or threads) race; consecutive calls on one stream are ordered. */
+int gh_integrate_noalloc(gh_stack* stack, const float* weights, const gh_integrate_params* params,
+                         const gh_integrate_outputs* outputs, void* stream);
+
 /* Parameters of gh_integrate_clip. */
 typedef struct gh_clip_params {
   int32_t sampler;            /* GH_CLIP_STACK, GH_INTERP_BILINEAR or GH_INTERP_LANCZOS3 */
