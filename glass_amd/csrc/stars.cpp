@@ -2,7 +2,7 @@
 //
 // Star measurement follows src/glass/cpu/star_detect.py:45-118 on the window
 // patches gathered by the GPU candidate pass (stars.hip); the transform search
-// follows src/glass/cpu/registration.py:101-235, :316-371, :509-569 for the
+// follows src/glass/cpu/registration.py:101-233, :306-358, :476-566 for the
 // translation, similarity and affine models.  Everything here is scalar FP64
 // built without contraction, with numpy's summation order where a sum feeds a
 // result (pairwise_sum below), so star lists match the CPU path bit for bit.

@@ -9,7 +9,7 @@ Mirrors the reference operator surface that produces the warp matrices:
   resident frame without a copy.
 * ``estimate_star_transform(reference, moving, transform_model="translation",
   min_inliers=6, tolerance_px=3.0, max_stars=80)`` --
-  src/glass/cpu/registration.py:509-569 (translation, similarity, affine).
+  src/glass/cpu/registration.py:476-566 (translation, similarity, affine).
 
 Detection runs the background selection and the 3x3 local-maximum candidate
 pass on the GPU (libglass_hip.so) and measures the candidate windows in FP64
@@ -115,7 +115,7 @@ def _xy(stars) -> np.ndarray:
 
 def estimate_star_transform(reference, moving, transform_model: str = "translation", min_inliers: int = 6,
                             tolerance_px: float = 3.0, max_stars: int = 80) -> StarRegistration:
-    """registration.py:509-569 for the translation, similarity and affine models."""
+    """registration.py:476-566 for the translation, similarity and affine models."""
     if transform_model not in _MODELS:
         raise ValueError(f"unsupported star registration model: {transform_model}")
     ref, mov = _xy(reference), _xy(moving)

@@ -490,7 +490,7 @@ int gh_detect_stars(gh_stack* stack, int64_t index, double threshold_sigma, int6
                     int64_t origin_y, gh_star* out, int64_t capacity, int64_t* count, double* background_out);
 
 /* estimate_star_transform(reference, moving, transform_model, min_inliers,
- * tolerance_px, max_stars) (src/glass/cpu/registration.py:509-569) on star
+ * tolerance_px, max_stars) (src/glass/cpu/registration.py:476-566) on star
  * positions ref_xy[2*n_ref], mov_xy[2*n_mov] (brightest first), FP64, host
  * only.  model: GH_MODEL_*.  matrix[9] (row-major, moving -> reference);
  * pairs[2 * max_stars] receives (moving, reference) index pairs, *n_pairs the

@@ -30,9 +30,13 @@ Pinning status (what checks each function; see tests/test_oracle_*.py):
                                 star fields), closed forms (centroid of a
                                 centred star, float32 median pair average)
   registration.estimate_star_transform
-                             -- reference literal cases (translation with
-                                outlier, similarity), exact affine recovery;
-                                the lstsq / svd last bits are parity unpinned
+                             -- golden vectors of the reference's own
+                                detect_stars / estimate_star_transform
+                                (tests/golden/registration_golden.npz,
+                                bit-exact incl. lstsq / svd), reference
+                                literal cases, exact affine recovery
+  warp_matrix_resident clamp -- closed forms (step-edge overshoot lands on
+                                the bounds, ramps unchanged)
 """
 
 from .calibration import calibrate_light, decode_fits_u16be_bzero, decode_u16le

@@ -4,21 +4,21 @@ infrastructure; see oracle/__init__.py).
 Restates the reference CPU path that produces the warp matrices:
 
 * ``detect_stars`` -- src/glass/cpu/star_detect.py:121-168 with
-  ``robust_background`` (:28-42), ``_measure_candidate`` (:45-102) and
-  ``_dedupe_stars`` (:105-118).  numpy semantics the product must follow:
+  ``robust_background`` (:29-42), ``_measure_candidate`` (:45-100) and
+  ``_dedupe_stars`` (:103-118).  numpy semantics the product must follow:
   float32 medians average the middle pair in float32, the FP64 threshold is
   compared against the float32 image after rounding to float32 (weak Python
   scalar), candidate sums are FP64 over the contiguous patch in numpy's
   pairwise order, sorts are stable.
-* ``estimate_star_transform`` -- src/glass/cpu/registration.py:509-569 with
-  ``_translation_candidates`` (:219-235), ``_asterism_candidates`` (:346-371),
-  ``_triangle_descriptors`` (:316-343), ``_greedy_pairs`` (:177-205) and the
-  model fits (:101-171) for the translation, similarity and affine models.
+* ``estimate_star_transform`` -- src/glass/cpu/registration.py:476-566 with
+  ``_translation_candidates`` (:218-233), ``_asterism_candidates`` (:334-358),
+  ``_triangle_descriptors`` (:306-331), ``_greedy_pairs`` (:177-205) and the
+  model fits (:101-174) for the translation, similarity and affine models.
 
-The affine fit is ``np.linalg.lstsq`` (LAPACK gelsd) and the similarity fit a
-2x2 ``np.linalg.svd``: their last bits (and numpy's BLAS matrix products in
-``_apply_matrix``) are not pinned by any reference test, so matrices of those
-models are compared with a tolerance ("parity unpinned" at the last bit).
+Pinned bit for bit (lstsq / svd last bits, tie orders) by the golden vectors of
+the reference's own functions in tests/golden/registration_golden.npz
+(tests/golden/make_registration_golden.py), besides the reference tests'
+literal cases.
 """
 
 from __future__ import annotations
@@ -54,7 +54,7 @@ class Registration:
 
 
 def robust_background(image):
-    """star_detect.py:28-42: float32 median / MAD of the finite samples,
+    """star_detect.py:29-42: float32 median / MAD of the finite samples,
     noise = 1.4826 MAD, falling back to the float32 std, then 1."""
     finite = image[np.isfinite(image)]
     if finite.size == 0:
@@ -71,7 +71,7 @@ def robust_background(image):
 
 
 def measure_candidate(image, x, y, background, noise, radius, origin_x=0, origin_y=0):
-    """star_detect.py:45-102: FP64 flux, centroid and second moments of the
+    """star_detect.py:45-100: FP64 flux, centroid and second moments of the
     background-subtracted, zero-clipped window around (x, y)."""
     h, w = image.shape
     y0, y1 = max(0, y - radius), min(h, y + radius + 1)
@@ -109,7 +109,7 @@ def measure_candidate(image, x, y, background, noise, radius, origin_x=0, origin
 
 
 def dedupe_stars(stars, min_separation_px):
-    """star_detect.py:105-118: brightest first (stable), drop stars closer
+    """star_detect.py:103-118: brightest first (stable), drop stars closer
     than the separation to a kept one."""
     if min_separation_px <= 0.0 or len(stars) <= 1:
         return stars
@@ -160,7 +160,7 @@ def _translation(dx, dy):
 
 
 def apply_matrix(points, matrix):
-    """registration.py:49-54."""
+    """registration.py:46-51."""
     hom = np.hstack([points, np.ones((points.shape[0], 1), dtype=np.float64)])
     out = hom @ matrix.T
     wcol = out[:, 2:3]
@@ -168,13 +168,13 @@ def apply_matrix(points, matrix):
 
 
 def fit_translation(moving, reference):
-    """registration.py:96-98."""
+    """registration.py:101-103."""
     d = np.median(reference - moving, axis=0)
     return _translation(d[0], d[1])
 
 
 def fit_affine(moving, reference):
-    """registration.py:101-115 (FP64 least squares)."""
+    """registration.py:106-119 (FP64 least squares)."""
     if len(moving) < 3:
         raise ValueError("affine registration requires at least three matched stars")
     a = np.zeros((2 * len(moving), 6), dtype=np.float64)
@@ -188,7 +188,7 @@ def fit_affine(moving, reference):
 
 
 def fit_similarity(moving, reference):
-    """registration.py:136-158 (orthogonal Procrustes with scale)."""
+    """registration.py:140-162 (orthogonal Procrustes with scale)."""
     if len(moving) < 2:
         raise ValueError("similarity registration requires at least two matched stars")
     mm, rm = np.mean(moving, axis=0), np.mean(reference, axis=0)
@@ -237,7 +237,7 @@ def greedy_pairs(reference, transformed, tolerance_px):
 
 
 def translation_candidates(reference, moving, brightest):
-    """registration.py:219-235: one candidate per distinct rounded offset."""
+    """registration.py:218-233: one candidate per distinct rounded offset."""
     out, seen = [], set()
     for rp in reference[: min(len(reference), brightest)]:
         for mp in moving[: min(len(moving), brightest)]:
@@ -250,7 +250,7 @@ def translation_candidates(reference, moving, brightest):
 
 
 def triangle_descriptors(points, max_points, max_triangles):
-    """registration.py:316-343: (short/longest, middle/longest) of every
+    """registration.py:306-331: (short/longest, middle/longest) of every
     non-degenerate triangle of the first max_points, largest area first."""
     pts = points[: min(len(points), max_points)]
     tris = []
@@ -271,7 +271,7 @@ def triangle_descriptors(points, max_points, max_triangles):
 
 
 def asterism_candidates(reference, moving, model, tolerance, max_triangles):
-    """registration.py:346-371."""
+    """registration.py:334-358."""
     ref_t = triangle_descriptors(reference, 30, max_triangles)
     mov_t = triangle_descriptors(moving, 30, max_triangles)
     out = []
@@ -293,7 +293,7 @@ def asterism_candidates(reference, moving, model, tolerance, max_triangles):
 
 def estimate_star_transform(reference, moving, transform_model="translation", min_inliers=6, tolerance_px=3.0,
                             max_stars=80):
-    """registration.py:509-569 (translation / similarity / affine)."""
+    """registration.py:476-566 (translation / similarity / affine)."""
     ref, mov = _points(reference, max_stars), _points(moving, max_stars)
     minimum = 3 if transform_model == "affine" else 2 if transform_model == "similarity" else 1
     if len(ref) < minimum or len(mov) < minimum:

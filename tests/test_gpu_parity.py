@@ -343,6 +343,28 @@ def test_warp_resident_semantics_nan_taps_and_clamp(gpu):
     assert np.allclose(got[cov > 0], exp[cov > 0], atol=3e-5)
 
 
+@pytest.mark.parametrize("t", [0.0, 0.01])
+def test_warp_resident_clamp_step_edge_closed_form(gpu, t):
+    # A half-pixel shift of a 0/100 step rings ~11% past both plateaus: the
+    # clamp must put those pixels exactly on 100 + 100 t / -100 t (float32),
+    # and leave the others at the unclamped Lanczos3 value.
+    yy, xx = np.indices((16, 20), dtype=np.float32)
+    img = np.where(xx >= 10, 100.0, 0.0).astype(np.float32)
+    m = np.array([[1, 0, 0.5], [0, 1, 0.0], [0, 0, 1.0]])
+    with _stack_of(img[None]) as s:
+        s.apply_matrix_lanczos3_frame(0, m, np.nan, t)
+        got = s.download_frame(0)
+    raw, cov = oracle.warp_matrix_resident(img, m, np.nan, "lanczos3", -1.0)
+    ok = cov > 0
+    hi, lo = np.float32(100.0 + 100.0 * t), np.float32(-100.0 * t)
+    over, under = ok & (raw > hi + 1.0), ok & (raw < lo - 1.0)
+    assert over.sum() >= 10 and under.sum() >= 10
+    assert np.all(got[over] == hi) and np.all(got[under] == lo)
+    inside = ok & (raw < hi - 1.0) & (raw > lo + 1.0)
+    assert np.allclose(got[inside], raw[inside], rtol=0, atol=2e-4)
+    assert np.array_equal(np.isfinite(got), ok)
+
+
 def test_warp_reference_case_bilinear(gpu):
     # tests/test_gpu_warp_vs_cpu.py:154-173 on the resident class
     data = np.arange(49, dtype=np.float32).reshape(7, 7)

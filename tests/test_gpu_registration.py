@@ -216,3 +216,18 @@ def test_matrix_translation_candidates_errors(gpu):
             s.refine_matrix_translation_candidates_to_reference(0, 1, np.eye(2))
         with pytest.raises(ValueError, match="invertible"):
             s.refine_matrix_translation_candidates_to_reference(0, 1, np.zeros((3, 3)))
+
+
+@pytest.mark.parametrize("name", ["plain", "nan_pairs_plateau", "given_background_truncated"])
+def test_detect_stars_matches_reference_golden(gpu, name):
+    # inputs and outputs of the reference's own glass.cpu.star_detect.detect_stars
+    # (tests/golden/make_registration_golden.py): the GPU path must give the
+    # same list, field by field, bit for bit
+    from test_oracle_registration import GOLDEN, STAR_FIELDS, golden_detection_kwargs
+
+    print(f"golden detection case {name}", flush=True)
+    stars = R.detect_stars(GOLDEN[f"det/{name}/image"], **golden_detection_kwargs(name))
+    assert len(stars) == len(GOLDEN[f"det/{name}/x"])
+    for f in STAR_FIELDS:
+        got = np.asarray([np.nan if getattr(s, f) is None else getattr(s, f) for s in stars], np.float64)
+        assert np.array_equal(got, GOLDEN[f"det/{name}/{f}"].astype(np.float64), equal_nan=True), f

@@ -415,3 +415,55 @@ def test_translation_oracle_closed_forms():
     # output 16385 samples between 16384 (0) and 16385 (1): the value is the
     # float32 fraction, 0.69921875 there, not 0.7
     assert w32[0, 16385] == sx - np.float32(16384) == np.float32(0.69921875)
+
+
+# ------------------------------------------- resident Lanczos3 clamp branch
+# Closed forms of the clamp (tests/test_gpu_warp_vs_cpu.py:116-124): the value
+# is limited to [min - t*range, max + t*range] of the finite taps.  A step
+# edge shifted by half a pixel rings past both plateaus (about 11%), so the
+# overshooting pixels must land exactly on the bounds; values inside the
+# bounds, and every value of a smooth ramp, must be left unchanged.
+def _step_edge():
+    yy, xx = np.indices((16, 20), dtype=np.float32)
+    return np.where(xx >= 10, 100.0, 0.0).astype(np.float32), np.array([[1, 0, 0.5], [0, 1, 0.0], [0, 0, 1.0]])
+
+
+@pytest.mark.parametrize("t", [0.0, 0.01, 0.05])
+def test_resident_clamp_step_edge_closed_form(t):
+    img, m = _step_edge()
+    raw, cov = oracle.warp_matrix_resident(img, m, np.nan, "lanczos3", -1.0)
+    got, cov2 = oracle.warp_matrix_resident(img, m, np.nan, "lanczos3", t)
+    assert np.array_equal(cov, cov2)
+    ok = cov > 0
+    hi, lo = np.float32(100.0 + 100.0 * t), np.float32(-100.0 * t)
+    over, under = ok & (raw > hi), ok & (raw < lo)
+    assert over.sum() >= 10 and under.sum() >= 10           # the edge rings past both bounds
+    assert np.all(got[over] == hi) and np.all(got[under] == lo)
+    inside = ok & ~over & ~under
+    assert np.array_equal(got[inside], raw[inside])
+    assert np.nanmax(got) == hi and np.nanmin(got) == lo
+
+
+def test_resident_clamp_leaves_smooth_ramp_and_loose_bounds_unchanged():
+    yy, xx = np.indices((14, 17), dtype=np.float32)
+    ramp = (3.0 * xx + 0.5 * yy).astype(np.float32)
+    m = np.array([[np.cos(0.01), -np.sin(0.01), 0.37], [np.sin(0.01), np.cos(0.01), -0.21], [0, 0, 1.0]])
+    raw, cov = oracle.warp_matrix_resident(ramp, m, np.nan, "lanczos3", -1.0)
+    for t in (0.0, 0.2):
+        got, _ = oracle.warp_matrix_resident(ramp, m, np.nan, "lanczos3", t)
+        assert np.array_equal(got, raw, equal_nan=True)
+    img, m = _step_edge()
+    raw, _ = oracle.warp_matrix_resident(img, m, np.nan, "lanczos3", -1.0)
+    got, _ = oracle.warp_matrix_resident(img, m, np.nan, "lanczos3", 1.0e6)
+    assert np.array_equal(got, raw, equal_nan=True)
+
+
+def test_resident_clamp_bounds_ignore_nonfinite_taps():
+    img, m = _step_edge()
+    img = img.copy()
+    img[5:11, 3] = np.nan     # NaN taps inside the low plateau's windows
+    img[5:11, 15] = -np.inf   # and -inf inside the high plateau's
+    got, cov = oracle.warp_matrix_resident(img, m, np.nan, "lanczos3", 0.0)
+    ok = cov > 0
+    assert np.all(got[ok] >= 0.0) and np.all(got[ok] <= 100.0)
+    assert np.all(np.isfinite(got[ok]))

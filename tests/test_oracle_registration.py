@@ -183,3 +183,81 @@ def test_matrix_candidate_refinement_closed_forms():
     # the sums: every valid pixel pairs reference(x, y) with itself
     s = R.alignment_sums(ref, mov, R.invert_matrix_f32(best["matrix"]), 1)
     assert s[0] == s[1] and s[5] == 0.0 and s[7] == (40 - 1) * (52 - 1)
+
+
+# ---------------------------- golden vectors of the reference's own functions
+# tests/golden/registration_golden.npz holds inputs and outputs of the
+# reference's glass.cpu.star_detect.detect_stars and
+# glass.cpu.registration.estimate_star_transform /
+# estimate_triangle_asterism_transform (written by
+# tests/golden/make_registration_golden.py).
+from pathlib import Path  # noqa: E402
+
+GOLDEN = np.load(Path(__file__).resolve().parent / "golden" / "registration_golden.npz", allow_pickle=False)
+STAR_FIELDS = ("x", "y", "flux", "peak", "fwhm_px", "eccentricity", "snr", "sharpness", "npix")
+
+
+def golden_detection_kwargs(name):
+    p = lambda k: float(GOLDEN[f"det/{name}/{k}"])  # noqa: E731
+    kw = {"threshold_sigma": p("threshold_sigma"), "max_stars": int(p("max_stars"))}
+    for k in ("background", "noise", "min_separation_px"):
+        if math.isfinite(p(k)):
+            kw[k] = p(k)
+    for k in ("window_radius", "origin_x", "origin_y"):
+        if math.isfinite(p(k)):
+            kw[k] = int(p(k))
+    return kw
+
+
+def golden_star_lists(name):
+    ref = GOLDEN[f"reg/{name}/reference_xy"]
+    mov = GOLDEN[f"reg/{name}/moving_xy"]
+    rs = [O.Star(float(x), float(y), float(f)) for (x, y), f in zip(ref, np.linspace(1000, 10, len(ref)))]
+    ms = [O.Star(float(x), float(y), float(f)) for (x, y), f in zip(mov, np.linspace(900, 9, len(mov)))]
+    return rs, ms
+
+
+def star_arrays(stars):
+    return {f: np.asarray([np.nan if getattr(s, f) is None else getattr(s, f) for s in stars], np.float64)
+            for f in STAR_FIELDS}
+
+
+@pytest.mark.parametrize("name", [str(n) for n in GOLDEN["det_names"]])
+def test_oracle_detection_matches_reference_golden(name):
+    stars = O.detect_stars(GOLDEN[f"det/{name}/image"], **golden_detection_kwargs(name))
+    got = star_arrays(stars)
+    assert len(stars) == len(GOLDEN[f"det/{name}/x"]) > 0
+    for f in STAR_FIELDS:
+        assert np.array_equal(got[f], GOLDEN[f"det/{name}/{f}"].astype(np.float64), equal_nan=True), f
+
+
+@pytest.mark.parametrize("model", ["translation", "similarity", "affine"])
+@pytest.mark.parametrize("name", [str(n) for n in GOLDEN["reg_names"]])
+def test_oracle_transform_matches_reference_golden(name, model):
+    rs, ms = golden_star_lists(name)
+    key = f"reg/{name}/star_{model}"
+    r = O.estimate_star_transform(rs, ms, model)
+    assert r.status == str(GOLDEN[f"{key}/status"])
+    assert np.array_equal(np.asarray(r.pairs, np.int64).reshape(-1, 2), GOLDEN[f"{key}/pairs"])
+    # bit-exact, lstsq / svd included (same numpy build as the reference run)
+    assert np.array_equal(np.asarray(r.matrix), GOLDEN[f"{key}/matrix"])
+    assert r.rms_px == float(GOLDEN[f"{key}/rms_px"]) or (math.isnan(r.rms_px) and
+                                                          math.isnan(float(GOLDEN[f"{key}/rms_px"])))
+
+
+@pytest.mark.parametrize("model", ["translation", "similarity", "affine"])
+@pytest.mark.parametrize("name", [str(n) for n in GOLDEN["reg_names"]])
+def test_library_transform_matches_reference_golden(name, model):
+    from glass_amd import registration as R
+
+    rs, ms = golden_star_lists(name)
+    key = f"reg/{name}/star_{model}"
+    g = R.estimate_star_transform(rs, ms, model)
+    assert g.status == str(GOLDEN[f"{key}/status"])
+    assert np.array_equal(np.asarray(g.pairs, np.int64).reshape(-1, 2), GOLDEN[f"{key}/pairs"])
+    # the library solves the fits with its own FP64 QR / closed forms: the
+    # matrices agree with LAPACK's to 1e-12 absolute (pixel units; 1e-8 px
+    # over a 10^4 px frame), the translation model exactly
+    tol = 0.0 if model == "translation" else 1e-12
+    assert np.max(np.abs(np.asarray(g.matrix) - GOLDEN[f"{key}/matrix"])) <= tol
+    assert g.rms_px == pytest.approx(float(GOLDEN[f"{key}/rms_px"]), rel=1e-12, abs=1e-12)
