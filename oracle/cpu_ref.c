@@ -105,6 +105,39 @@ void cpu_warp(const float* src, int64_t h, int64_t w, const double* inv, int int
   }
 }
 
+/* CPU-path Lanczos3 through an affine source map b (reference pixel ->
+ * source pixel, s = (b0 x + b1 y) + b2, (b3 x + b4 y) + b5), every tap
+ * summed (engine/warp.py:121-152 on finite sources), FP64 output: the finite
+ * difference reference of the warp adjoint.  `fill` outside 2 <= s < dim-3. */
+void cpu_warp_map64(const float* src, int64_t h, int64_t w, const double* b, double fill, double* out) {
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t y = 0; y < h; ++y) {
+    for (int64_t x = 0; x < w; ++x) {
+      double sx = b[0] * (double)x + b[1] * (double)y;
+      sx = sx + b[2];
+      double sy = b[3] * (double)x + b[4] * (double)y;
+      sy = sy + b[5];
+      if (!(sx >= 2.0 && sx < (double)(w - 3) && sy >= 2.0 && sy < (double)(h - 3))) {
+        out[y * w + x] = fill;
+        continue;
+      }
+      const int64_t bx = (int64_t)floor(sx), by = (int64_t)floor(sy);
+      double wxs[6];
+      for (int k = 0; k < 6; ++k) wxs[k] = lanczos3(sx - (double)(bx - 2 + k));
+      double acc = 0.0, ws = 0.0;
+      for (int j = 0; j < 6; ++j) {
+        const double wy = lanczos3(sy - (double)(by - 2 + j));
+        for (int k = 0; k < 6; ++k) {
+          const double wt = wy * wxs[k];
+          acc += (double)src[(by - 2 + j) * w + (bx - 2 + k)] * wt;
+          ws += wt;
+        }
+      }
+      out[y * w + x] = acc / ws;
+    }
+  }
+}
+
 static int cmp_float(const void* a, const void* b) {
   const float x = *(const float*)a, y = *(const float*)b;
   return (x > y) - (x < y);

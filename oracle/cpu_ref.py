@@ -43,6 +43,8 @@ def load():
                                             ctypes.c_float, ctypes.c_float, ctypes.c_void_p]
         lib.cpu_warp.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int,
                                  ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p]
+        lib.cpu_warp_map64.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
+                                       ctypes.c_double, ctypes.c_void_p]
         lib.cpu_integrate_winsorized.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
                                                  ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_float,
                                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -72,6 +74,15 @@ def warp(src, inverse, interpolation="lanczos3", fill=np.nan):
     load().cpu_warp(_p(src), src.shape[0], src.shape[1], _p(inv), 3 if interpolation == "lanczos3" else 1,
                     float(fill), _p(out), _p(cov))
     return out, cov
+
+
+def warp_map64(src, source_map, fill=0.0):
+    """FP64 CPU-path Lanczos3 through an affine source map (2x3)."""
+    src = np.ascontiguousarray(src, dtype=np.float32)
+    b = np.ascontiguousarray(np.asarray(source_map, dtype=np.float64).reshape(6))
+    out = np.empty(src.shape, dtype=np.float64)
+    load().cpu_warp_map64(_p(src), src.shape[0], src.shape[1], _p(b), float(fill), _p(out))
+    return out
 
 
 def integrate_winsorized(stack, weights, low_sigma=3.0, high_sigma=3.0, min_samples=3, max_frac=0.5):

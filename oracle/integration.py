@@ -60,10 +60,12 @@ def weighted_integrate_stack(
     high_sigma=3.0,
     min_samples=3,
     max_reject_fraction=0.5,
+    all_valid=None,
 ):
     """src/glass/cpu/integration.py:36-118.
 
     Returns (master, weight_map, coverage_map, low_reject, high_reject), all f32.
+    ``all_valid``: see ``center_and_scale`` (None: decided from this stack).
     """
     frames = np.asarray(stack, dtype=np.float32)
     if frames.ndim != 3:
@@ -91,7 +93,7 @@ def weighted_integrate_stack(
     high_map = np.zeros((height, width), dtype=np.float32)
     if rejection in {"sigma_clip", "winsorized_sigma"}:
         center, scale = center_and_scale(
-            frames, valid, method=rejection, low_sigma=low_sigma, high_sigma=high_sigma
+            frames, valid, method=rejection, low_sigma=low_sigma, high_sigma=high_sigma, all_valid=all_valid
         )
         lo_t = center - np.float32(low_sigma) * scale
         hi_t = center + np.float32(high_sigma) * scale

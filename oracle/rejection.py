@@ -117,9 +117,15 @@ def _winsorized_valid(stack, valid, low_sigma, high_sigma):
     )
 
 
-def center_and_scale(stack, valid, *, method="winsorized_sigma", low_sigma=3.0, high_sigma=3.0):
-    """rejection.py:31-65 (winsorized_sigma and sigma_clip methods)."""
-    if method == "winsorized_sigma" and _all_valid_finite(stack, valid):
+def center_and_scale(stack, valid, *, method="winsorized_sigma", low_sigma=3.0, high_sigma=3.0, all_valid=None):
+    """rejection.py:31-65 (winsorized_sigma and sigma_clip methods).
+
+    ``all_valid`` None decides the arithmetic from this array, as the
+    reference does (rejection.py:39-44); True / False impose the decision
+    taken over a whole frame when this array is a block of it."""
+    if all_valid is None:
+        all_valid = _all_valid_finite(stack, valid)
+    if method == "winsorized_sigma" and all_valid:
         return _winsorized_all_valid(stack, low_sigma, high_sigma)
     masked = np.where(valid, stack, np.nan)
     if method in {"sigma", "sigma_clip"}:

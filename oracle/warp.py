@@ -34,10 +34,11 @@ def invert_matrix(matrix):
         raise ValueError("warp matrix is singular") from exc
 
 
-def source_coordinates(inverse, height, width):
-    """engine/warp.py:177-195: FP64 source coordinates, NaN where the
-    projective denominator is <= 1e-12 in magnitude."""
-    ys, xs = np.mgrid[0:height, 0:width].astype(np.float64)
+def source_coordinates(inverse, height, width, x0=0, y0=0):
+    """engine/warp.py:177-195: FP64 source coordinates of output pixels
+    [y0, y0 + height) x [x0, x0 + width), NaN where the projective
+    denominator is <= 1e-12 in magnitude."""
+    ys, xs = np.mgrid[y0:y0 + height, x0:x0 + width].astype(np.float64)
     inv = np.asarray(inverse, dtype=np.float64)
     denom = inv[2, 0] * xs + inv[2, 1] * ys + inv[2, 2]
     ok = np.abs(denom) > 1.0e-12
@@ -61,8 +62,11 @@ def lanczos3_weight(x, threshold=1.0e-8):
     return np.where(ax >= 3.0, 0.0, w)
 
 
-def _bilinear(image, sx, sy):
-    h, w = image.shape
+def _bilinear(image, sx, sy, origin=(0, 0), full_shape=None):
+    """`image` holds the source rows/columns from `origin` (x, y) of a frame
+    of `full_shape` (the whole frame by default)."""
+    h, w = full_shape if full_shape is not None else image.shape
+    ox, oy = origin
     x0 = np.floor(sx).astype(np.int64)
     y0 = np.floor(sy).astype(np.int64)
     x1 = np.minimum(x0 + 1, w - 1)
@@ -70,13 +74,16 @@ def _bilinear(image, sx, sy):
     tx = (sx - x0).astype(np.float32)
     ty = (sy - y0).astype(np.float32)
     one = np.float32(1.0)
+    x0, x1, y0, y1 = x0 - ox, x1 - ox, y0 - oy, y1 - oy
     top = image[y0, x0] * (one - tx) + image[y0, x1] * tx
     bottom = image[y1, x0] * (one - tx) + image[y1, x1] * tx
     return (top * (one - ty) + bottom * ty).astype(np.float32)
 
 
-def _lanczos_cpu(image, sx, sy):
-    """engine/warp.py:121-152 vectorised: yy outer, xx inner, FP64 sums."""
+def _lanczos_cpu(image, sx, sy, origin=(0, 0)):
+    """engine/warp.py:121-152 vectorised: yy outer, xx inner, FP64 sums.
+    `image` holds the source from `origin` (x, y) on."""
+    ox, oy = origin
     bx = np.floor(sx).astype(np.int64)
     by = np.floor(sy).astype(np.int64)
     acc = np.zeros(sx.shape, dtype=np.float64)
@@ -89,7 +96,7 @@ def _lanczos_cpu(image, sx, sy):
             wx = lanczos3_weight(sx - xx.astype(np.float64), 1.0e-8)
             take = (wy != 0.0) & (wx != 0.0)
             weight = wx * wy
-            value = image[yy, xx].astype(np.float64)
+            value = image[yy - oy, xx - ox].astype(np.float64)
             acc = np.where(take, acc + value * weight, acc)
             wsum = np.where(take, wsum + weight, wsum)
     out = np.zeros(sx.shape, dtype=np.float32)
@@ -155,6 +162,59 @@ def warp_matrix(image, matrix, fill=0.0, interpolation="bilinear"):
     out[valid] = sampled
     cov[valid] = 1.0
     return out, cov
+
+
+def warp_matrix_window(image, matrix, window, origin=(0, 0), full_shape=None, fill=0.0, interpolation="lanczos3"):
+    """``warp_matrix`` restricted to the output window (x0, y0, width,
+    height) of a frame of `full_shape` (H, W): the same per-pixel arithmetic
+    (coordinates from the absolute pixel indices, footprint against the full
+    frame), with `image` holding the source from `origin` (x, y) on, a crop
+    that must contain every tap of the window's in-footprint pixels.
+    Returns (warped f32, coverage f32) of the window."""
+    img = np.asarray(image, dtype=np.float32)
+    h, w = full_shape if full_shape is not None else img.shape
+    x0, y0, ww, wh = (int(v) for v in window)
+    out = np.full((wh, ww), fill, dtype=np.float32)
+    cov = np.zeros((wh, ww), dtype=np.float32)
+    sx, sy = source_coordinates(invert_matrix(matrix), wh, ww, x0, y0)
+    valid = np.isfinite(sx) & np.isfinite(sy)
+    if interpolation == "bilinear":
+        valid &= (sx >= 0.0) & (sx <= float(w - 1)) & (sy >= 0.0) & (sy <= float(h - 1))
+        reach = (0, 1)
+    elif interpolation == "lanczos3":
+        valid &= (sx >= 2.0) & (sx < float(w - 3)) & (sy >= 2.0) & (sy < float(h - 3))
+        reach = (-2, 3)
+    else:
+        raise ValueError(f"unsupported warp interpolation: {interpolation}")
+    if not np.any(valid):
+        return out, cov
+    vx, vy = sx[valid], sy[valid]
+    ox, oy = origin
+    ix, iy = np.floor(vx).astype(np.int64), np.floor(vy).astype(np.int64)
+    if (ix.min() + reach[0] < ox or iy.min() + reach[0] < oy or ix.max() + reach[1] >= ox + img.shape[1]
+            or iy.max() + reach[1] >= oy + img.shape[0]):
+        raise ValueError("source crop does not hold every tap of the window")
+    if interpolation == "bilinear":
+        out[valid] = _bilinear(img, vx, vy, origin, (h, w))
+    else:
+        out[valid] = _lanczos_cpu(img, vx, vy, origin)
+    cov[valid] = 1.0
+    return out, cov
+
+
+def source_box(matrix, window, full_shape, interpolation="lanczos3"):
+    """Source rows/columns (x0, y0, x1, y1), clipped to the frame, that hold
+    every tap of the output window of an affine matrix (corners in FP64)."""
+    inv = invert_matrix(matrix)
+    x0, y0, ww, wh = window
+    h, w = full_shape
+    xs = np.array([x0, x0 + ww - 1, x0, x0 + ww - 1], np.float64)
+    ys = np.array([y0, y0, y0 + wh - 1, y0 + wh - 1], np.float64)
+    sx = (inv[0, 0] * xs + inv[0, 1] * ys) + inv[0, 2]
+    sy = (inv[1, 0] * xs + inv[1, 1] * ys) + inv[1, 2]
+    pad = 4 if interpolation == "lanczos3" else 2
+    return (max(int(np.floor(sx.min())) - pad, 0), max(int(np.floor(sy.min())) - pad, 0),
+            min(int(np.floor(sx.max())) + pad + 1, w), min(int(np.floor(sy.max())) + pad + 1, h))
 
 
 def warp_matrix_resident(image, matrix, fill=np.nan, interpolation="lanczos3", clamping_threshold=-1.0):

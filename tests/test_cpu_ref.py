@@ -67,3 +67,15 @@ def test_integrate_matches_oracle(ref, frames):
             assert np.allclose(g, e, rtol=2e-7, atol=0), name
         else:
             assert np.array_equal(g, e), name
+
+
+def test_warp_map64_matches_oracle(ref):
+    from oracle.warp import warp_lanczos3_source_map
+
+    rng = np.random.default_rng(14)
+    img = rng.normal(100, 10, (29, 35)).astype(np.float32)
+    b = np.array([0.9998, -0.017, 1.3, 0.017, 0.9998, -0.8])
+    got = ref.warp_map64(img, b, 0.0)
+    exp = warp_lanczos3_source_map(img, b, 0.0, np.float64)
+    assert np.array_equal(got == 0.0, exp == 0.0)
+    assert np.allclose(got, exp, rtol=1e-14, atol=0)

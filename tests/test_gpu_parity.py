@@ -410,6 +410,25 @@ def test_winsorized_unit_weights_matches_oracle(gpu, frames):
     _compare(got, exp)
 
 
+@pytest.mark.parametrize("frames,weighted", [(24, False), (200, False), (256, True), (300, False), (300, True)])
+def test_winsorized_sums_straddling_zero_and_wide_magnitudes(gpu, frames, weighted):
+    # Samples of both signs spanning ~2^40 in magnitude, with NaNs: FP64 sums
+    # of such samples are not exact, so the kernel must add them in the
+    # oracle's frame order for means, thresholds and counts to match.
+    rng = np.random.default_rng(700 + frames + weighted)
+    h, w = 7, 29
+    mag = 10.0 ** rng.uniform(-6, 6, size=(frames, h, w))
+    stack = (np.where(rng.random((frames, h, w)) < 0.5, -1.0, 1.0) * mag).astype(np.float32)
+    stack[:, :, :10] = rng.normal(0.0, 1e-3, (frames, h, 10)).astype(np.float32) + np.float32(1e-4)
+    stack[rng.random(stack.shape) < 0.02] = np.nan
+    weights = rng.uniform(0.5, 2.0, frames).astype(np.float32) if weighted else None
+    print(f"winsorized wide-magnitude sums frames={frames} weighted={weighted}", flush=True)
+    with _stack_of(stack) as s:
+        got = s.integrate_hardened_winsorized_sigma(weights, 3.0, 3.0, 3, 0.5)
+    exp = oracle.weighted_integrate_stack(stack, weights=weights, rejection="winsorized_sigma")
+    _compare(got, exp)
+
+
 @pytest.mark.parametrize("frames", [4, 20, 40, 120])
 def test_winsorized_weights_and_zero_weight_frames(gpu, frames):
     rng = np.random.default_rng(100 + frames)
