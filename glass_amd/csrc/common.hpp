@@ -63,10 +63,15 @@ __device__ __forceinline__ T load_uniform(const T* p) {
   return out;
 }
 
-// Entry k of a device table of pointers, as a wave-uniform pointer.
+// Entry k of a device table of pointers, as a wave-uniform pointer.  The
+// table is read through the constant address space, so a wave-uniform k
+// becomes a scalar load into SGPRs (a global load would fetch the same 8
+// bytes into every lane and then need two v_readfirstlane per entry).
+typedef __attribute__((address_space(4))) const unsigned long long const_u64;
+typedef __attribute__((address_space(4))) const float const_f32;  // wave-uniform reads: scalar loads
 template <typename T>
 __device__ __forceinline__ T* table_at(T* const* table, long long k) {
-  const unsigned long long v = ((const global_u64*)table)[k];
+  const unsigned long long v = ((const_u64*)table)[k];
   const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(v));
   const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(v >> 32));
   return reinterpret_cast<T*>((static_cast<unsigned long long>(hi) << 32) | lo);

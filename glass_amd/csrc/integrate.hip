@@ -138,7 +138,10 @@ struct Samples {
   __device__ __forceinline__ void set(float value) {
     x[I] = value;
   }
-  // Element `idx` for a wave-uniform idx.
+  // Element `idx` for a wave-uniform idx: scalar branches to the chunk and
+  // to the element, then one move.  The empty volatile asm in each branch
+  // body keeps the branches (a side effect cannot be speculated), or the
+  // optimiser if-converts all of them into CAP per-lane selects.
   __device__ __forceinline__ float at_uniform(int idx) const {
     const int chunk = idx >> 4;
     const int lane = idx & 15;
@@ -146,12 +149,16 @@ struct Samples {
     static_for<kChunks>([&](auto C) GH_ALWAYS_INLINE {
       constexpr int ci = decltype(C)::value;
       if (chunk == ci) {
-        float y = 0.0f;
+        asm volatile("");
         static_for<16>([&](auto J) GH_ALWAYS_INLINE {
           constexpr int j = decltype(J)::value;
-          if constexpr (ci * 16 + j < CAP) y = (lane == j) ? x[ci * 16 + j] : y;
+          if constexpr (ci * 16 + j < CAP) {
+            if (lane == j) {
+              asm volatile("");
+              r = x[ci * 16 + j];
+            }
+          }
         });
-        r = y;
       }
     });
     return r;
@@ -181,6 +188,50 @@ __device__ __forceinline__ void sort_network(Samples<CAP>& v, std::integer_seque
 template <int CAP>
 __device__ __forceinline__ void sort_registers(Samples<CAP>& v) {
   sort_network<CAP>(v, std::make_integer_sequence<int, Network<CAP>::kSize>{});
+}
+
+// ------------------------------------------------------ scans of the ends
+// The keys are sorted, so the samples beyond a per-lane bound sit at one end
+// of the array.  These walk the slots from the bottom (or the top) in groups
+// of kScanGroup and stop after the first group in which no lane of the wave
+// meets `hit`: the cost follows the wave's largest count of such samples,
+// not the capacity.  The group test is a wave-uniform branch; the empty
+// volatile asm keeps it one (a side effect cannot be speculated, so the chain
+// is not if-converted into selects over every slot).
+constexpr int kScanGroup = 8;
+
+template <int CAP, bool FROM_TOP, typename Hit, typename Body>
+__device__ __forceinline__ void scan_ends(const Samples<CAP>& v, Hit&& hit, Body&& body) {
+  constexpr int kGroups = (CAP + kScanGroup - 1) / kScanGroup;
+  bool go = true;
+  static_for<kGroups>([&](auto G) GH_ALWAYS_INLINE {
+    constexpr int g = decltype(G)::value;
+    if (go) {
+      asm volatile("");
+      bool any = false;
+      static_for<kScanGroup>([&](auto J) GH_ALWAYS_INLINE {
+        constexpr int j = g * kScanGroup + decltype(J)::value;
+        if constexpr (j < CAP) {
+          constexpr int k = FROM_TOP ? CAP - 1 - j : j;
+          const float x = v.template get<k>();
+          const bool h = hit(x);
+          any |= h;
+          body(x, h);
+        }
+      });
+      if (__ballot(any) == 0ull) go = false;
+    }
+  });
+}
+
+template <int CAP, typename Hit, typename Body>
+__device__ __forceinline__ void scan_from_bottom(const Samples<CAP>& v, Hit&& hit, Body&& body) {
+  scan_ends<CAP, false>(v, hit, body);
+}
+
+template <int CAP, typename Hit, typename Body>
+__device__ __forceinline__ void scan_from_top(const Samples<CAP>& v, Hit&& hit, Body&& body) {
+  scan_ends<CAP, true>(v, hit, body);
 }
 
 // ------------------------------------------------------- quartile extraction
@@ -301,6 +352,7 @@ __device__ __forceinline__ void winsorized_pixel(const IntegrateArgs& a, long lo
   Samples<CAP> v;
   int cnt = 0;
   double swx = 0.0, sw = 0.0;
+  double su = 0.0;  // unweighted sum of the valid samples (weighted stacks)
   // Load pass: every capacity slot issues its load before any use, so all
   // loads of the pixel are in flight together.  The frame table is padded to
   // the capacity with the last active frame (re-reads hit L2), which keeps the
@@ -325,17 +377,22 @@ __device__ __forceinline__ void winsorized_pixel(const IntegrateArgs& a, long lo
     if constexpr (UNIT) {
       swx += static_cast<double>(ok ? x : 0.0f);
     } else {
-      const double w = static_cast<double>(ld_global(a.frame_w, src));
-      swx += ok ? static_cast<double>(x) * w : 0.0;
+      const double w = static_cast<double>(((const_f32*)a.frame_w)[src]);
+      const double xd = static_cast<double>(ok ? x : 0.0f);
+      swx += ok ? xd * w : 0.0;
       sw += ok ? w : 0.0;
+      su += xd;
     }
     v.template set<k>(ok ? x : INFINITY);
   });
-  if constexpr (UNIT) sw = static_cast<double>(cnt);
   // Force the load-pass reductions to complete here.  Otherwise the compiler
   // defers them (they are only needed at the end) and keeps every raw sample
   // alive next to its sort key: twice the registers.
-  asm volatile("" ::"v"(swx), "v"(sw), "v"(cnt));
+  if constexpr (UNIT) {
+    asm volatile("" ::"v"(swx), "v"(cnt));
+  } else {
+    asm volatile("" ::"v"(swx), "v"(sw), "v"(su), "v"(cnt));
+  }
 
   // No branch between the load pass and the network: a branch here lets the
   // compiler sink the key selects past it and keep raw values and masks alive
@@ -386,20 +443,33 @@ __device__ __forceinline__ void winsorized_pixel(const IntegrateArgs& a, long lo
   const float lo_clip = center0 - a.low_sigma * scale0;
   const float hi_clip = center0 + a.high_sigma * scale0;
 
-  // Winsorized mean and population variance in FP64.  np.clip(x, lo, hi) is
-  // the median of (x, lo, hi) because lo <= hi.  The passes run over all CAP
-  // slots without masking: slots k >= cnt hold +inf (sorted last), clip to
-  // hi_clip, and their known contribution is removed once at the end.  The
-  // FP64 sums of clipped f32 samples are exact (and order-free) whenever the
-  // samples' exponents span less than the 29 spare bits of FP64, which the
-  // clip bounds guarantee in practice; the sorted-order sums rely on the same.
-  const double pad = static_cast<double>(CAP - cnt);
-  double ws = 0.0;
-  static_for<CAP>([&](auto K) GH_ALWAYS_INLINE {
-    constexpr int k = decltype(K)::value;
-    ws += static_cast<double>(__builtin_amdgcn_fmed3f(v.template get<k>(), lo_clip, hi_clip));
+  // Winsorized mean and population variance in FP64.  np.clip(x, lo, hi)
+  // changes only the samples below lo_clip or above hi_clip, which the sorted
+  // keys hold at their ends: the winsorized sum is the sum of the valid
+  // samples (load pass, frame order) with those few replaced by the bound.
+  // The FP64 sums of f32 samples are exact, so equal to the oracle's frame
+  // order, whenever the samples' exponents span less than 53 - 24 - log2(n)
+  // bits (about 21 at 193 frames); beyond that they can differ in the last
+  // FP64 bits, which moves a float32 result only at a rounding boundary.
+  double sum_lo = 0.0, sum_hi = 0.0;
+  int c_lo = 0, c_hi = 0;
+  scan_from_bottom<CAP>(v, [&](float x) { return x < lo_clip; }, [&](float x, bool h) {
+    c_lo += h ? 1 : 0;
+    sum_lo += static_cast<double>(h ? x : 0.0f);
   });
-  ws -= pad * static_cast<double>(hi_clip);
+  // from the top: the +inf keys (slots past the valid count) come first
+  scan_from_top<CAP>(v, [&](float x) { return x > hi_clip; }, [&](float x, bool h) {
+    const bool real = h && x != INFINITY;
+    c_hi += real ? 1 : 0;
+    sum_hi += static_cast<double>(real ? x : 0.0f);
+  });
+  const double pad = static_cast<double>(CAP - cnt);
+  if constexpr (UNIT) {
+    sw = static_cast<double>(cnt);
+    su = swx;
+  }
+  double ws = ((su - sum_lo) - sum_hi) + static_cast<double>(c_lo) * static_cast<double>(lo_clip) +
+              static_cast<double>(c_hi) * static_cast<double>(hi_clip);
   double wmean = ws / n;
   float center = static_cast<float>(wmean);
   if (!all_valid) wmean = static_cast<double>(center);
@@ -429,21 +499,21 @@ __device__ __forceinline__ void winsorized_pixel(const IntegrateArgs& a, long lo
   const float hi_t = center + a.high_sigma * scale;
 
   int n_low = 0, n_high = 0;
+  double rej_sum = 0.0;  // rejected samples (unit weights: survivors = all - rejected)
   const bool usable = !a.require_positive_scale || (isfinite(scale) && scale > 0.0f);
-  // Sorted keys: the smallest valid sample is slot 0, the largest rank cnt-1.
-  const float mn = v.template get<0>(), mx = q.mx;
-  if (usable && !(mn >= lo_t && mx <= hi_t)) {
-    // Branch-free: invalid slots hold +inf, which is never below lo_t and
-    // always above hi_c (hi_t capped at FLT_MAX, which leaves every finite
-    // comparison unchanged), so exactly CAP - cnt high counts are removed.
-    const float hi_c = fminf(hi_t, 3.402823466e38f);
-    static_for<CAP>([&](auto K) GH_ALWAYS_INLINE {
-      constexpr int k = decltype(K)::value;
-      const float x = v.template get<k>();
-      n_low += (x < lo_t) ? 1 : 0;
-      n_high += (x > hi_c) ? 1 : 0;
-    });
-    n_high -= CAP - cnt;
+  // hi_t capped at FLT_MAX: the +inf keys past the valid count stay above it
+  // (the scan walks through them) without being counted.
+  const float hi_c = fminf(hi_t, 3.402823466e38f);
+  scan_from_bottom<CAP>(v, [&](float x) { return usable && x < lo_t; }, [&](float x, bool h) {
+    n_low += h ? 1 : 0;
+    if constexpr (UNIT) rej_sum += static_cast<double>(h ? x : 0.0f);
+  });
+  scan_from_top<CAP>(v, [&](float x) { return usable && x > hi_c; }, [&](float x, bool h) {
+    const bool real = h && x != INFINITY;
+    n_high += real ? 1 : 0;
+    if constexpr (UNIT) rej_sum += static_cast<double>(real ? x : 0.0f);
+  });
+  {
     const int rejected = n_low + n_high;
     if (rejected > 0) {
       const int remaining = cnt - rejected;
@@ -461,19 +531,10 @@ __device__ __forceinline__ void winsorized_pixel(const IntegrateArgs& a, long lo
     weight = static_cast<float>(sw);
     coverage = static_cast<float>(cnt);
   } else if constexpr (UNIT) {
-    // Opaque threshold copies keep the compares of the counting pass from
-    // being kept alive (CAP lane masks) for this pass.
-    float lo_t2 = lo_t, hi_t2 = hi_t;
-    asm volatile("" : "+v"(lo_t2), "+v"(hi_t2));
-    hi_t2 = fminf(hi_t2, 3.402823466e38f);  // excludes the +inf slots k >= cnt
-    double s = 0.0;
-    static_for<CAP>([&](auto K) GH_ALWAYS_INLINE {
-      constexpr int k = decltype(K)::value;
-      const float x = v.template get<k>();
-      s += static_cast<double>((x >= lo_t2 && x <= hi_t2) ? x : 0.0f);
-    });
+    // Survivors: every valid sample minus the rejected ones (exact whenever
+    // the sums are, see the winsorized sum above).
     const int kept = cnt - n_low - n_high;
-    master = static_cast<float>(s / static_cast<double>(kept));
+    master = static_cast<float>((swx - rej_sum) / static_cast<double>(kept));
     weight = static_cast<float>(kept);
     coverage = static_cast<float>(kept);
   } else {

@@ -1150,9 +1150,15 @@ class ResidentCalibratedStack:
         the coarse candidates of all frames share one launch, as do the fine
         ones."""
         moving = [int(i) for i in list(moving_indices)]
+        # parse_matrix_stack (native_bindings.cpp:1365-1388): one (3, 3) matrix
+        # is a stack of one; an empty stack is an error even without frames
         mats = np.asarray(matrices, dtype=np.float32)
+        if mats.shape == (3, 3):
+            mats = mats.reshape(1, 3, 3)
         if mats.ndim != 3 or mats.shape[1:] != (3, 3):
             raise ValueError("matrices must have shape (N, 3, 3) or (3, 3)")
+        if mats.shape[0] <= 0:
+            raise ValueError("matrices must contain at least one matrix")
         if moving and mats.shape[0] != len(moving):
             raise ValueError("batch refine requires one seed matrix per moving frame")
         prm = N.AlignSearchParams(float(search_radius_px), float(coarse_step_px), float(fine_radius_px),

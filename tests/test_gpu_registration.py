@@ -187,7 +187,13 @@ def test_matrix_translation_candidates_match_the_oracle(gpu):
     with _stack_of(ref, mov, mov) as s3:
         batch = s3.refine_matrix_translation_candidates_batch_to_reference(0, [1, 2], seeds, 4.0, 0.5, 0.5, 0.125,
                                                                            3, 1)
-        assert s3.refine_matrix_translation_candidates_batch_to_reference(0, [], seeds[:0]) == []
+        # no frames: [] when a seed is given; an empty seed stack is an error
+        # (parse_matrix_stack, native_bindings.cpp:1377-1379), one (3, 3) is a stack of one
+        assert s3.refine_matrix_translation_candidates_batch_to_reference(0, [], seeds[:1]) == []
+        with pytest.raises(ValueError, match="at least one matrix"):
+            s3.refine_matrix_translation_candidates_batch_to_reference(0, [], seeds[:0])
+        one = s3.refine_matrix_translation_candidates_batch_to_reference(0, [1], seeds[0])
+        assert len(one) == 1 and one[0]["moving_index"] == 1
         with pytest.raises(ValueError, match="one seed matrix per moving frame"):
             s3.refine_matrix_translation_candidates_batch_to_reference(0, [1], seeds)
     assert [b["moving_index"] for b in batch] == [1, 2]
