@@ -198,7 +198,7 @@ __device__ __forceinline__ void sort_registers(Samples<CAP>& v) {
 // not the capacity.  The group test is a wave-uniform branch; the empty
 // volatile asm keeps it one (a side effect cannot be speculated, so the chain
 // is not if-converted into selects over every slot).
-constexpr int kScanGroup = 8;
+constexpr int kScanGroup = 4;
 
 template <int CAP, bool FROM_TOP, typename Hit, typename Body>
 __device__ __forceinline__ void scan_ends(const Samples<CAP>& v, Hit&& hit, Body&& body) {
@@ -349,6 +349,20 @@ __device__ __forceinline__ void winsorized_pixel(const IntegrateArgs& a, long lo
   const bool in_range = p < a.npix;
   const long long pp = in_range ? p : 0;
 
+  // Per-tile choice of the statistics arithmetic (CPUStackEngine tiles) or
+  // one choice for the whole frame (weighted_integrate_stack).  Decided
+  // before the samples are loaded: the division needs registers that the
+  // sorted keys occupy later.
+  bool all_valid = (a.variant_ptr ? *(const global_u32*)a.variant_ptr : static_cast<unsigned>(a.variant_all_valid)) != 0u;
+  if (a.tile_flags) {
+    const unsigned w = static_cast<unsigned>(a.width);
+    const unsigned y = static_cast<unsigned>(pp) / w;  // npix < 2^30
+    const unsigned x = static_cast<unsigned>(pp) - y * w;
+    const unsigned t = static_cast<unsigned>(a.tile_size);
+    all_valid = a.tile_flags[(y / t) * static_cast<unsigned>(a.tiles_x) + (x / t)] != 0;
+  }
+  asm volatile("" ::"v"(all_valid));
+
   Samples<CAP> v;
   int cnt = 0;
   double swx = 0.0, sw = 0.0;
@@ -399,15 +413,6 @@ __device__ __forceinline__ void winsorized_pixel(const IntegrateArgs& a, long lo
   // together.  Pixels without samples (cnt == 0) run the network on +inf keys
   // and are written as zeros at the end.
   sort_registers<CAP>(v);
-
-  // Per-tile choice of the statistics arithmetic (CPUStackEngine tiles) or
-  // one choice for the whole frame (weighted_integrate_stack).
-  bool all_valid = (a.variant_ptr ? *(const global_u32*)a.variant_ptr : static_cast<unsigned>(a.variant_all_valid)) != 0u;
-  if (a.tile_flags) {
-    const int y = static_cast<int>(pp / a.width);
-    const int x = static_cast<int>(pp - static_cast<long long>(y) * a.width);
-    all_valid = a.tile_flags[(y / a.tile_size) * a.tiles_x + (x / a.tile_size)] != 0;
-  }
 
   // Waterfall over the distinct valid counts of the wave; each pass fetches
   // the six rank registers with a wave-uniform register index.
