@@ -835,8 +835,8 @@ void launch_integrate(const IntegrateArgs& a, hipStream_t stream) {
     launch_cap<64>(a, stream);
   } else if (a.n_active <= 128) {
     launch_cap<128>(a, stream);
-  } else if (a.n_active <= 208) {
-    launch_cap<208>(a, stream);
+  } else if (a.n_active <= 200) {
+    launch_cap<200>(a, stream);
   } else if (a.n_active <= 256) {
     launch_cap<256>(a, stream);
   } else if (a.n_active <= kLdsSortMaxFrames) {
