@@ -35,6 +35,7 @@ constexpr int kBlock = 256;
 // first active lane's words marks the values wave-uniform: pointers become
 // scalar (SGPR) bases.  Valid only where every lane reads the same entry.
 typedef __attribute__((address_space(1))) const unsigned global_u32;
+typedef __attribute__((address_space(1))) const char global_char;
 typedef __attribute__((address_space(1))) const unsigned long long global_u64;
 
 typedef __attribute__((address_space(1))) float global_f32;

@@ -97,7 +97,6 @@ constexpr int kLdsSortMaxFrames = kLdsSortBytes / 4;
 // also occupy the LDS/scalar counter and serialise behind every scalar load)
 // into global loads that can all be in flight at once.
 typedef __attribute__((address_space(1))) const float global_float;
-typedef __attribute__((address_space(1))) const char global_char;
 __device__ __forceinline__ float load_global(const float* base, long long i) {
   return ((const global_float*)base)[i];  // C-style cast = addrspacecast
 }
