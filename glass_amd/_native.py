@@ -211,6 +211,53 @@ class DeviceProperties(ctypes.Structure):
     ]
 
 
+class TriangleInfo(ctypes.Structure):
+    _fields_ = [("count", c_int64), ("raw_count", c_int64), ("neighbors", c_int32), ("reserved", c_int32)]
+
+
+class SimilarityFit(ctypes.Structure):
+    _fields_ = [
+        ("matrix", c_float * 9),
+        ("scale", c_float),
+        ("rotation_rad", c_float),
+        ("rms_px", c_float),
+        ("inliers", c_int32),
+        ("best_candidate_index", c_int64),
+        ("candidate_count", c_int64),
+        ("scored_candidates", c_int64),
+        ("refined_inliers", c_int32),
+        ("refit_status", c_int32),
+        ("refit_rms_px", c_float),
+        ("reserved", c_int32),
+    ]
+
+
+class CatalogSimilarityParams(ctypes.Structure):
+    _fields_ = [
+        ("tolerance_px", c_float),
+        ("min_pair_distance", c_float),
+        ("prior_dx", c_float),
+        ("prior_dy", c_float),
+        ("prior_radius_px", c_float),
+        ("min_scale", c_float),
+        ("max_scale", c_float),
+        ("max_abs_rotation_rad", c_float),
+        ("top_k", c_int32),
+    ]
+
+
+class SimilarityCandidate(ctypes.Structure):
+    _fields_ = [
+        ("candidate_index", c_int64),
+        ("inliers", c_int32),
+        ("rms_px", c_float),
+        ("a", c_float),
+        ("b", c_float),
+        ("tx", c_float),
+        ("ty", c_float),
+    ]
+
+
 GH_BATCH_USE_GRAPH = 1
 GH_BATCH_AUTOTUNE = 2
 
@@ -296,6 +343,20 @@ _SIGNATURES = {
     "gh_batch_run": [c_void_p, c_void_p],
     "gh_batch_info": [c_void_p, POINTER(c_int64)],
     "gh_batch_destroy": [c_void_p],
+    "gh_triangle_descriptors": [c_void_p, c_void_p, c_int64, c_int32, c_int32, c_int32, c_void_p, c_void_p,
+                                c_void_p, POINTER(TriangleInfo)],
+    "gh_triangle_descriptors_batch": [POINTER(c_void_p), POINTER(c_void_p), POINTER(c_int64), c_int64, c_int32,
+                                      c_int32, c_int32, c_void_p, c_void_p, c_void_p, POINTER(TriangleInfo)],
+    "gh_similarity_from_triangles": [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
+                                     c_int64, c_void_p, c_void_p, c_int64, c_float, c_float,
+                                     POINTER(SimilarityFit)],
+    "gh_similarity_from_triangles_batch": [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int64, c_int64,
+                                           POINTER(c_void_p), POINTER(c_void_p), POINTER(c_int64),
+                                           POINTER(c_void_p), POINTER(c_void_p), POINTER(c_int64), c_float,
+                                           c_float, POINTER(SimilarityFit)],
+    "gh_similarity_from_catalogs": [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int64,
+                                    POINTER(CatalogSimilarityParams), POINTER(SimilarityFit),
+                                    POINTER(SimilarityCandidate), POINTER(c_int32)],
     "gh_estimate_star_transform": [POINTER(c_double), c_int64, POINTER(c_double), c_int64, c_int32, c_int64,
                                    c_double, c_int64, POINTER(c_double), POINTER(c_int64), POINTER(c_int64),
                                    POINTER(c_double), POINTER(c_int32)],

@@ -17,6 +17,10 @@ Attributes provided (each replaces the reference binding of the same name):
 * ``host_pinned_empty_f32(h, w)`` / ``host_pinned_empty_u8(n)``
   native_bindings.cpp:1946-1990
 * ``ResidentCalibratedStack``     native_bindings.cpp:20624-21213
+* ``triangle_asterism_descriptors_f32`` / ``_batch_f32``,
+  ``estimate_similarity_from_triangle_descriptors_f32`` / ``_batch_f32``,
+  ``estimate_similarity_from_catalogs_f32``  native_bindings.cpp:17044-18678
+  (the resident default registration route, resident_cuda.py:9985-11399)
 
 Deliberately absent, so the reference's ``hasattr`` probes take their Python
 routes: ``RawFitsReadQueue`` (``raw_fits_read_queue_available()`` is then
@@ -35,6 +39,13 @@ import numpy as np
 
 from . import _native
 from .resident import ResidentCalibratedStack
+from .triangles import (
+    estimate_similarity_from_catalogs_f32,
+    estimate_similarity_from_triangle_descriptors_batch_f32,
+    estimate_similarity_from_triangle_descriptors_f32,
+    triangle_asterism_descriptors_batch_f32,
+    triangle_asterism_descriptors_f32,
+)
 
 __all__ = [
     "cuda_available",
@@ -43,6 +54,11 @@ __all__ = [
     "host_pinned_empty_f32",
     "host_pinned_empty_u8",
     "ResidentCalibratedStack",
+    "triangle_asterism_descriptors_f32",
+    "triangle_asterism_descriptors_batch_f32",
+    "estimate_similarity_from_triangle_descriptors_f32",
+    "estimate_similarity_from_triangle_descriptors_batch_f32",
+    "estimate_similarity_from_catalogs_f32",
 ]
 
 

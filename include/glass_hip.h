@@ -499,6 +499,118 @@ int gh_estimate_star_transform(const double* ref_xy, int64_t n_ref, const double
                                int32_t model, int64_t min_inliers, double tolerance_px, int64_t max_stars,
                                double* matrix, int64_t* pairs, int64_t* n_pairs, double* rms, int32_t* ok);
 
+/* ------------------------------------- triangle-asterism similarity (host arrays)
+ * The resident default registration route similarity_cuda_triangle: local
+ * triangle descriptors of compact star catalogues, similarity transforms from
+ * matched descriptors, and the two-star-pair catalogue seed matcher
+ * (src/glass_cuda.py:1726-2140; cpp/src/native_bindings.cpp:17044-18678;
+ * cpp/cuda/registration_kernels.cu:746-1821).  Inputs and outputs are host
+ * memory; each call synchronises before it returns.  Calls on one process
+ * serialise on a per-device workspace. */
+
+typedef struct gh_triangle_info {
+  int64_t count;     /* descriptors written (<= max_descriptors) */
+  int64_t raw_count; /* anchors x C(neighbors, 3) */
+  int32_t neighbors; /* neighbour count used: clamp(min(stars, neighbors), 3, 16) */
+  int32_t reserved;
+} gh_triangle_info;
+
+/* triangle_asterism_descriptors_f32(x, y, max_stars, neighbors,
+ * max_descriptors) (glass_cuda.py:1835-1901, native_bindings.cpp:17044-17253):
+ * the first min(n, max_stars) stars; every star and its nearest neighbours
+ * give C(neighbors, 3) triangles; triangles with area < 2 px^2 or a side
+ * <= 1e-6 drop out; one descriptor per star triple (the first in anchor /
+ * combination order), ordered by area descending then by the sorted triple.
+ * descriptors[2 * max_descriptors] (longest/middle, middle/shortest),
+ * indices[3 * max_descriptors] (longest side's far-from-third vertex, the
+ * other end, the third vertex), areas[max_descriptors]. */
+int gh_triangle_descriptors(const float* x, const float* y, int64_t n, int32_t max_stars, int32_t neighbors,
+                            int32_t max_descriptors, float* descriptors, int32_t* indices, float* areas,
+                            gh_triangle_info* info);
+
+/* triangle_asterism_descriptors_batch_f32 (glass_cuda.py:1935-1986,
+ * native_bindings.cpp:17255-17595): `batch` catalogues in one launch; outputs
+ * of catalogue b at descriptors + 2 * b * max_descriptors, indices + 3 * b *
+ * max_descriptors, areas + b * max_descriptors, info[b]. */
+int gh_triangle_descriptors_batch(const float* const* x, const float* const* y, const int64_t* n, int64_t batch,
+                                  int32_t max_stars, int32_t neighbors, int32_t max_descriptors, float* descriptors,
+                                  int32_t* indices, float* areas, gh_triangle_info* info);
+
+/* One similarity estimate: matrix row-major moving -> reference
+ * [[a, -b, tx], [b, a, ty], [0, 0, 1]] (identity when nothing matched). */
+typedef struct gh_similarity_fit {
+  float matrix[9];
+  float scale;
+  float rotation_rad;
+  float rms_px;                 /* NaN when nothing matched */
+  int32_t inliers;              /* mutual nearest-neighbour matches of the best seed */
+  int64_t best_candidate_index; /* -1 when nothing matched */
+  int64_t candidate_count;      /* seeds enumerated */
+  int64_t scored_candidates;    /* seeds that passed the descriptor / pair gates */
+  int32_t refined_inliers;      /* catalogue estimate: one-sided matches of the refit */
+  int32_t refit_status;         /* catalogue estimate: 0 ok, 1 < 2 matches, 2 degenerate,
+                                   3 rejected (seed kept); -1 not run */
+  float refit_rms_px;
+  int32_t reserved;
+} gh_similarity_fit;
+
+/* estimate_similarity_from_triangle_descriptors_f32(reference_x, reference_y,
+ * moving_x, moving_y, reference_descriptors, reference_indices,
+ * moving_descriptors, moving_indices, tolerance_px, descriptor_radius)
+ * (glass_cuda.py:2050-2140, native_bindings.cpp:17597-17848): every
+ * (reference, moving) descriptor pair within descriptor_radius, with both
+ * orders of the reference triangle's first two vertices, seeds an FP64
+ * three-point similarity fit scored by mutual nearest neighbours within
+ * tolerance_px; the best seed has most inliers, then lowest RMS, then lowest
+ * candidate index (rd * n_mov_desc + md) * 2 + order. */
+int gh_similarity_from_triangles(const float* rx, const float* ry, int64_t nr, const float* mx, const float* my,
+                                 int64_t nm, const float* rdesc, const int32_t* ridx, int64_t nrd, const float* mdesc,
+                                 const int32_t* midx, int64_t nmd, float tolerance_px, float descriptor_radius,
+                                 gh_similarity_fit* out);
+
+/* estimate_similarity_from_triangle_descriptors_batch_f32 (glass_cuda.py:
+ * 2143-2240, native_bindings.cpp:17850-18252): one shared reference catalogue
+ * and descriptor set, `batch` moving catalogues; out[b] equals the single
+ * call on moving catalogue b. */
+int gh_similarity_from_triangles_batch(const float* rx, const float* ry, int64_t nr, const float* rdesc,
+                                       const int32_t* ridx, int64_t nrd, int64_t batch, const float* const* mx,
+                                       const float* const* my, const int64_t* nm, const float* const* mdesc,
+                                       const int32_t* const* midx, const int64_t* nmd, float tolerance_px,
+                                       float descriptor_radius, gh_similarity_fit* out);
+
+/* Parameters of gh_similarity_from_catalogs with the binding's conventions
+ * (native_bindings.cpp:18283-18302): prior_radius_px < 0 disables the prior
+ * (prior_dx / prior_dy then read 0), max_abs_rotation_rad < 0 disables the
+ * rotation gate, max_scale FLT_MAX is unbounded. */
+typedef struct gh_catalog_similarity_params {
+  float tolerance_px;
+  float min_pair_distance;
+  float prior_dx, prior_dy, prior_radius_px;
+  float min_scale, max_scale;
+  float max_abs_rotation_rad;
+  int32_t top_k;
+} gh_catalog_similarity_params;
+
+typedef struct gh_similarity_candidate {
+  int64_t candidate_index;
+  int32_t inliers;
+  float rms_px;
+  float a, b, tx, ty; /* matrix [[a, -b, tx], [b, a, ty], [0, 0, 1]] */
+} gh_similarity_candidate;
+
+/* estimate_similarity_from_catalogs_f32 (glass_cuda.py:1726-1824,
+ * native_bindings.cpp:18254-18678): every ordered reference star pair against
+ * every ordered moving star pair seeds a similarity (scale / rotation / prior
+ * gates), scored like the triangle seeds; the best seed is refitted in FP64
+ * over its one-sided matches and the refit replaces it unless it has fewer
+ * matches or a higher RMS.  top[top_k] receives the diversified top-k seed
+ * list (per inlier count the ceil(top_k / 4) lowest-RMS seeds from the highest
+ * count down, then the overall best), *top_count its length.  Requires
+ * nr (nr - 1) nm (nm - 1) < 2^31. */
+int gh_similarity_from_catalogs(const float* rx, const float* ry, int64_t nr, const float* mx, const float* my,
+                                int64_t nm, const gh_catalog_similarity_params* params, gh_similarity_fit* out,
+                                gh_similarity_candidate* top, int32_t* top_count);
+
 /* --------------------------------------- affine Lanczos3 adjoint / refinement
  * No counterpart in the reference: the warp of src/glass/engine/warp.py:121-228
  * (CPU-path Lanczos3, normalised weights) for an affine source map

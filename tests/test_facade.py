@@ -26,6 +26,11 @@ MODULE_ATTRIBUTES = [
     ("host_pinned_empty_f32", 219),  # resident_cuda.py host rings
     ("host_pinned_empty_u8", 226),
     ("ResidentCalibratedStack", 2665),  # resident_cuda.py:205 hasattr, stack construction
+    ("estimate_similarity_from_catalogs_f32", 1753),  # resident_cuda.py:9985
+    ("triangle_asterism_descriptors_f32", 1855),  # resident_cuda.py:10852 (required at :10311)
+    ("triangle_asterism_descriptors_batch_f32", 1967),  # resident_cuda.py:11220
+    ("estimate_similarity_from_triangle_descriptors_f32", 2084),  # resident_cuda.py:11399 (required at :10311)
+    ("estimate_similarity_from_triangle_descriptors_batch_f32", 2202),  # resident_cuda.py:11294
 ]
 
 # Probed with hasattr; absent means the reference's own default route.
@@ -91,6 +96,26 @@ def test_optional_attribute_absent(name, line):
 def test_stack_member(name, line):
     assert hasattr(native_module.ResidentCalibratedStack, name), \
         f"glass_cuda.py:{line} calls the native stack's {name}"
+
+
+def test_triangle_signatures_match_native_positional_order():
+    """glass_cuda.py passes the native calls positionally (:1753-1767,
+    :1855-1861, :1967-1973, :2084-2095, :2202-2213)."""
+    order = {
+        "estimate_similarity_from_catalogs_f32": [
+            "reference_x", "reference_y", "moving_x", "moving_y", "tolerance_px", "min_pair_distance", "prior_dx",
+            "prior_dy", "prior_radius_px", "min_scale", "max_scale", "max_abs_rotation_rad", "top_k"],
+        "triangle_asterism_descriptors_f32": ["x", "y", "max_stars", "neighbors", "max_descriptors"],
+        "triangle_asterism_descriptors_batch_f32": ["x_list", "y_list", "max_stars", "neighbors", "max_descriptors"],
+        "estimate_similarity_from_triangle_descriptors_f32": [
+            "reference_x", "reference_y", "moving_x", "moving_y", "reference_descriptors", "reference_indices",
+            "moving_descriptors", "moving_indices", "tolerance_px", "descriptor_radius"],
+        "estimate_similarity_from_triangle_descriptors_batch_f32": [
+            "reference_x", "reference_y", "reference_descriptors", "reference_indices", "moving_x_list",
+            "moving_y_list", "moving_descriptors_list", "moving_indices_list", "tolerance_px", "descriptor_radius"],
+    }
+    for name, params in order.items():
+        assert list(inspect.signature(getattr(native_module, name)).parameters) == params, name
 
 
 def test_integrate_signature_matches_reference_keywords():
