@@ -78,34 +78,89 @@ def cpu_baseline(args, light_exposure, dark_exposure):
     from glass_amd.synthetic import affine_matrix, encode_fits_u16be_bzero, make_set, to_adu
     from oracle import cpu_ref
 
-    f, h, w = args.frames, args.cpu_rows, args.width
-    data = make_set(seed=args.seed, n_lights=f, height=h, width=w, n_cal=3, n_stars=max(4, 3000 * h // args.height),
-                    fwhm=3.0, max_shift=2.0, max_rot_deg=0.005, sky=0.02)
-    raw = encode_fits_u16be_bzero(to_adu(data.light)).reshape(f, -1)
-    bias = to_adu(data.bias.mean(0)).astype(np.float32)
-    dark = to_adu(data.dark.mean(0)).astype(np.float32)
-    flat = np.maximum(data.flat.mean(0) / np.median(data.flat.mean(0)), 1e-6).astype(np.float32)
-    weights = np.ones(f, np.float32)
-    weights[1:1 + args.zero_weight] = 0.0
-    invs = [np.linalg.inv(m) for m in data.matrices]
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     cpu_ref.load()
-    t0 = time.perf_counter()
-    stack = cpu_ref.calibrate_u16be(raw, h * w, bias, dark, flat, float(np.float32(light_exposure / dark_exposure)))
-    stack = stack.reshape(f, h, w)
-    for i in range(1, f):
-        stack[i], _ = cpu_ref.warp(stack[i], invs[i], args.interpolation, np.nan)
-    cpu_ref.integrate_winsorized(stack, weights)
-    elapsed = time.perf_counter() - t0
-    frame_mpix = f * h * w / 1e6
+
+    def timed(h, n_threads):
+        f, w = args.frames, args.width
+        data = make_set(seed=args.seed, n_lights=f, height=h, width=w, n_cal=3,
+                        n_stars=max(4, 3000 * h // args.height), fwhm=3.0, max_shift=2.0, max_rot_deg=0.005,
+                        sky=0.02)
+        raw = encode_fits_u16be_bzero(to_adu(data.light)).reshape(f, -1)
+        bias = to_adu(data.bias.mean(0)).astype(np.float32)
+        dark = to_adu(data.dark.mean(0)).astype(np.float32)
+        flat = np.maximum(data.flat.mean(0) / np.median(data.flat.mean(0)), 1e-6).astype(np.float32)
+        weights = np.ones(f, np.float32)
+        weights[1:1 + args.zero_weight] = 0.0
+        invs = [np.linalg.inv(m) for m in data.matrices]
+        cpu_ref.set_threads(n_threads)
+        t0 = time.perf_counter()
+        stack = cpu_ref.calibrate_u16be(raw, h * w, bias, dark, flat,
+                                        float(np.float32(light_exposure / dark_exposure)))
+        stack = stack.reshape(f, h, w)
+        for i in range(1, f):
+            stack[i], _ = cpu_ref.warp(stack[i], invs[i], args.interpolation, np.nan)
+        cpu_ref.integrate_winsorized(stack, weights)
+        elapsed = time.perf_counter() - t0
+        return f * h * w / 1e6 / elapsed, elapsed
+
+    rate, elapsed = timed(args.cpu_rows, threads)
+    # one core on an eighth of the rows (the same per-pixel work, bounded time)
+    rows1 = max(1, args.cpu_rows // 8)
+    rate1, elapsed1 = timed(rows1, 1)
+    cpu_ref.set_threads(threads)
     return {
-        "value": round(frame_mpix / elapsed, 3),
+        "value": round(rate, 3),
         "unit": "frame-Mpix/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{f} frames x {h} x {w} (first {h} rows; sub-pixel drift), calibrate+{args.interpolation} "
-                  f"warp+winsorized integrate in oracle/cpu_ref.c, {elapsed:.2f} s",
+        "sample": f"{args.frames} frames x {args.cpu_rows} x {args.width} (first {args.cpu_rows} rows; sub-pixel "
+                  f"drift), calibrate+{args.interpolation} warp+winsorized integrate in oracle/cpu_ref.c, "
+                  f"{elapsed:.2f} s",
+        "one_core": {"value": round(rate1, 3), "unit": "frame-Mpix/s", "cores": 1,
+                     "sample": f"{args.frames} frames x {rows1} x {args.width}, {elapsed1:.2f} s"},
     }
+
+
+# GPU stack band of the refined-registration leg, checked against the CPU
+# oracle inside the cpu_baseline leg (the only leg that runs the oracle)
+_REFINE_STACK_CHECK: dict = {}
+
+
+def refine_stack_vs_oracle():
+    """RMS and P99 of the refined-leg stack band against the CPU oracle
+    (oracle windowed Lanczos3 warp with the same refined matrices, oracle
+    winsorized integration), relative to the oracle band's RMS / range."""
+    import numpy as np
+
+    import oracle
+    from oracle.warp import source_box, warp_matrix_window
+
+    c = _REFINE_STACK_CHECK
+    size = c["size"]
+    x0, y0, w, h = c["band"]
+    samples = np.empty((len(c["matrices"]), h, w), np.float32)
+    cov = np.empty_like(samples)
+    for k, m in enumerate(c["matrices"]):
+        crop = c["crops"][k]
+        if np.array_equal(m, np.eye(3)):
+            samples[k] = crop[y0 - c["crop_y0"]:y0 - c["crop_y0"] + h]
+            cov[k] = 1.0
+            continue
+        bx0, by0, bx1, by1 = source_box(m, c["band"], (size, size))
+        assert by0 >= c["crop_y0"] and by1 <= c["crop_y0"] + crop.shape[0], "band source outside the kept rows"
+        samples[k], cov[k] = warp_matrix_window(crop[by0 - c["crop_y0"]:by1 - c["crop_y0"], bx0:bx1], m, c["band"],
+                                                origin=(bx0, by0), full_shape=(size, size), fill=np.nan)
+    weights = np.ones(len(samples), np.float32)
+    exp = oracle.weighted_integrate_stack(samples, coverage=cov, weights=weights, rejection="winsorized_sigma",
+                                          low_sigma=3.0, high_sigma=3.0, min_samples=3, max_reject_fraction=0.5)
+    has = exp[1] > 0
+    ref = exp[0][has].astype(np.float64)
+    d = np.abs(c["master"][has].astype(np.float64) - ref)
+    return {"band": f"rows {y0}..{y0 + h - 1} x {w}", "pixels": int(has.sum()),
+            "rms_rel": float(np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(ref * ref))),
+            "p99_of_range": float(np.percentile(d, 99) / (ref.max() - ref.min())),
+            "weight_map_equal": bool(np.array_equal(c["weight"][has], exp[1][has]))}
 
 
 def refine_leg(args):
@@ -168,13 +223,30 @@ def refine_leg(args):
     # one untimed refinement (workspace allocation)
     stack.refine_affine(0, 1, starts[0], max_iterations=2, tolerance_px=1e-3)
     t0 = time.perf_counter()
-    errs, iters = [], []
+    errs, iters, refined = [], [], [np.eye(3)]
     for k in range(n):
-        refined, rep = stack.refine_affine(0, k + 1, starts[k], max_iterations=20, tolerance_px=1e-3)
+        m, rep = stack.refine_affine(0, k + 1, starts[k], max_iterations=20, tolerance_px=1e-3)
+        refined.append(m)
         iters.append(rep["iterations"])
-        errs.append(float(np.max(np.abs((refined @ corners)[:2] - (truths[k] @ corners)[:2]))))
+        errs.append(float(np.max(np.abs((m @ corners)[:2] - (truths[k] @ corners)[:2]))))
     elapsed = time.perf_counter() - t0
     start_err = max(float(np.max(np.abs((starts[k] @ corners)[:2] - (truths[k] @ corners)[:2]))) for k in range(n))
+    # the stack of the refined frames (Lanczos3, CPU-path semantics,
+    # winsorized 3/3, unit weights) on the GPU; a row band of it and the
+    # source rows that band reads are kept for the CPU-oracle check of the
+    # cpu_baseline leg
+    y0, rows, margin = size // 2 - 16, 32, 64
+    cy0, cy1 = max(0, y0 - margin), min(size, y0 + rows + margin)
+    crops = np.stack([stack.download_frame_tile(k, 0, cy0, size, cy1) for k in range(n + 1)])
+    stack.warp_frames(list(range(1, n + 1)), np.stack(refined[1:]), "lanczos3", "cpu")
+    outs = {k: torch.empty((size, size), dtype=torch.float32, device=dev) for k in ("master", "weight", "coverage",
+                                                                                     "low", "high")}
+    stack.integrate_to_device(outs, None, "winsorized_sigma", 3.0, 3.0, 3, 0.5, "float32", dq_geometric=False)
+    torch.cuda.synchronize()
+    _REFINE_STACK_CHECK.update(size=size, band=(0, y0, size, rows), crop_y0=cy0, crops=crops, matrices=refined,
+                               master=outs["master"][y0:y0 + rows].cpu().numpy(),
+                               weight=outs["weight"][y0:y0 + rows].cpu().numpy())
+    del outs
     stack.close()
     torch.cuda.empty_cache()
     return {
@@ -407,6 +479,8 @@ def registration_leg(args, stack, light_raw, a, frames=9):
         tr = [stack.estimate_translation_from_stars_to_reference(0, i, thr, 64, 1.0) for i in idx[1:]]
         cat["catalogue_translation_ms_per_frame"] = round((time.perf_counter() - t) / max(1, f - 1) * 1e3, 2)
         cat["catalogue_translation_mutual_min"] = int(min(x["mutual_inliers"] for x in tr)) if tr else 0
+    if np.isfinite(thr):
+        cat["triangle_route"] = triangle_route(stack, light_raw, a, thr, corners)
     out = {"frames": f, "detect_ms_per_frame": round(detect_s * 1e3, 2), "fit_ms_per_frame": round(fit_s * 1e3, 2),
            **cat,
            "stars_per_frame": round(float(np.mean([len(x) for x in stars])), 1),
@@ -414,6 +488,47 @@ def registration_leg(args, stack, light_raw, a, frames=9):
            "corner_err_px_max": round(max(errs), 4) if errs else None,
            "corner_err_px_median": round(float(np.median(errs)), 4) if errs else None}
     log(f"registration: {out}")
+    return out
+
+
+def triangle_route(stack, light_raw, a, thr, corners):
+    """The resident default registration (star catalogues with centroids ->
+    triangle descriptors -> similarity fits, glass_amd.triangles.
+    register_resident_frames) over every calibrated, unwarped light against
+    frame 0, at the route's defaults (64 stars, 5 neighbours, 1200
+    descriptors, tolerance 1 px, descriptor radius 0.1).  Wall clock of the
+    whole chain per moving frame, and the corner error of each fit against
+    the generator's map."""
+    import numpy as np
+    import torch
+
+    from glass_amd.triangles import register_resident_frames
+
+    f = stack.frame_count
+    idx = list(range(f))
+    stack.calibrate_frames_device(idx, light_raw, "fits_u16be_bzero", [a["light_exp"]] * f, a["dark_exp"],
+                                  a["policy"])
+    torch.cuda.synchronize()
+    register_resident_frames(stack, 0, [1], thr)  # sizes the workspaces
+    route = register_resident_frames(stack, 0, idx[1:], thr)
+    errs, inl = [], []
+    for i, fit in route["fits"].items():
+        got = np.asarray(fit["matrix"], np.float64) @ corners
+        exp = np.asarray(a["mats"][i], np.float64) @ corners
+        errs.append(float(np.max(np.hypot(got[0] - exp[0], got[1] - exp[1]))))
+        inl.append(int(fit["inliers"]))
+    t = route["timing_s"]
+    n = max(1, f - 1)
+    out = {"frames": f - 1, "fitted": len(route["fits"]),
+           "catalogue_ms_per_frame": round(t["catalogues"] / f * 1e3, 3),
+           "descriptors_ms_per_frame": round(t["descriptors"] / f * 1e3, 3),
+           "similarity_ms_per_frame": round(t["similarity"] / n * 1e3, 3),
+           "total_ms": round(t["total"] * 1e3, 2),
+           "stars_min": int(min(route["catalogue_counts"].values())),
+           "inliers_min": min(inl) if inl else 0,
+           "corner_err_px_max": round(max(errs), 4) if errs else None,
+           "corner_err_px_median": round(float(np.median(errs)), 4) if errs else None}
+    log(f"triangle route: {out}")
     return out
 
 
@@ -743,6 +858,9 @@ def main():
     if not args.no_cpu_baseline:
         log("timing the CPU baseline sample")
         line["cpu_baseline"] = cpu_baseline(args, light_exp, dark_exp)
+        if _REFINE_STACK_CHECK and "refined_registration" in line:
+            log("refined-leg stack band against the CPU oracle")
+            line["refined_registration"]["stack_vs_oracle"] = refine_stack_vs_oracle()
     print(json.dumps(line), flush=True)
 
 

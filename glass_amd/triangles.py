@@ -325,3 +325,68 @@ def estimate_similarity_from_catalogs_f32(reference_x, reference_y, moving_x, mo
         "status": "ok" if fit.inliers > 0 else "failed",
         "model": "catalog_pair_similarity_hip",
     }
+
+
+# ---------------------------------------------------------------- resident route
+
+
+def register_resident_frames(stack, reference_index: int, indices, threshold: float, max_stars: int = 64,
+                             neighbors: int = 5, max_descriptors: int = 1200, tolerance_px: float = 1.0,
+                             descriptor_radius: float = 0.1, scan_candidates: int | None = None,
+                             min_separation_px: float | None = None, centroid_radius: int = 4,
+                             centroid_global_mean_background: bool = True) -> dict:
+    """The catalogue -> descriptor -> similarity chain of the resident default
+    registration (resident_cuda.py:10820-10860 catalogues, :11200-11300 batched
+    descriptors and fits; defaults of resident_cuda.py:7134-7141 and
+    :10320-10510) on resident frames of a ``ResidentCalibratedStack``:
+
+    * catalogue of each frame: the brightest ``scan_candidates`` (default
+      4 x max_stars) local maxima above ``threshold``, greedy separation
+      ``min_separation_px`` (default max(32, min(h, w) / 100)), ``max_stars``
+      kept, window centroids of radius ``centroid_radius``;
+    * descriptors of every catalogue in one batched call;
+    * similarity of every moving frame against the reference in one batched
+      call.
+
+    Returns {"fits": {index: fit dict (matrix moving -> reference)},
+    "catalogue_counts": {index: stars}, "timing_s": {...}}.  Frames whose
+    catalogue has fewer than 3 stars or no descriptor get no fit (the
+    reference skips them the same way, resident_cuda.py:11209-11212,
+    :11283-11287)."""
+    h, w = stack.height, stack.width
+    scan = int(scan_candidates) if scan_candidates is not None else max(int(max_stars), int(max_stars) * 4)
+    sep = float(min_separation_px) if min_separation_px is not None else max(32.0, float(min(h, w)) / 100.0)
+    frames = [int(reference_index)] + [int(i) for i in indices if int(i) != int(reference_index)]
+    t0 = time.perf_counter()
+    catalogs = {}
+    for i in frames:
+        catalogs[i] = stack.star_top_nms_candidates_centroid(i, float(threshold), scan, int(max_stars), sep,
+                                                             int(centroid_radius),
+                                                             bool(centroid_global_mean_background))
+    t1 = time.perf_counter()
+    usable = [i for i in frames if int(catalogs[i]["stored_count"]) >= 3]
+    desc = {}
+    if usable:
+        for i, d in zip(usable, triangle_asterism_descriptors_batch_f32(
+                [catalogs[i]["x"] for i in usable], [catalogs[i]["y"] for i in usable], max_stars=max_stars,
+                neighbors=neighbors, max_descriptors=max_descriptors)):
+            desc[i] = d
+    t2 = time.perf_counter()
+    fits = {}
+    ref = int(reference_index)
+    moving = [i for i in usable if i != ref and desc[i]["count"] > 0]
+    if ref in desc and desc[ref]["count"] > 0 and moving:
+        rc = catalogs[ref]
+        batch = estimate_similarity_from_triangle_descriptors_batch_f32(
+            rc["x"], rc["y"], desc[ref]["descriptors"], desc[ref]["indices"], [catalogs[i]["x"] for i in moving],
+            [catalogs[i]["y"] for i in moving], [desc[i]["descriptors"] for i in moving],
+            [desc[i]["indices"] for i in moving], tolerance_px=tolerance_px, descriptor_radius=descriptor_radius)
+        fits = dict(zip(moving, batch))
+    t3 = time.perf_counter()
+    return {
+        "fits": fits,
+        "catalogs": catalogs,
+        "descriptors": desc,
+        "catalogue_counts": {i: int(catalogs[i]["stored_count"]) for i in frames},
+        "timing_s": {"catalogues": t1 - t0, "descriptors": t2 - t1, "similarity": t3 - t2, "total": t3 - t0},
+    }

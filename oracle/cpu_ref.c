@@ -12,10 +12,22 @@
  *                arithmetic; linear percentiles; FP64 sums)
  * Build: gcc -O3 -ffp-contract=off -fopenmp -shared -fPIC (oracle/cpu_ref.py).
  */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* Worker threads of the OpenMP loops below (<= 0: the runtime default). */
+void cpu_set_threads(int threads) {
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
+}
 
 static float decode_be(const uint8_t* p) { return (float)((((unsigned)p[0] << 8) | p[1]) ^ 0x8000u); }
 

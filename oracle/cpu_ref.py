@@ -38,6 +38,7 @@ def load():
             build()
         lib = ctypes.CDLL(str(LIBRARY))
         f32p = ctypes.POINTER(ctypes.c_float)
+        lib.cpu_set_threads.argtypes = [ctypes.c_int]
         lib.cpu_calibrate_u16be.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_int,
                                             ctypes.c_float, ctypes.c_float, ctypes.c_void_p]
@@ -55,6 +56,11 @@ def load():
 
 def _p(a):
     return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def set_threads(threads: int) -> None:
+    """OpenMP worker threads of the restated loops (<= 0: runtime default)."""
+    load().cpu_set_threads(int(threads))
 
 
 def calibrate_u16be(raw, npix, bias, dark, flat, dark_scale, dark_includes_bias=True, flat_floor=1e-6, pedestal=0.0):

@@ -417,3 +417,34 @@ def test_best_seed_parameters_on_repeated_calls(gpu):
     assert exp["scored_candidates"] > 8192  # several candidates per scoring wave
     for _ in range(12):
         assert_fit_equal(G.estimate_similarity_from_triangle_descriptors_f32(*args, 3.0, 0.1), exp)
+
+
+def test_resident_triangle_route_recovers_frames(gpu):
+    """Catalogue -> descriptors -> similarity on resident frames (the resident
+    default route, resident_cuda.py:10820-11420) on lights with known affine
+    maps; every fit equals the oracle's on the same catalogues."""
+    from glass_amd import ResidentCalibratedStack
+    from glass_amd.synthetic import make_set
+
+    data = make_set(seed=5, n_lights=6, height=512, width=640, n_cal=1, n_stars=220, fwhm=3.0, max_shift=20.0,
+                    max_rot_deg=0.5, max_scale=5e-4, sky=0.02, noise=0.002)
+    with ResidentCalibratedStack(6, 512, 640) as stack:
+        for i in range(6):
+            stack.upload_calibrated_frame(i, data.light[i])
+        st = stack.frame_global_stats(0, exact_median=True)
+        thr = float(st["median"] + 8.0 * 1.4826 * st["mad"])
+        route = G.register_resident_frames(stack, 0, range(1, 6), thr, max_stars=64, tolerance_px=1.0)
+    corners = np.array([[0, 0, 1], [639, 0, 1], [0, 511, 1], [639, 511, 1]], np.float64).T
+    assert sorted(route["fits"]) == [1, 2, 3, 4, 5]
+    rc = route["catalogs"][0]
+    rd = T.triangle_descriptors(rc["x"], rc["y"], 64, 5, 1200)
+    for i, fit in route["fits"].items():
+        assert fit["status"] == "ok" and fit["inliers"] >= 20
+        err = np.abs((np.asarray(fit["matrix"], np.float64) @ corners)[:2] - (data.matrices[i] @ corners)[:2]).max()
+        assert err < 0.25, (i, err)
+        mc = route["catalogs"][i]
+        md = T.triangle_descriptors(mc["x"], mc["y"], 64, 5, 1200)
+        assert_descriptors_equal(route["descriptors"][i] | {"max_stars": 64}, md | {"max_stars": 64})
+        exp = T.similarity_from_triangles(rc["x"], rc["y"], mc["x"], mc["y"], rd["descriptors"], rd["indices"],
+                                          md["descriptors"], md["indices"], 1.0, 0.1)
+        assert_fit_equal(fit, exp)
