@@ -81,10 +81,11 @@ def cpu_baseline(args, light_exposure, dark_exposure):
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     cpu_ref.load()
 
-    def timed(h, n_threads):
-        f, w = args.frames, args.width
+    def timed(h, w, n_threads):
+        f = args.frames
         data = make_set(seed=args.seed, n_lights=f, height=h, width=w, n_cal=3,
-                        n_stars=max(4, 3000 * h // args.height), fwhm=3.0, max_shift=2.0, max_rot_deg=0.005,
+                        n_stars=max(4, 3000 * h * w // (args.height * args.width)), fwhm=3.0, max_shift=2.0,
+                        max_rot_deg=0.005,
                         sky=0.02)
         raw = encode_fits_u16be_bzero(to_adu(data.light)).reshape(f, -1)
         bias = to_adu(data.bias.mean(0)).astype(np.float32)
@@ -104,10 +105,11 @@ def cpu_baseline(args, light_exposure, dark_exposure):
         elapsed = time.perf_counter() - t0
         return f * h * w / 1e6 / elapsed, elapsed
 
-    rate, elapsed = timed(args.cpu_rows, threads)
-    # one core on an eighth of the rows (the same per-pixel work, bounded time)
-    rows1 = max(1, args.cpu_rows // 8)
-    rate1, elapsed1 = timed(rows1, 1)
+    rate, elapsed = timed(args.cpu_rows, args.width, threads)
+    # one core on an eighth of the columns (the same rows and frames, so the
+    # same per-pixel work and warp footprint; bounded time)
+    cols1 = max(64, args.width // 8)
+    rate1, elapsed1 = timed(args.cpu_rows, cols1, 1)
     cpu_ref.set_threads(threads)
     return {
         "value": round(rate, 3),
@@ -118,7 +120,8 @@ def cpu_baseline(args, light_exposure, dark_exposure):
                   f"drift), calibrate+{args.interpolation} warp+winsorized integrate in oracle/cpu_ref.c, "
                   f"{elapsed:.2f} s",
         "one_core": {"value": round(rate1, 3), "unit": "frame-Mpix/s", "cores": 1,
-                     "sample": f"{args.frames} frames x {rows1} x {args.width}, {elapsed1:.2f} s"},
+                     "sample": f"{args.frames} frames x {args.cpu_rows} x {cols1} (first rows and columns), "
+                               f"{elapsed1:.2f} s"},
     }
 
 
@@ -429,7 +432,9 @@ def batched_leg(args):
 # Measured HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, rocprofv3 --pmc, each
 # scaled by a known-size copy) of the default workload's kernels.
 _TRAFFIC_KERNELS = {"calibrate": "gh::calibrate_batch_kernel<2, false, 16>", "warp": "gh::warp_lanczos3_tiled_kernel",
-                    "integrate": "gh::integrate_winsorized_kernel<208, true>"}
+                    "integrate": "gh::integrate_winsorized_kernel<200, true>"}
+# VALU issue ceiling without packed FP32: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz
+VALU_PEAK_LANE_INSTS = 256 * 4 * 16 * 2.4e9
 
 
 def registration_leg(args, stack, light_raw, a, frames=9):
@@ -674,6 +679,18 @@ def measured_traffic(args, stage):
     return int(k["read_bytes"] + k["write_bytes"]), str(path.relative_to(ROOT))
 
 
+def measured_valu(args, stage):
+    """VALU wave-instructions per launch of a stage's kernel from the committed
+    SQ_INSTS_VALU profile of the default workload (None otherwise)."""
+    path = ROOT / "profiles" / "hbm_traffic_200x9600x6422.json"
+    default = (args.frames, args.height, args.width, args.interpolation, args.zero_weight) == (200, 6422, 9600,
+                                                                                                "lanczos3", 7)
+    if not default or not path.exists():
+        return None
+    k = json.loads(path.read_text())["kernels"].get(_TRAFFIC_KERNELS[stage], {})
+    return k.get("valu_wave_insts")
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -834,6 +851,15 @@ def main():
             "avg_launch_ms": round(avg_launch_ms, 4),
         },
     }
+    valu = measured_valu(args, dominant)
+    if valu:
+        rate = valu * 64 / (avg_launch_ms * 1e-3)
+        line["compute_roofline"] = {
+            "kernel": dominant, "bound": "valu", "achieved": round(rate / 1e12, 2), "peak": round(
+                VALU_PEAK_LANE_INSTS / 1e12, 2), "unit": "T lane-instr/s", "frac": round(rate / VALU_PEAK_LANE_INSTS, 4),
+            "valu_wave_insts_per_launch": valu, "source": "profiles/hbm_traffic_200x9600x6422.json "
+                                                           "(SQ_INSTS_VALU, profiles/sq_counters_warp_integrate_"
+                                                           "200x9600x6422.md)"}
     if not args.no_registration:
         log("star-registration leg: detect stars and fit affine matrices on calibrated, unwarped lights")
         line["registration"] = registration_leg(args, stack, light_raw, step_args)
