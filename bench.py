@@ -44,7 +44,7 @@ def parse_args():
     p.add_argument("--zero-weight", type=int, default=7)
     p.add_argument("--interpolation", default="lanczos3", choices=["lanczos3", "bilinear"])
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-rows", type=int, default=32, help="rows per frame of the CPU baseline sample")
+    p.add_argument("--cpu-rows", type=int, default=128, help="rows per frame of the CPU baseline sample")
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--no-refine", action="store_true", help="skip the refined-registration (config 4) leg")
     p.add_argument("--refine-frames", type=int, default=64)
@@ -119,6 +119,7 @@ def cpu_baseline(args, light_exposure, dark_exposure):
         "sample": f"{args.frames} frames x {args.cpu_rows} x {args.width} (first {args.cpu_rows} rows; sub-pixel "
                   f"drift), calibrate+{args.interpolation} warp+winsorized integrate in oracle/cpu_ref.c, "
                   f"{elapsed:.2f} s",
+        "numpy_oracle": numpy_oracle_timing(args.seed),
         "one_core": {"value": round(rate1, 3), "unit": "frame-Mpix/s", "cores": 1,
                      "sample": f"{args.frames} frames x {args.cpu_rows} x {cols1} (first rows and columns), "
                                f"{elapsed1:.2f} s"},
@@ -128,6 +129,54 @@ def cpu_baseline(args, light_exposure, dark_exposure):
 # GPU stack band of the refined-registration leg, checked against the CPU
 # oracle inside the cpu_baseline leg (the only leg that runs the oracle)
 _REFINE_STACK_CHECK: dict = {}
+
+
+def numpy_oracle_timing(seed):
+    """The numpy restatement of the CPU path (the oracle), single process,
+    timed on config 0 whole (8 x 128^2, masters from 4+4+4 frames, bilinear,
+    winsorized 3/3) and on a 64-row band of config 1's frame size (32 lights
+    of 64 x 2048, Lanczos3, winsorized 3/3, masters given).  frame-Mpix/s of
+    the calibrate + warp + integrate part."""
+    import numpy as np
+
+    import oracle
+    from glass_amd.synthetic import make_set
+
+    def run(data, interpolation, build_masters):
+        t0 = time.perf_counter()
+        if build_masters:
+            mb = oracle.build_master_bias(data.bias, {"tile_size": 512})
+            md = oracle.build_master_dark(data.dark, mb, {"tile_size": 512})
+            mf, _ = oracle.build_master_flat(data.flat, mb, {"tile_size": 512})
+        else:
+            mb, md = data.bias.mean(0), data.dark.mean(0)
+            mf = (data.flat.mean(0) / np.median(data.flat.mean(0))).astype(np.float32)
+        t1 = time.perf_counter()
+        cal = [oracle.calibrate_light(li, mb, md, mf, data.light_exposure_s, data.dark_exposure_s)
+               for li in data.light]
+        warped, covs = [], []
+        for i, c in enumerate(cal):
+            if np.array_equal(data.matrices[i], np.eye(3)):
+                warped.append(c)
+                covs.append(np.ones_like(c))
+            else:
+                w, cov = oracle.warp_matrix(c, data.matrices[i], fill=0.0, interpolation=interpolation)
+                warped.append(w)
+                covs.append(cov)
+        oracle.weighted_integrate_stack(np.stack(warped), coverage=np.stack(covs), rejection="winsorized_sigma")
+        t2 = time.perf_counter()
+        fpx = data.light.shape[0] * data.light.shape[1] * data.light.shape[2] / 1e6
+        return round(fpx / (t2 - t1), 4), round(t2 - t1, 3), round(t1 - t0, 3)
+
+    c0 = make_set(seed=seed)
+    r0, s0, m0 = run(c0, "bilinear", True)
+    c1 = make_set(seed=seed + 1, n_lights=32, height=64, width=2048, n_cal=2, n_stars=13, fwhm=2.5,
+                  max_shift=2.0, max_rot_deg=0.005, max_scale=5e-4)
+    r1, s1, _ = run(c1, "lanczos3", False)
+    return {"cores": 1, "config0": {"value": r0, "unit": "frame-Mpix/s", "path_s": s0, "masters_s": m0,
+                                    "sample": "8 x 128 x 128, bilinear, masters from 4+4+4 frames"},
+            "config1_band": {"value": r1, "unit": "frame-Mpix/s", "path_s": s1,
+                             "sample": "32 x 64 x 2048 (a row band of config 1's frame), Lanczos3"}}
 
 
 def refine_stack_vs_oracle():
@@ -580,7 +629,9 @@ def host_raw_leg(args, stack, light_raw, a):
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t
     sh = a["stream"].cuda_stream
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    maps_host = {k: torch.empty(v.shape, dtype=v.dtype, pin_memory=True) for k, v in a["outs"].items()}
+    maps_bytes = sum(v.numel() * v.element_size() for v in a["outs"].values())
     out = {}
     for run in range(2):
         torch.cuda.synchronize()
@@ -606,11 +657,20 @@ def host_raw_leg(args, stack, light_raw, a):
         ev[2].record(a["stream"])
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        # the output maps back to pinned host memory (the end of a run)
+        with torch.cuda.stream(a["stream"]):
+            for k, v in a["outs"].items():
+                maps_host[k].copy_(v, non_blocking=True)
+            ev[3].record(a["stream"])
+        torch.cuda.synchronize()
+        wall_maps = time.perf_counter() - t0
         upload_ms = ev[0].elapsed_time(ev[1])
         out = {"ms": round(wall * 1e3, 2), "frame_mpix_s": round(f * stack.pixels_per_frame / 1e6 / wall, 1),
                "upload_calibrate_warp_ms": round(upload_ms, 2),
                "upload_gbps": round(raw_bytes / (upload_ms * 1e-3) / 1e9, 1),
                "integrate_ms": round(ev[1].elapsed_time(ev[2]), 2), "raw_gb": round(raw_bytes / 1e9, 2),
+               "maps_download_ms": round(ev[2].elapsed_time(ev[3]), 2), "maps_gb": round(maps_bytes / 1e9, 3),
+               "end_to_end_with_maps_ms": round(wall_maps * 1e3, 2),
                "pinned_setup_s": round(setup_s, 2)}
         log(f"host raw run {run}: {out}")
     del host

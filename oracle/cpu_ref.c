@@ -71,10 +71,11 @@ static double lanczos3(double x) {
  * out gets fill outside the footprint; cov 0/1. */
 void cpu_warp(const float* src, int64_t h, int64_t w, const double* inv, int interp, float fill, float* out,
               float* cov) {
-#pragma omp parallel for schedule(dynamic, 16)
-  for (int64_t y = 0; y < h; ++y) {
-    for (int64_t x = 0; x < w; ++x) {
-      const int64_t o = y * w + x;
+  /* pixels in chunks, so a band of few rows still spreads over every thread */
+#pragma omp parallel for schedule(dynamic, 2048)
+  for (int64_t o = 0; o < h * w; ++o) {
+    {
+      const int64_t y = o / w, x = o - y * w;
       double den = inv[6] * (double)x + inv[7] * (double)y;
       den = den + inv[8];
       double sx = inv[0] * (double)x + inv[1] * (double)y;
@@ -98,12 +99,14 @@ void cpu_warp(const float* src, int64_t h, int64_t w, const double* inv, int int
       } else {
         if (!(sx >= 2.0 && sx < (double)(w - 3) && sy >= 2.0 && sy < (double)(h - 3))) continue;
         const int64_t bx = (int64_t)floor(sx), by = (int64_t)floor(sy);
+        double wxs[6];
+        for (int k = 0; k < 6; ++k) wxs[k] = lanczos3(sx - (double)(bx - 2 + k));
         double acc = 0.0, ws = 0.0;
         for (int64_t yy = by - 2; yy <= by + 3; ++yy) {
           const double wy = lanczos3(sy - (double)yy);
           if (wy == 0.0) continue;
           for (int64_t xx = bx - 2; xx <= bx + 3; ++xx) {
-            const double wx = lanczos3(sx - (double)xx);
+            const double wx = wxs[xx - bx + 2];
             if (wx == 0.0) continue;
             const double wt = wx * wy;
             acc += (double)src[yy * w + xx] * wt;
