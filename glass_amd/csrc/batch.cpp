@@ -253,6 +253,19 @@ int gh_batch_create(const gh_batch_job* jobs, int64_t n_jobs, uint32_t flags, gh
           fr.dst_flag = b->warped_flags.ptr + flag_base + static_cast<size_t>(k);
           wframes.push_back(fr);
         }
+        // Lanczos3 kernel choice per group of kWarpBatch frames, as the
+        // per-job path launches them.
+        for (int64_t g0 = 0; g0 < in.n_warp; g0 += gh::kWarpBatch) {
+          const int n = static_cast<int>(std::min<int64_t>(gh::kWarpBatch, in.n_warp - g0));
+          double inv[gh::kWarpBatch][9];
+          for (int j = 0; j < n; ++j) std::memcpy(inv[j], in.inverse + (g0 + j) * 9, sizeof(inv[j]));
+          const int streamed = in.interpolation == GH_INTERP_LANCZOS3 &&
+                                       gh::lanczos_stream_eligible(inv, n, static_cast<int>(s->width),
+                                                                   static_cast<int>(s->height))
+                                   ? 1
+                                   : 0;
+          for (int j = 0; j < n; ++j) wframes[wj.first_frame + g0 + j].stream = streamed;
+        }
         wjobs.push_back(wj);
       }
       const bool need_cov = in.outputs.dq && in.outputs.dq_use_geometric;

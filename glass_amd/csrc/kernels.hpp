@@ -114,6 +114,10 @@ void launch_snapshot_warp_flags(const WarpBatch& b, int count, unsigned* snap, h
 
 void launch_warp(const WarpBatch& batch, int count, const WarpParams& p, hipStream_t stream);
 
+// Whether an affine Lanczos3 group of `count` inverse maps (up to kWarpBatch,
+// launched together) takes the streamed kernel rather than the tiled one.
+bool lanczos_stream_eligible(const double (*inv)[9], int count, int width, int height);
+
 // ---- batched multi-job warp (one persistent launch over many jobs)
 struct BatchWarpFrame {
   const float* src;
@@ -121,6 +125,8 @@ struct BatchWarpFrame {
   double inv[9];
   const unsigned* src_flag;  // the source frame's non-finite flag (read only)
   unsigned* dst_flag;        // raised when an output sample is non-finite
+  int stream;                // Lanczos3: the frame's 8-frame group takes the streamed kernel
+  int pad;
 };
 struct BatchWarpJob {
   int height, width;

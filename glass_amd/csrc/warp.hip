@@ -21,6 +21,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <utility>
 
 #include "common.hpp"
@@ -746,6 +748,559 @@ __global__ __launch_bounds__(kTiledThreads) __attribute__((amdgpu_waves_per_eu(3
   lanczos_tile<false>(get, count, p, ox0, oy0, lut, tile);
 }
 
+// ------------------------------------------------------------------------
+// Streamed Lanczos3 for near-identity affine maps (registration drift: a
+// small rotation and a scale close to one, so |m1| is small and m4 is close
+// to 1).  Lane tx of wave ty owns output column ox0 + 64 (ty % kSColGroups) +
+// tx over kSLaneRows rows, walked in strips of kSStrip rows.  Within a strip
+// every output row moves the source row by exactly one (checked per strip),
+// so the lane keeps a register window of the source columns it needs and
+// reads one new window row per output row, from a window origin fixed for the
+// strip.  Fractions advance by the uniform per-row drifts (m1 for x, m4 - 1
+// for y, in table units) in f32 from the strip's first row, computed in FP64
+// exactly as the CPU path does; the footprint test is exact FP64 at the
+// strip's two end rows (the map is linear along the column, so the rows
+// between lie between them).  Where some lane's x fraction wraps within the
+// strip the window holds seven columns and each lane selects its six per row
+// from the column sums.  So a pixel costs its filter (36 + 6 FMAs, 24 for
+// the weights) and a few instructions of coordinates, with no FP64 and no
+// per-pixel address arithmetic.
+//
+// The window rows come from the tile's source box, staged into LDS by loads
+// that write LDS directly (no vector register, no vector instruction per
+// element); a frame whose box does not fit streams them from the frame with
+// buffer loads whose row offset is a scalar.  A strip where a lane straddles
+// the footprint edge (or lies within one pixel of it in x), where a source row
+// is skipped or repeated, or whose y table interval would drift out of
+// [-0.25, 1.25] of the strip's anchor runs row by row with FP64 coordinates.
+// Frames holding non-finite samples, and clamped resident warps, take the
+// per-row paths of the tiled kernel for every pixel.
+//
+// Compared with the rounded FP64 coordinates of the CPU path, the f32
+// fraction differs by about 1e-7 px; a floor decision can only differ where
+// the fraction is that close to an integer, where both tap windows give the
+// same value to the table's accuracy (the weights are continuous there and
+// the source is finite).
+#ifndef GH_WARP_STREAM_STRIP
+#define GH_WARP_STREAM_STRIP 8
+#endif
+#ifndef GH_WARP_STREAM_ROWS
+#define GH_WARP_STREAM_ROWS 16
+#endif
+#ifndef GH_WARP_STREAM_AHEAD
+#define GH_WARP_STREAM_AHEAD 1
+#endif
+#ifndef GH_WARP_STREAM_PIN
+#define GH_WARP_STREAM_PIN 1
+#endif
+constexpr int kSStrip = GH_WARP_STREAM_STRIP;
+constexpr int kSLaneRows = GH_WARP_STREAM_ROWS;
+constexpr int kSAhead = GH_WARP_STREAM_AHEAD;
+constexpr int kSWaves = 4;
+constexpr int kSThreads = 64 * kSWaves;
+constexpr int kSOutW = 128;
+constexpr int kSColGroups = kSOutW / 64;
+constexpr int kSOutH = kSLaneRows * kSWaves / kSColGroups;
+static_assert(kSLaneRows % kSStrip == 0 && kSLaneRows % kLaneRows == 0, "lane rows split into strips and groups");
+static_assert(kSStrip <= 16, "a strip's coverage counts fit one 64-bit word");
+static_assert(kSThreads == kTiledThreads, "the table loader strides by the block size");
+
+typedef __amdgpu_buffer_rsrc_t frame_rsrc_t;
+
+__device__ __forceinline__ float ld_rsrc(frame_rsrc_t r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
+// Per-frame uniform values of the streamed path.
+struct StreamFrame {
+  frame_rsrc_t src;
+  float* dst;
+  double m1, m2, m4, m5;
+  float dtx, dty;  // per-row drift of the x / y table coordinate
+  float thr;       // 7-column mode: lanes whose x table coordinate is >= thr use columns 1..6
+  int neg;         // m1 < 0: the 7-column window starts one column further left
+};
+
+// Staged source box of one output tile and frame (LDS, row stride
+// kBoxStride floats): rows by0 .. by0 + bh - 1, columns bx0 .. bx0 + bw - 1.
+#ifndef GH_WARP_BOX_ROWS
+#define GH_WARP_BOX_ROWS 48
+#endif
+#ifndef GH_WARP_BOX_AHEAD
+#define GH_WARP_BOX_AHEAD 1
+#endif
+#ifndef GH_WARP_LUT_AHEAD
+#define GH_WARP_LUT_AHEAD 1
+#endif
+#ifndef GH_WARP_BOX_STRIDE
+#define GH_WARP_BOX_STRIDE 144
+#endif
+constexpr int kBoxStride = GH_WARP_BOX_STRIDE;
+static_assert(kBoxStride > 128 && kBoxStride <= 192, "a box row is two 64-lane segments and a partial third");
+constexpr int kBoxRows = GH_WARP_BOX_ROWS;
+constexpr int kBoxFloats = kBoxStride * kBoxRows;
+static_assert(kBoxFloats <= kMaxTile, "the batched kernel stages boxes in the tiled kernel's buffer");
+static_assert(kSOutW == kOutW && kSOutH == kOutH, "both affine Lanczos3 kernels use the same output tiles");
+struct BoxGeom {
+  int bx0, by0, bw, bh;
+  bool ok;  // staged (finite source, no clamp, and the box fits)
+};
+
+// Window rows of a strip: loaded from the frame with buffer loads into
+// registers (lane byte offset voff, row j at soffset j * row_bytes), or read
+// from the staged LDS box (lane element offset base, row j at j * kBoxStride).
+struct RowsFromFrame {
+  frame_rsrc_t rs;
+  unsigned voff, row_bytes;
+  static constexpr int kAhead = kSAhead;
+  static constexpr bool kLutAhead = false;
+  __device__ __forceinline__ float at(int j, int c) const { return ld_rsrc(rs, voff + 4u * c, row_bytes * j); }
+  __device__ __forceinline__ void pin() { asm volatile("" : "+v"(voff)); }
+};
+struct RowsFromBox {
+  const float* box;
+  int base;
+  static constexpr int kAhead = GH_WARP_BOX_AHEAD;
+  static constexpr bool kLutAhead = GH_WARP_LUT_AHEAD;
+  // one address per row; its columns are immediate offsets
+  __device__ __forceinline__ float at(int j, int c) const {
+    int o = base + j * kBoxStride;
+    asm("" : "+v"(o));  // one base register per row; its columns are immediate offsets
+    return box[o + c];
+  }
+  __device__ __forceinline__ void pin() { asm volatile("" : "+v"(base)); }
+};
+
+// The 18 quadratic coefficients of interval i and the six weights at u.
+__device__ __forceinline__ void lut_eval(const float4* lut, int i, float u, float (&w)[6]) {
+  float c[18];
+  lut_coefficients(lut, i, c);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) w[k] = fmaf(u, fmaf(u, c[3 * k + 2], c[3 * k + 1]), c[3 * k]);
+}
+
+// One strip on the streamed path; NC = 6 (no lane's x fraction wraps) or 7.
+// Row r of the strip is output row y0 + r; window row j is source row
+// iy0 - 2 + j of the lane.  The column sums run down the window first, so a
+// 7-column strip gives the same values as a 6-column one at the same
+// fractions.
+template <int S, int NC, typename Rows>
+__device__ __forceinline__ bool stream_strip_rows(const StreamFrame& fr, const float4* lut, Rows src, float tx0,
+                                                  float uy0, const float (&cy)[18], bool live, bool present, int x,
+                                                  int y0, int W, float fill) {
+  constexpr int kAhead = Rows::kAhead;
+  float win[S + 5][NC];
+  auto load_row = [&](auto J) __attribute__((always_inline)) {
+    constexpr int j = decltype(J)::value;
+    static_for<NC>([&](auto C) __attribute__((always_inline)) {
+      constexpr int c = decltype(C)::value;
+      win[j][c] = src.at(j, c);
+    });
+  };
+  static_for<5 + kAhead>([&](auto J) __attribute__((always_inline)) { load_row(J); });
+  float nf = 0.0f;
+  float* const dst_row0 = fr.dst + static_cast<long long>(y0) * W;
+  // x table interval, position in it and column choice of row r
+  auto x_coord = [&](int r, int& ix, float& u, bool& sel) __attribute__((always_inline)) {
+    const float txr = fmaf(static_cast<float>(r), fr.dtx, tx0);
+    sel = false;
+    if constexpr (NC == 6) {
+      ix = static_cast<int>(txr);
+      u = __builtin_amdgcn_fractf(txr);
+    } else {
+      const float fl = floorf(txr);
+      ix = static_cast<int>(fl) & (kLanczosLutIntervals - 1);
+      u = txr - fl;
+      sel = txr >= fr.thr;
+    }
+  };
+  // From the box, the x coefficients of the next row are read one row ahead
+  // (registers allow it there).
+  constexpr bool kLutAhead = Rows::kLutAhead;
+  int ix_n = 0;
+  float u_n = 0.0f;
+  bool sel_n = false;
+  float cx_n[18];
+  if constexpr (kLutAhead) {
+    x_coord(0, ix_n, u_n, sel_n);
+    lut_coefficients(lut, ix_n, cx_n);
+  }
+  static_for<S>([&](auto R) __attribute__((always_inline)) {
+    constexpr int r = decltype(R)::value;
+#if GH_WARP_STREAM_PIN
+    // keeps the next row's loads in this step (hoisted, every window row of
+    // the strip would be live at once)
+    src.pin();
+#endif
+    if constexpr (r + 5 + kAhead < S + 5) load_row(std::integral_constant<int, r + 5 + kAhead>{});
+    float wx[6], wy[6];
+    float u;
+    bool sel;
+    if constexpr (kLutAhead) {
+      float cx[18];
+#pragma unroll
+      for (int k = 0; k < 18; ++k) cx[k] = cx_n[k];
+      u = u_n;
+      sel = sel_n;
+      if constexpr (r + 1 < S) {
+        x_coord(r + 1, ix_n, u_n, sel_n);
+        lut_coefficients(lut, ix_n, cx_n);
+      }
+#pragma unroll
+      for (int k = 0; k < 6; ++k) wx[k] = fmaf(u, fmaf(u, cx[3 * k + 2], cx[3 * k + 1]), cx[3 * k]);
+    } else {
+      int ix;
+      x_coord(r, ix, u, sel);
+      lut_eval(lut, ix, u, wx);
+    }
+    const float uy = fmaf(static_cast<float>(r), fr.dty, uy0);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) wy[k] = fmaf(uy, fmaf(uy, cy[3 * k + 2], cy[3 * k + 1]), cy[3 * k]);
+    float v[NC];
+    static_for<NC>([&](auto C) __attribute__((always_inline)) {
+      constexpr int c = decltype(C)::value;
+      float a = win[r][c] * wy[0];
+      static_for<5>([&](auto J) __attribute__((always_inline)) {
+        constexpr int j = decltype(J)::value + 1;
+        a = fmaf(win[r + j][c], wy[j], a);
+      });
+      v[c] = a;
+    });
+    float h[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      if constexpr (NC == 6) h[i] = v[i];
+      else h[i] = sel ? v[i + 1] : v[i];
+    }
+    float value = h[0] * wx[0];
+#pragma unroll
+    for (int i = 1; i < 6; ++i) value = fmaf(h[i], wx[i], value);
+    value = live ? value : fill;
+    // 0 * value is NaN exactly when value is not finite: a vector-only test
+    nf = fmaf(value, 0.0f, nf);
+    if (present) st_global_u32(dst_row0 + static_cast<long long>(r) * W, static_cast<unsigned>(x), value);
+  });
+  return present && !isfinite(nf);
+}
+
+// One strip row by row: FP64 coordinates as the CPU path, table weights, taps
+// read from the frame (finite source) or, BOX, from the staged box (which
+// holds every tap of the tile).  Row r's coverage count goes to bits 4 r of
+// cov.
+template <int S, bool BOX>
+__device__ __forceinline__ bool stream_strip_exact(const FrameRef& ref, const float4* lut, double px, double py,
+                                                   int x, int y0, int W, int H, float fill, unsigned long long& cov,
+                                                   const float* box, const BoxGeom& g) {
+  if (x >= W) return false;
+  const double xmax = static_cast<double>(W - 3), ymax = static_cast<double>(H - 3);
+  bool bad = false;
+#pragma unroll 1
+  for (int r = 0; r < S; ++r) {
+    const int y = y0 + r;
+    if (y >= H) break;
+    const double sx = (px + ref.inv[1] * static_cast<double>(y)) + ref.inv[2];
+    const double sy = (py + ref.inv[4] * static_cast<double>(y)) + ref.inv[5];
+    float value = fill;
+    if (sx >= 2.0 && sx < xmax && sy >= 2.0 && sy < ymax) {
+      const double fx0 = floor(sx), fy0 = floor(sy);
+      const int x0 = static_cast<int>(fx0), yy0 = static_cast<int>(fy0);
+      float wx[6], wy[6];
+      lut_weights(lut, static_cast<float>(sx - fx0), wx);
+      lut_weights(lut, static_cast<float>(sy - fy0), wy);
+      if constexpr (BOX) {
+        value = lanczos_taps(box + ((yy0 - 2 - g.by0) * kBoxStride + (x0 - 2 - g.bx0)), kBoxStride, wx, wy);
+      } else {
+        value = lanczos_taps((const global_cf32*)(ref.src + static_cast<long long>(yy0 - 2) * W + (x0 - 2)), W, wx,
+                             wy);
+      }
+      cov += 1ull << (4 * r);
+    }
+    st_global(ref.dst, static_cast<long long>(y) * W + x, value);
+    bad |= !isfinite(value);
+  }
+  return bad;
+}
+
+// One strip of a finite-source frame: decides (wave-wide) between the
+// streamed rows and the exact rows.  BOX: the window rows come from the
+// staged box g (each lane's window must lie inside it).
+template <int S, bool BOX>
+__device__ __forceinline__ bool stream_strip(const FrameRef& ref, const StreamFrame& fr, const float4* lut, double px,
+                                             double py, int x, int y0, int W, int H, float fill,
+                                             unsigned long long& cov, const float* box, const BoxGeom& g) {
+  const bool present = x < W;
+  const double yd0 = static_cast<double>(y0), yde = static_cast<double>(y0 + S - 1);
+  const double sx0 = (px + fr.m1 * yd0) + fr.m2, sy0 = (py + fr.m4 * yd0) + fr.m5;
+  const double sxe = (px + fr.m1 * yde) + fr.m2, sye = (py + fr.m4 * yde) + fr.m5;
+  const double lox = fmin(sx0, sxe), hix = fmax(sx0, sxe), loy = fmin(sy0, sye), hiy = fmax(sy0, sye);
+  // Exact footprint decisions need a margin far above the FP64 rounding of
+  // the coordinates (~1e-12 px); x keeps one more pixel so that the 7-column
+  // window stays inside the row.
+  constexpr double kEdge = 1.0e-7;
+  const double xmax = static_cast<double>(W - 3), ymax = static_cast<double>(H - 3);
+  const bool inside = lox >= 3.0 && hix < xmax - 1.0 && loy >= 2.0 + kEdge && hiy < ymax - kEdge;
+  const bool outside = hix < 2.0 - kEdge || lox >= xmax + kEdge || hiy < 2.0 - kEdge || loy >= ymax + kEdge;
+  // In the footprint s >= 2, so truncation is floor and fract(s) = s - floor(s)
+  // (the hardware conversion saturates coordinates far outside).
+  int ix0, iy0, iye;
+  asm("v_cvt_i32_f64 %0, %1" : "=v"(ix0) : "v"(sx0));
+  asm("v_cvt_i32_f64 %0, %1" : "=v"(iy0) : "v"(sy0));
+  asm("v_cvt_i32_f64 %0, %1" : "=v"(iye) : "v"(sye));
+  const float tx0 = static_cast<float>(__builtin_amdgcn_fract(sx0)) * static_cast<float>(kLanczosLutIntervals);
+  const float ty0 = static_cast<float>(__builtin_amdgcn_fract(sy0)) * static_cast<float>(kLanczosLutIntervals);
+  // y coefficients anchored on the interval at the strip's middle row.
+  const float tym = fmaf(0.5f * static_cast<float>(S - 1), fr.dty, ty0);
+  const int iyl = min(max(static_cast<int>(floorf(tym)), 0), kLanczosLutIntervals - 1);
+  const float uy0 = ty0 - static_cast<float>(iyl);
+  const float uye = fmaf(static_cast<float>(S - 1), fr.dty, uy0);
+  const float txe = fmaf(static_cast<float>(S - 1), fr.dtx, tx0);
+  const bool rows_ok = iye - iy0 == S - 1;
+  const bool anchor_ok = fminf(uy0, uye) >= -0.25f && fmaxf(uy0, uye) <= 1.25f;
+  // At most one wrap of the x fraction per strip (the 7-column window).
+  const bool wrap_ok = txe >= -static_cast<float>(kLanczosLutIntervals) &&
+                       txe < 2.0f * static_cast<float>(kLanczosLutIntervals);
+  // Box mode: the window's columns ix0 - 3 .. ix0 + 4 and rows iy0 - 2 ..
+  // iy0 + S + 2 lie inside the staged box.
+  bool in_box = true;
+  if constexpr (BOX) {
+    in_box = ix0 - 3 >= g.bx0 && ix0 + 4 < g.bx0 + g.bw && iy0 - 2 >= g.by0 && iy0 + S + 2 < g.by0 + g.bh;
+  }
+  const bool live = present && inside;
+  const bool lane_ok = !present || outside || (inside && rows_ok && anchor_ok && wrap_ok && in_box);
+  if (y0 + S <= H && __all(lane_ok)) {
+    float cy[18];
+    lut_coefficients(lut, iyl, cy);
+    const bool cross = live && (txe < 0.0f || txe >= static_cast<float>(kLanczosLutIntervals));
+    const bool seven = __any(cross);
+    const int cb = ix0 - 2 - (seven ? fr.neg : 0);
+    bool bad;
+    if constexpr (BOX) {
+      const RowsFromBox rows{box, live ? (iy0 - 2 - g.by0) * kBoxStride + (cb - g.bx0) : 0};
+      bad = seven ? stream_strip_rows<S, 7>(fr, lut, rows, tx0, uy0, cy, live, present, x, y0, W, fill)
+                  : stream_strip_rows<S, 6>(fr, lut, rows, tx0, uy0, cy, live, present, x, y0, W, fill);
+    } else {
+      const RowsFromFrame rows{fr.src, live ? static_cast<unsigned>(((iy0 - 2) * W + cb) * 4) : 0u,
+                               static_cast<unsigned>(W) * 4u};
+      bad = seven ? stream_strip_rows<S, 7>(fr, lut, rows, tx0, uy0, cy, live, present, x, y0, W, fill)
+                  : stream_strip_rows<S, 6>(fr, lut, rows, tx0, uy0, cy, live, present, x, y0, W, fill);
+    }
+    if (live) cov += 0x1111111111111111ull >> (64 - 4 * S);
+    return bad;
+  }
+  return stream_strip_exact<S, BOX>(ref, lut, px, py, x, y0, W, H, fill, cov, box, g);
+}
+
+// Source box of output tile [ox0, ox1] x [oy0, oy1] for frame `ref`: the
+// affine map takes its extremes at the corners; one pixel of margin beyond
+// the 7-column window and the strip rows covers the FP64 rounding.  Every
+// value is wave-uniform.
+__device__ __forceinline__ BoxGeom box_geometry(const FrameRef& ref, bool eligible, int ox0, int oy0, int ox1,
+                                                int oy1, int W, int H) {
+  BoxGeom g{0, 0, 0, 0, false};
+  if (!eligible) return g;
+  const double m0 = ref.inv[0], m1 = ref.inv[1], m2 = ref.inv[2], m3 = ref.inv[3], m4 = ref.inv[4], m5 = ref.inv[5];
+  const double x0 = ox0, x1 = ox1, y0 = oy0, y1 = oy1;
+  const double sxl = (m0 * (m0 >= 0.0 ? x0 : x1) + m1 * (m1 >= 0.0 ? y0 : y1)) + m2;
+  const double sxh = (m0 * (m0 >= 0.0 ? x1 : x0) + m1 * (m1 >= 0.0 ? y1 : y0)) + m2;
+  const double syl = (m3 * (m3 >= 0.0 ? x0 : x1) + m4 * (m4 >= 0.0 ? y0 : y1)) + m5;
+  const double syh = (m3 * (m3 >= 0.0 ? x1 : x0) + m4 * (m4 >= 0.0 ? y1 : y0)) + m5;
+  int a, b, c, d;
+  asm("v_cvt_i32_f64 %0, %1" : "=v"(a) : "v"(floor(sxl)));
+  asm("v_cvt_i32_f64 %0, %1" : "=v"(b) : "v"(floor(sxh)));
+  asm("v_cvt_i32_f64 %0, %1" : "=v"(c) : "v"(floor(syl)));
+  asm("v_cvt_i32_f64 %0, %1" : "=v"(d) : "v"(floor(syh)));
+  const int bx0 = max(a - 4, 0), bx1 = min(b + 5, W - 1), by0 = max(c - 3, 0), by1 = min(d + 4, H - 1);
+  g.bx0 = __builtin_amdgcn_readfirstlane(bx0);
+  g.by0 = __builtin_amdgcn_readfirstlane(by0);
+  g.bw = __builtin_amdgcn_readfirstlane(bx1 - bx0 + 1);
+  g.bh = __builtin_amdgcn_readfirstlane(by1 - by0 + 1);
+  g.ok = g.bw >= 1 && g.bh >= 1 && g.bw <= kBoxStride && g.bh <= kBoxRows;
+  return g;
+}
+
+// Buffer resource over `bytes` bytes from `base` (raw buffer: stride 0,
+// byte offsets), for the LDS loads below.
+typedef int rsrc_words_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ rsrc_words_t rsrc_words(const float* base, int bytes) {
+  const unsigned long long a = reinterpret_cast<unsigned long long>(base);
+  rsrc_words_t r;
+  r.x = __builtin_amdgcn_readfirstlane(static_cast<int>(a));
+  r.y = __builtin_amdgcn_readfirstlane(static_cast<int>(a >> 32));
+  r.z = bytes;
+  r.w = 0x00020000;
+  return r;
+}
+
+// One dword per lane from byte offset voff + soff of the buffer into LDS at
+// lds_byte + 4 * lane.  Written as inline assembly so that the compiler does
+// not order its own LDS reads behind these loads (it cannot tell the box
+// being read from the box being filled); nothing is returned in a register,
+// and the caller waits (s_waitcnt vmcnt(0) and a barrier) before reading the
+// data.  Waits the compiler places for its own loads stay correct: vector
+// memory operations of a wave complete in order.
+__device__ __forceinline__ void load_dword_to_lds(rsrc_words_t rs, unsigned lds_byte, unsigned voff, unsigned soff) {
+  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tbuffer_load_dword %1, %2, %3 offen lds"
+               :
+               : "s"(lds_byte), "v"(voff), "s"(rs), "s"(soff)
+               : "memory", "m0");
+}
+
+// Issues the box of frame `ref` into LDS (no wait here).  Wave w loads rows
+// w, w + 4, ...; each row is up to three segments of 64 lanes (the third 16
+// lanes wide).  Lanes past the box width load its last column again, so every
+// address lies inside the frame.
+__device__ __forceinline__ void stage_box_lds(const FrameRef& ref, const BoxGeom& g, int W, int bytes, float* box) {
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  const rsrc_words_t rs = rsrc_words(ref.src, bytes);
+  const unsigned v0 = static_cast<unsigned>(min(lane, g.bw - 1) * 4);
+  const unsigned v1 = static_cast<unsigned>(min(64 + lane, g.bw - 1) * 4);
+  const unsigned v2 = static_cast<unsigned>(min(128 + lane, g.bw - 1) * 4);
+  typedef __attribute__((address_space(3))) float lds_f32;
+  const unsigned box_byte = static_cast<unsigned>(reinterpret_cast<unsigned long long>((lds_f32*)box));
+#pragma unroll 1
+  for (int row = wave; row < g.bh; row += kSWaves) {
+    const unsigned soff = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(((g.by0 + row) * W + g.bx0) * 4));
+    const unsigned d = __builtin_amdgcn_readfirstlane(box_byte + static_cast<unsigned>(row * kBoxStride * 4));
+    load_dword_to_lds(rs, d, v0, soff);
+    if (g.bw > 64) load_dword_to_lds(rs, d + 256u, v1, soff);
+    if (g.bw > 128 && lane < kBoxStride - 128) load_dword_to_lds(rs, d + 512u, v2, soff);
+  }
+}
+
+// Coordinates and drifts of a finite-source frame on the streamed path.
+__device__ __forceinline__ StreamFrame stream_frame(const FrameRef& ref, int bytes) {
+  StreamFrame fr;
+  fr.src = __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(ref.src), 0, bytes, 0x00020000);
+  fr.dst = ref.dst;
+  fr.m1 = ref.inv[1];
+  fr.m2 = ref.inv[2];
+  fr.m4 = ref.inv[4];
+  fr.m5 = ref.inv[5];
+  fr.dtx = static_cast<float>(ref.inv[1]) * static_cast<float>(kLanczosLutIntervals);
+  fr.dty = static_cast<float>(ref.inv[4] - 1.0) * static_cast<float>(kLanczosLutIntervals);
+  fr.neg = ref.inv[1] < 0.0 ? 1 : 0;
+  fr.thr = fr.neg ? 0.0f : static_cast<float>(kLanczosLutIntervals);
+  return fr;
+}
+
+// The streamed Lanczos3 work of one output tile over `count` frames; every
+// thread of the block calls it.  A finite-source frame whose box fits (no
+// clamp) is staged into LDS by loads that write LDS directly; the block waits
+// for them (every wave's loads: s_waitcnt vmcnt(0) and a barrier) and reads
+// its window rows from the box.  The staging latency is hidden by the other
+// blocks on the CU, not inside the block: a second box would halve the
+// blocks per CU, which measured slower.  Other finite-source frames stream
+// their window rows from the frame; the rest take the per-row paths of the
+// tiled kernel.
+template <bool ATOMIC_COVERAGE, typename GetFrame>
+__device__ __forceinline__ void lanczos_stream_tile(const GetFrame& get, int count, const WarpParams& p, int ox0,
+                                                    int oy0, const float4* lut, float* box) {
+  const int tx = threadIdx.x & 63;
+  const int ty = threadIdx.x >> 6;
+  const int x = ox0 + 64 * (ty % kSColGroups) + tx;
+  // wave-uniform (row bases and store addresses stay scalar)
+  const int y_first = __builtin_amdgcn_readfirstlane(oy0 + kSLaneRows * (ty / kSColGroups));
+  const int W = p.width, H = p.height;
+  const int ox1 = min(ox0 + kSOutW, W) - 1, oy1 = min(oy0 + kSOutH, H) - 1;
+  const bool clamp_on = p.skip_nonfinite && p.clamp >= 0.0f;
+  constexpr int kGroups = kSLaneRows / kLaneRows;
+  constexpr int kStripsPerGroup = kLaneRows / kSStrip;
+  unsigned long long cov[kGroups];
+#pragma unroll
+  for (int g = 0; g < kGroups; ++g) cov[g] = 0ull;
+  const int bytes = W * H * 4;  // < 2^31 (checked by the host)
+  const double xd = static_cast<double>(x);
+
+  for (int f = 0; f < count; ++f) {
+    const FrameRef ref = get(f);
+    const bool finite = !clamp_on && ref.src_flag == 0u;
+    const BoxGeom geo = box_geometry(ref, finite, ox0, oy0, ox1, oy1, W, H);
+    bool bad = false;
+    if (geo.ok) {
+      __syncthreads();  // every wave is done with the previous frame's box
+      stage_box_lds(ref, geo, W, bytes, box);
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      __syncthreads();  // every wave's part of the box has landed
+    }
+    if (finite) {
+      const StreamFrame fr = stream_frame(ref, bytes);
+      const double px = ref.inv[0] * xd, py = ref.inv[3] * xd;
+#pragma unroll
+      for (int g = 0; g < kGroups; ++g) {
+#pragma unroll 1
+        for (int s = 0; s < kStripsPerGroup; ++s) {
+          unsigned long long c = 0ull;
+          const int y0 = y_first + g * kLaneRows + s * kSStrip;
+          if (geo.ok)
+            bad |= stream_strip<kSStrip, true>(ref, fr, lut, px, py, x, y0, W, H, p.fill, c, box, geo);
+          else
+            bad |= stream_strip<kSStrip, false>(ref, fr, lut, px, py, x, y0, W, H, p.fill, c, box, geo);
+          cov[g] += c << (4 * kSStrip * s);
+        }
+      }
+    } else {
+      // Non-finite source (CPU semantics: table rows, then the non-finite
+      // results recomputed tap by tap) or resident semantics with skipping /
+      // clamp (direct rows).
+      TileFrame tf{};
+      tf.src = ref.src;
+      tf.dst = ref.dst;
+      tf.m0 = ref.inv[0]; tf.m1 = ref.inv[1]; tf.m2 = ref.inv[2];
+      tf.m3 = ref.inv[3]; tf.m4 = ref.inv[4]; tf.m5 = ref.inv[5];
+      tf.fast = !clamp_on && !p.skip_nonfinite;
+      tf.recheck = true;
+#pragma unroll
+      for (int g = 0; g < kGroups; ++g) {
+        const int yg = y_first + g * kLaneRows;
+        bool b = tf.fast ? column_global_rows(tf, lut, x, yg, W, H, p.fill, cov[g])
+                         : column_direct_rows(tf, p, x, yg, cov[g]);
+        if (tf.fast && b) b = recheck_column(tf, x, yg, W, H);
+        bad |= b;
+      }
+    }
+    if (__any(bad) && tx == 0) atomicOr(ref.dst_flag, 1u);
+  }
+  if (p.coverage && x < W) {
+#pragma unroll
+    for (int g = 0; g < kGroups; ++g) {
+#pragma unroll 1
+      for (int k = 0; k < kLaneRows; ++k) {
+        const int y = y_first + g * kLaneRows + k;
+        if (y >= H) break;
+        const float add = static_cast<float>((cov[g] >> (4 * k)) & 15ull);
+        const long long o = static_cast<long long>(y) * W + x;
+        if constexpr (ATOMIC_COVERAGE) {
+          if (add != 0.0f) atomicAdd(p.coverage + o, add);
+        } else {
+          st_global(p.coverage, o, ld_global(p.coverage, o) + add);
+        }
+      }
+    }
+  }
+}
+
+#ifndef GH_WARP_STREAM_WAVES_PER_EU
+#define GH_WARP_STREAM_WAVES_PER_EU 3
+#endif
+__global__ __launch_bounds__(kSThreads) __attribute__((amdgpu_waves_per_eu(GH_WARP_STREAM_WAVES_PER_EU))) void
+warp_lanczos3_stream_kernel(WarpBatch b, int count, WarpParams p, int tiles_x) {
+  __shared__ float4 lut[kLutRows * kLutVec];
+  __shared__ float box[kBoxFloats];
+  load_lut(p.lanczos_lut, lut);
+  const unsigned nb = gridDim.x, bid = blockIdx.x;
+  const unsigned xcd = bid & 7u, q = nb >> 3, r = nb & 7u;
+  const unsigned logical = xcd * q + min(xcd, r) + (bid >> 3);
+  const int ox0 = static_cast<int>(logical % tiles_x) * kSOutW;
+  const int oy0 = static_cast<int>(logical / tiles_x) * kSOutH;
+  auto get = [&](int f) {
+    const double* m = b.inv[f];
+    return FrameRef{b.src[f], b.dst[f], {m[0], m[1], m[2], m[3], m[4], m[5]}, b.src_flags[f], b.nonfinite[f]};
+  };
+  __syncthreads();  // the table
+  lanczos_stream_tile<false>(get, count, p, ox0, oy0, lut, box);
+}
+
 // ------------------------------------------------- batched multi-job warp
 // Kernel over the work items of many jobs: item = (job, batch of up to
 // kWarpBatch frames, output tile).  Blocks stride over the item list (one
@@ -788,7 +1343,10 @@ __global__ __launch_bounds__(kTiledThreads) __attribute__((amdgpu_waves_per_eu(3
                       static_cast<unsigned>(__builtin_amdgcn_readfirstlane(*(const global_u32*)fr.src_flag)), fr.dst_flag};
     };
     if (job.interpolation == 3) {
-      lanczos_tile<true>(get, count, p, ox0, oy0, lut, tile);
+      // the same kernel choice as a per-job launch of this 8-frame group
+      const int streamed = __builtin_amdgcn_readfirstlane(*(const global_u32*)&frames[f0].stream);
+      if (streamed) lanczos_stream_tile<true>(get, count, p, ox0, oy0, lut, tile);
+      else lanczos_tile<true>(get, count, p, ox0, oy0, lut, tile);
     } else {
       bilinear_tile(get, count, p, ox0, oy0);
     }
@@ -1145,10 +1703,37 @@ void launch_snapshot_warp_flags(const WarpBatch& b, int count, unsigned* snap, h
   check(hipGetLastError(), "snapshot_warp_flags launch");
 }
 
+// Kernel choice for an affine Lanczos3 group of frames (a per-job launch
+// of up to kWarpBatch frames, or the same group inside a batched launch).
+// The streamed kernel needs 32-bit byte offsets into a frame and room for its
+// window; it pays off when every map is near the identity: |m1| small (the x
+// fraction wraps at most once per strip) and |m4 - 1| small enough that the y
+// table interval anchored at a strip's middle row stays within [-0.25, 1.25]
+// over the strip.  GLASS_HIP_WARP=tiled / stream forces a kernel (stream only
+// where its size conditions hold); the streamed kernel is correct for any
+// affine map.
+bool lanczos_stream_eligible(const double (*inv)[9], int count, int width, int height) {
+  const long long bytes = static_cast<long long>(width) * height * 4;
+  if (width < 64 || height < 64 || bytes >= (1ll << 31)) return false;
+  const char* force = std::getenv("GLASS_HIP_WARP");
+  if (force && std::strcmp(force, "tiled") == 0) return false;
+  if (force && std::strcmp(force, "stream") == 0) return true;
+  const double max_dy = 0.25 / (0.5 * (kSStrip - 1) * kLanczosLutIntervals);
+  for (int f = 0; f < count; ++f) {
+    if (!(std::fabs(inv[f][1]) <= 0.1 && std::fabs(inv[f][4] - 1.0) <= max_dy)) return false;
+  }
+  return true;
+}
+
 void launch_warp(const WarpBatch& batch, int count, const WarpParams& p, hipStream_t stream) {
   const dim3 grid((p.width + kTileX - 1) / kTileX, (p.height + kTileY - 1) / kTileY);
   const dim3 block(kTileX * kTileY);
-  if (p.interpolation == 3 && p.affine && p.lanczos_lut != nullptr) {
+  if (p.interpolation == 3 && p.affine && p.lanczos_lut != nullptr &&
+      lanczos_stream_eligible(batch.inv, count, p.width, p.height)) {
+    const int tiles_x = (p.width + kSOutW - 1) / kSOutW;
+    const int tiles_y = (p.height + kSOutH - 1) / kSOutH;
+    warp_lanczos3_stream_kernel<<<tiles_x * tiles_y, kSThreads, 0, stream>>>(batch, count, p, tiles_x);
+  } else if (p.interpolation == 3 && p.affine && p.lanczos_lut != nullptr) {
     const int tiles_x = (p.width + kOutW - 1) / kOutW;
     const int tiles_y = (p.height + kOutH - 1) / kOutH;
     warp_lanczos3_tiled_kernel<<<tiles_x * tiles_y, kTiledThreads, 0, stream>>>(batch, count, p, tiles_x);
