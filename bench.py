@@ -480,7 +480,7 @@ def batched_leg(args):
 
 # Measured HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, rocprofv3 --pmc, each
 # scaled by a known-size copy) of the default workload's kernels.
-_TRAFFIC_KERNELS = {"calibrate": "gh::calibrate_batch_kernel<2, false, 16>", "warp": "gh::warp_lanczos3_tiled_kernel",
+_TRAFFIC_KERNELS = {"calibrate": "gh::calibrate_batch_kernel<2, false, 16>", "warp": "gh::warp_lanczos3_stream_kernel",
                     "integrate": "gh::integrate_winsorized_kernel<200, true>"}
 # VALU issue ceiling without packed FP32: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz
 VALU_PEAK_LANE_INSTS = 256 * 4 * 16 * 2.4e9
