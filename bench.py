@@ -209,10 +209,24 @@ def refine_stack_vs_oracle():
     has = exp[1] > 0
     ref = exp[0][has].astype(np.float64)
     d = np.abs(c["master"][has].astype(np.float64) - ref)
+    # Weight map (kept samples after rejection) against the oracle's; a pixel
+    # may differ only where one of its samples lies within 1e-5 sigma of one
+    # of the oracle's rejection thresholds (the tolerance contract).
+    from oracle.rejection import center_and_scale
+
+    ok = (cov > 0) & np.isfinite(samples)
+    center, scale = center_and_scale(samples, ok, method="winsorized_sigma", low_sigma=3.0, high_sigma=3.0)
+    lo = (center - np.float32(3.0) * scale).astype(np.float64)
+    hi = (center + np.float32(3.0) * scale).astype(np.float64)
+    tol = 1e-5 * np.abs(scale.astype(np.float64))
+    xd = samples.astype(np.float64)
+    near = np.any(ok & ((np.abs(xd - lo) <= tol) | (np.abs(xd - hi) <= tol)), axis=0)
+    differ = c["weight"] != exp[1]
     return {"band": f"rows {y0}..{y0 + h - 1} x {w}", "pixels": int(has.sum()),
             "rms_rel": float(np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(ref * ref))),
             "p99_of_range": float(np.percentile(d, 99) / (ref.max() - ref.min())),
-            "weight_map_equal": bool(np.array_equal(c["weight"][has], exp[1][has]))}
+            "weight_map_mismatches": int(differ.sum()),
+            "weight_map_mismatches_off_threshold": int((differ & ~near).sum())}
 
 
 def refine_leg(args):

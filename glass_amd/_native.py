@@ -342,6 +342,7 @@ _SIGNATURES = {
     "gh_batch_create": [POINTER(BatchJob), c_int64, c_uint32, POINTER(c_void_p)],
     "gh_batch_run": [c_void_p, c_void_p],
     "gh_batch_info": [c_void_p, POINTER(c_int64)],
+    "gh_batch_set_schedule": [c_void_p, c_int64, c_int32, c_int32, c_int32],
     "gh_batch_destroy": [c_void_p],
     "gh_triangle_descriptors": [c_void_p, c_void_p, c_int64, c_int32, c_int32, c_int32, c_void_p, c_void_p,
                                 c_void_p, POINTER(TriangleInfo)],

@@ -15,12 +15,15 @@ dicts, each naming a loaded ``ResidentCalibratedStack`` and what to do with it:
      "dq_geometric": False, "dq_active_frames": 0}
 
 ``run(stream=None)`` executes every job: one batched warp launch over all
-jobs' (frame batch, tile) items, one launch deciding each job's statistics
+jobs' (frame group, tile) items, one launch deciding each job's statistics
 arithmetic from the warped frames' non-finite flags, and one integration launch
-per job reading that decision -- captured into a hipGraph on the first run
-(after the autotuner has timed candidate warp schedules) and replayed.
-Outputs equal the per-job ``warp_frames`` + ``integrate_to_device`` calls;
-the jobs' stacks are read, not modified.
+per sort-network capacity class over the pixels of all its jobs (a device job
+table) -- captured into a hipGraph on the first run (after the autotuner has
+timed candidate schedules: warp grid, item order and frames per item;
+integration capacity classes) and replayed.  ``set_schedule`` pins a
+schedule instead.  Outputs equal the per-job ``warp_frames`` +
+``integrate_to_device`` calls whatever the schedule; the jobs' stacks are
+read, not modified.
 """
 
 from __future__ import annotations
@@ -100,11 +103,20 @@ class StackingBatch:
         N.call("gh_batch_run", self._h, ctypes.c_void_p(stream or None))
 
     def info(self) -> dict:
-        out = (ctypes.c_int64 * 8)()
+        out = (ctypes.c_int64 * 10)()
         N.call("gh_batch_info", self._h, out)
         keys = ("jobs", "warp_items", "integrate_launches", "graph", "warp_grid", "warp_item_stride",
-                "device_variant_jobs", "autotune_trials")
+                "device_variant_jobs", "autotune_trials", "warp_frames_per_item", "integrate_classes")
         return {k: int(v) for k, v in zip(keys, out)}
+
+    def set_schedule(self, warp_grid: int, warp_item_stride: int = 1, frames_per_item: int = 8,
+                     integrate_classes: int = 0) -> None:
+        """Pin the schedule (no autotuning): warp workgroups, work-item order
+        stride (1 or coprime to the item count), frames per warp work item (2,
+        4 or 8) and integration capacity classes (0 smallest capacity per job,
+        1 powers of two, 2 one launch).  Results do not depend on it."""
+        N.call("gh_batch_set_schedule", self._h, int(warp_grid), int(warp_item_stride), int(frames_per_item),
+               int(integrate_classes))
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value:

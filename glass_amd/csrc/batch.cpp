@@ -1,12 +1,19 @@
 // Batched multi-job stacking (gh_batch_*): the warps of many resident jobs
-// run as one launch over (job, frame batch, output tile) work items; a small
+// run as one launch over (job, frame group, output tile) work items; a small
 // kernel then decides each job's statistics arithmetic from the frames'
-// non-finite flags, and each job integrates with the per-job winsorized
-// kernel reading that decision from device memory, so the whole sequence
-// needs no host round trip and can be captured into a hipGraph once and
-// replayed.  An autotuner times candidate warp schedules (grid size,
-// work-item order) on the first run and caches the choice per batch shape
-// for the process.
+// non-finite flags; the integrations run as one launch per sort-network
+// capacity class over the pixels of all jobs of the class (a job table read on
+// the device: block -> job, the job's arguments, its first block), each job
+// reading its statistics decision from device memory.  The whole sequence
+// needs no host round trip and is captured into a hipGraph once and replayed.
+// An autotuner times candidate schedules on the first run and caches the
+// choice per batch shape for the process: the warp's grid size, work-item
+// order and frames per work item, and the integration's capacity classes
+// (each job at its smallest capacity, powers of two, or one launch at the
+// largest capacity).  None of these choices changes a result: the warp
+// computes each pixel of a frame the same way whatever the item holding it,
+// and a sorting network of any capacity >= the sample count sorts the same
+// keys (the padding sorts after them).
 
 #include <chrono>
 #include <map>
@@ -24,7 +31,30 @@ namespace {
 struct Schedule {
   unsigned warp_grid = 0;
   int warp_stride = 1;
+  int frames_per_item = gh::kWarpBatch;  // 2, 4 or 8 frames of one job per warp work item
+  int classes = 0;                       // integration capacity classes (kClassSets)
 };
+
+// Frames per warp work item the schedule may use (divisors of kWarpBatch,
+// so an item never straddles two of the per-job path's 8-frame groups).
+constexpr int kItemFrames[] = {2, 4, 8};
+constexpr int kItemFrameOptions = 3;
+int item_frames_index(int f) { return f == 2 ? 0 : f == 4 ? 1 : 2; }
+
+// Capacity class sets of the integration: 0 each job at its smallest
+// register capacity, 1 powers of two, 2 every job at the largest capacity
+// any job needs (a single launch per weight kind: unit or per-frame).
+constexpr int kClassSets = 3;
+
+int class_capacity(int set, int n_active, int largest) {
+  if (set == 0) return gh::integrate_cap_for(n_active);
+  if (set == 1) {
+    int c = 16;
+    while (c < n_active) c <<= 1;
+    return c;
+  }
+  return largest;
+}
 
 std::mutex g_tune_mutex;
 std::map<std::string, Schedule> g_tune_cache;
@@ -41,6 +71,25 @@ int coprime_stride(long long n) {
 }  // namespace
 
 struct gh_batch {
+  // Warp work items for one frames-per-item choice.
+  struct WarpPlan {
+    DeviceBuffer<gh::BatchWarpJob> jobs;
+    DeviceBuffer<unsigned short> item_job;  // work item -> warp job
+    long long total_items = 0;
+  };
+  // One integration launch over the jobs of a capacity class.
+  struct ClassLaunch {
+    int cap = 0;
+    unsigned grid = 0;
+    gh::IntegrateArgs first{};  // kernel argument (the table overrides it)
+    DeviceBuffer<gh::IntegrateArgs> args;
+    DeviceBuffer<unsigned short> block_job;
+    DeviceBuffer<unsigned> first_block;
+  };
+  struct IntegratePlan {
+    std::vector<ClassLaunch> launches;
+    std::vector<size_t> per_job;  // jobs launched on their own (mean, more than 256 frames)
+  };
   struct Job {
     gh_stack* stack = nullptr;
     float* warped = nullptr;          // n_warp frames, owned
@@ -52,21 +101,21 @@ struct gh_batch {
     DeviceBuffer<const unsigned*> flag_ptrs;
   };
   std::vector<Job> jobs;
-  DeviceBuffer<gh::BatchWarpJob> warp_jobs;
+  WarpPlan wplans[kItemFrameOptions];
+  IntegratePlan iplans[kClassSets];
   DeviceBuffer<gh::BatchWarpFrame> warp_frames;
-  DeviceBuffer<unsigned short> item_job;   // warp work item -> warp job
   DeviceBuffer<gh::VariantJob> variant_jobs;
   DeviceBuffer<int> variants;              // per variant job, written on the device
   int n_variant_jobs = 0;
   DeviceBuffer<unsigned> warped_flags;
   DeviceBuffer<float> lut;
   int n_warp_jobs = 0;
-  long long total_items = 0;
   size_t total_warped = 0;
   uint32_t flags = 0;
   std::string signature;
   Schedule sched;
   bool tuned = false;
+  bool pinned = false;  // schedule set by gh_batch_set_schedule (no autotune)
   int trials = 0;
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -83,9 +132,18 @@ struct gh_batch {
       j.weights.release();
       j.flag_ptrs.release();
     }
-    warp_jobs.release();
+    for (WarpPlan& w : wplans) {
+      w.jobs.release();
+      w.item_job.release();
+    }
+    for (IntegratePlan& ip : iplans) {
+      for (ClassLaunch& l : ip.launches) {
+        l.args.release();
+        l.block_job.release();
+        l.first_block.release();
+      }
+    }
     warp_frames.release();
-    item_job.release();
     variant_jobs.release();
     variants.release();
     warped_flags.release();
@@ -102,12 +160,32 @@ struct gh_batch {
       if (!j.coverage) continue;
       gh::launch_fill(j.coverage, j.stack->npix, static_cast<float>(j.n_unwarped), st);
     }
-    if (total_items) {
-      gh::launch_warp_batched(warp_jobs.ptr, item_job.ptr, warp_frames.ptr, total_items,
+    warp(st, sc);
+    gh::launch_decide_variants(variant_jobs.ptr, n_variant_jobs, variants.ptr, st);
+    integrate(st, sc);
+  }
+
+  void warp(hipStream_t st, const Schedule& sc) {
+    const WarpPlan& w = wplans[item_frames_index(sc.frames_per_item)];
+    if (w.total_items) {
+      gh::launch_warp_batched(w.jobs.ptr, w.item_job.ptr, warp_frames.ptr, w.total_items,
                               reinterpret_cast<const float4*>(lut.ptr), sc.warp_grid, sc.warp_stride, st);
     }
-    gh::launch_decide_variants(variant_jobs.ptr, n_variant_jobs, variants.ptr, st);
-    for (Job& j : jobs) gh::launch_integrate(j.args, st);
+  }
+
+  void integrate(hipStream_t st, const Schedule& sc) {
+    const IntegratePlan& ip = iplans[sc.classes];
+    for (const ClassLaunch& l : ip.launches) {
+      gh::launch_integrate_cap(l.cap, l.first, gh::IntegrateJobTable{l.args.ptr, l.block_job.ptr, l.first_block.ptr},
+                               l.grid, st);
+    }
+    for (size_t j : ip.per_job) gh::launch_integrate(jobs[j].args, st);
+  }
+
+  long long total_items() const { return wplans[item_frames_index(sched.frames_per_item)].total_items; }
+  int integrate_launches() const {
+    const IntegratePlan& ip = iplans[sched.classes];
+    return static_cast<int>(ip.launches.size() + ip.per_job.size());
   }
 
   // Device time of `launch` on the batch's stream (best of two runs).
@@ -132,33 +210,57 @@ struct gh_batch {
     return best;
   }
 
-  // The warp launch is tuned on its own timing over its grid (persistent
-  // multiples of the resident-block count, or one workgroup per item) and
-  // item order.  Tuning runs write the batch's own buffers; every real run
-  // starts by clearing them.
+  // The warp launch is tuned on its own timing: first the frames per work
+  // item at the default grid, then the grid (persistent multiples of the
+  // resident-block count, or one workgroup per item) and the item order.  The
+  // integration is tuned over its capacity class sets.  Tuning runs write the
+  // batch's own buffers; every real run starts by clearing them.
   void autotune() {
     enqueue(stream, sched);  // warm-up
     check(hipStreamSynchronize(stream), "synchronize(batch)");
-    if (total_items) {
+    if (wplans[item_frames_index(gh::kWarpBatch)].total_items) {
       const unsigned base_w = gh::warp_batched_max_blocks();
-      auto warp_time = [&](unsigned grid, int stride) {
-        return time_launch([&] {
-          gh::launch_warp_batched(warp_jobs.ptr, item_job.ptr, warp_frames.ptr, total_items,
-                                  reinterpret_cast<const float4*>(lut.ptr), grid, stride, stream);
-        });
-      };
-      float best = warp_time(sched.warp_grid, sched.warp_stride);
-      const unsigned all_items = static_cast<unsigned>(std::min<long long>(total_items, 0x7fffffffLL));
-      for (unsigned grid : {base_w, base_w * 2u, base_w * 4u, base_w * 8u, all_items}) {
-        for (int stride : {1, coprime_stride(total_items)}) {
-          if (grid == sched.warp_grid && stride == sched.warp_stride) continue;
-          const float ms = warp_time(grid, stride);
+      auto warp_time = [&](const Schedule& sc) { return time_launch([&] { warp(stream, sc); }); };
+      float best = warp_time(sched);
+      for (int fpi : kItemFrames) {
+        if (fpi == sched.frames_per_item) continue;
+        Schedule sc = sched;
+        sc.frames_per_item = fpi;
+        const float ms = warp_time(sc);
+        if (ms < best) {
+          best = ms;
+          sched = sc;
+        }
+      }
+      const long long items = total_items();
+      const unsigned all_items = static_cast<unsigned>(std::min<long long>(items, 0x7fffffffLL));
+      const Schedule base = sched;
+      for (unsigned grid : {base_w, base_w * 4u, all_items}) {
+        for (int stride : {1, coprime_stride(items)}) {
+          Schedule sc = base;
+          sc.warp_grid = grid;
+          sc.warp_stride = stride;
+          if (sc.warp_grid == base.warp_grid && sc.warp_stride == base.warp_stride) continue;
+          const float ms = warp_time(sc);
           if (ms < best) {
             best = ms;
-            sched.warp_grid = grid;
-            sched.warp_stride = stride;
+            sched = sc;
           }
         }
+      }
+    }
+    warp(stream, sched);  // the integration candidates read these warped frames
+    gh::launch_decide_variants(variant_jobs.ptr, n_variant_jobs, variants.ptr, stream);
+    float best_i = time_launch([&] { integrate(stream, sched); });
+    const Schedule base = sched;
+    for (int set = 0; set < kClassSets; ++set) {
+      if (set == base.classes) continue;
+      Schedule sc = base;
+      sc.classes = set;
+      const float ms = time_launch([&] { integrate(stream, sc); });
+      if (ms < best_i) {
+        best_i = ms;
+        sched = sc;
       }
     }
   }
@@ -344,33 +446,38 @@ int gh_batch_create(const gh_batch_job* jobs, int64_t n_jobs, uint32_t flags, gh
       a.dq_active_frames = o.dq_active_frames;
       sig << s->height << 'x' << s->width << ':' << in.n_warp << ':' << in.interpolation << ':' << n_active << ';';
     }
-    // Device job tables.
-    long long items = 0;
-    for (gh::BatchWarpJob& wj : wjobs) {
-      wj.first_item = items;
-      items += gh::warp_batched_items(wj);
-    }
-    b->total_items = items;
+    // Warp work items, one table per frames-per-item choice.
     b->n_warp_jobs = static_cast<int>(wjobs.size());
     if (wjobs.size() > 65535) throw gh::ValueError("a batch holds at most 65535 warped jobs");
     if (!wjobs.empty()) {
-      // Work item -> warp job, so a workgroup finds its job with one load.
-      std::vector<unsigned short> item_job(static_cast<size_t>(items));
-      for (size_t k = 0; k < wjobs.size(); ++k) {
-        const long long end = k + 1 < wjobs.size() ? wjobs[k + 1].first_item : items;
-        std::fill(item_job.begin() + wjobs[k].first_item, item_job.begin() + end, static_cast<unsigned short>(k));
-      }
-      b->item_job.ensure(item_job.size(), "hipMalloc(batch item table)");
-      check(hipMemcpy(b->item_job.ptr, item_job.data(), item_job.size() * sizeof(unsigned short),
-                      hipMemcpyHostToDevice),
-            "hipMemcpy(batch item table)");
-      b->warp_jobs.ensure(wjobs.size(), "hipMalloc(batch warp jobs)");
       b->warp_frames.ensure(wframes.size(), "hipMalloc(batch warp frames)");
-      check(hipMemcpy(b->warp_jobs.ptr, wjobs.data(), wjobs.size() * sizeof(gh::BatchWarpJob), hipMemcpyHostToDevice),
-            "hipMemcpy(batch warp jobs)");
       check(hipMemcpy(b->warp_frames.ptr, wframes.data(), wframes.size() * sizeof(gh::BatchWarpFrame),
                       hipMemcpyHostToDevice),
             "hipMemcpy(batch warp frames)");
+      for (int o = 0; o < kItemFrameOptions; ++o) {
+        std::vector<gh::BatchWarpJob> pj = wjobs;
+        long long items = 0;
+        for (gh::BatchWarpJob& wj : pj) {
+          wj.frames_per_item = kItemFrames[o];
+          wj.first_item = items;
+          items += gh::warp_batched_items(wj);
+        }
+        // Work item -> warp job, so a workgroup finds its job with one load.
+        std::vector<unsigned short> item_job(static_cast<size_t>(items));
+        for (size_t k = 0; k < pj.size(); ++k) {
+          const long long end = k + 1 < pj.size() ? pj[k + 1].first_item : items;
+          std::fill(item_job.begin() + pj[k].first_item, item_job.begin() + end, static_cast<unsigned short>(k));
+        }
+        gh_batch::WarpPlan& w = b->wplans[o];
+        w.total_items = items;
+        w.item_job.ensure(item_job.size(), "hipMalloc(batch item table)");
+        check(hipMemcpy(w.item_job.ptr, item_job.data(), item_job.size() * sizeof(unsigned short),
+                        hipMemcpyHostToDevice),
+              "hipMemcpy(batch item table)");
+        w.jobs.ensure(pj.size(), "hipMalloc(batch warp jobs)");
+        check(hipMemcpy(w.jobs.ptr, pj.data(), pj.size() * sizeof(gh::BatchWarpJob), hipMemcpyHostToDevice),
+              "hipMemcpy(batch warp jobs)");
+      }
     }
     b->n_variant_jobs = static_cast<int>(vjobs.size());
     if (!vjobs.empty()) {
@@ -380,6 +487,56 @@ int gh_batch_create(const gh_batch_job* jobs, int64_t n_jobs, uint32_t flags, gh
                       hipMemcpyHostToDevice),
             "hipMemcpy(batch variant jobs)");
       for (size_t k = 0; k < vjob_of.size(); ++k) b->jobs[vjob_of[k]].args.variant_ptr = b->variants.ptr + k;
+    }
+    // Integration launches per capacity class set: winsorized jobs of at most
+    // 256 positive-weight frames share register-kernel launches through job
+    // tables; rejection-free and larger jobs launch on their own.
+    int largest = 0;
+    for (const gh_batch::Job& job : b->jobs) {
+      const gh::IntegrateArgs& a = job.args;
+      if (a.rejection != 0 && a.n_active <= gh::integrate_register_capacity())
+        largest = std::max(largest, gh::integrate_cap_for(a.n_active));
+    }
+    for (int set = 0; set < kClassSets; ++set) {
+      gh_batch::IntegratePlan& ip = b->iplans[set];
+      // (capacity, unit weights): the kernel variant of a launch
+      std::map<std::pair<int, int>, std::vector<size_t>> classes;
+      for (size_t j = 0; j < b->jobs.size(); ++j) {
+        const gh::IntegrateArgs& a = b->jobs[j].args;
+        if (a.rejection != 0 && a.n_active <= gh::integrate_register_capacity())
+          classes[{class_capacity(set, a.n_active, largest), a.unit_weights}].push_back(j);
+        else
+          ip.per_job.push_back(j);
+      }
+      for (const auto& [key, members] : classes) {
+        gh_batch::ClassLaunch l;
+        l.cap = key.first;
+        std::vector<gh::IntegrateArgs> args;
+        std::vector<unsigned> first;
+        std::vector<unsigned short> block_job;
+        for (size_t k = 0; k < members.size(); ++k) {
+          const gh::IntegrateArgs& a = b->jobs[members[k]].args;
+          args.push_back(a);
+          first.push_back(static_cast<unsigned>(block_job.size()));
+          const long long blocks = (a.npix + gh::kBlock - 1) / gh::kBlock;
+          if (static_cast<long long>(block_job.size()) + blocks > 0x7fffffffLL)
+            throw gh::ValueError("a batch integration class covers at most 2^31 - 1 blocks of pixels");
+          block_job.insert(block_job.end(), static_cast<size_t>(blocks), static_cast<unsigned short>(k));
+        }
+        l.first = args.front();
+        l.grid = static_cast<unsigned>(block_job.size());
+        l.args.ensure(args.size(), "hipMalloc(batch integration jobs)");
+        l.first_block.ensure(first.size(), "hipMalloc(batch integration job blocks)");
+        l.block_job.ensure(block_job.size(), "hipMalloc(batch integration block table)");
+        check(hipMemcpy(l.args.ptr, args.data(), args.size() * sizeof(gh::IntegrateArgs), hipMemcpyHostToDevice),
+              "hipMemcpy(batch integration jobs)");
+        check(hipMemcpy(l.first_block.ptr, first.data(), first.size() * sizeof(unsigned), hipMemcpyHostToDevice),
+              "hipMemcpy(batch integration job blocks)");
+        check(hipMemcpy(l.block_job.ptr, block_job.data(), block_job.size() * sizeof(unsigned short),
+                        hipMemcpyHostToDevice),
+              "hipMemcpy(batch integration block table)");
+        ip.launches.push_back(l);
+      }
     }
     b->sched.warp_grid = gh::warp_batched_max_blocks();
     b->sched.warp_stride = 1;
@@ -396,7 +553,7 @@ int gh_batch_run(gh_batch* b, void* stream_ptr) {
       // Work queued on the caller's stream before this call must be complete
       // before the batch's own stream runs tuning or capture.
       if (st != b->stream) check(hipStreamSynchronize(st), "synchronize(caller stream)");
-      if (b->flags & GH_BATCH_AUTOTUNE) {
+      if ((b->flags & GH_BATCH_AUTOTUNE) && !b->pinned) {
         bool cached = false;
         {
           std::lock_guard<std::mutex> lock(g_tune_mutex);
@@ -442,13 +599,42 @@ int gh_batch_info(gh_batch* b, int64_t* out) {
   return guarded([&] {
     if (!b || !out) throw gh::ValueError("argument is NULL");
     out[0] = static_cast<int64_t>(b->jobs.size());
-    out[1] = b->total_items;
-    out[2] = static_cast<int64_t>(b->jobs.size());  // one integration launch per job
+    out[1] = b->total_items();
+    out[2] = b->integrate_launches();
     out[3] = b->exec ? 1 : 0;
     out[4] = b->sched.warp_grid;
     out[5] = b->sched.warp_stride;
     out[6] = b->n_variant_jobs;
     out[7] = b->trials;
+    out[8] = b->sched.frames_per_item;
+    out[9] = b->sched.classes;
+  });
+}
+
+int gh_batch_set_schedule(gh_batch* b, int64_t warp_grid, int32_t warp_item_stride, int32_t frames_per_item,
+                          int32_t integrate_classes) {
+  return guarded([&] {
+    if (!b) throw gh::ValueError("batch is NULL");
+    if (frames_per_item != 2 && frames_per_item != 4 && frames_per_item != 8)
+      throw gh::ValueError("frames_per_item must be 2, 4 or 8");
+    if (integrate_classes < 0 || integrate_classes >= kClassSets)
+      throw gh::ValueError("integrate_classes must be 0, 1 or 2");
+    if (warp_grid < 1 || warp_grid > 0x7fffffffLL) throw gh::ValueError("warp_grid must be in [1, 2^31 - 1]");
+    if (warp_item_stride < 1) throw gh::ValueError("warp_item_stride must be positive");
+    const long long items = b->wplans[item_frames_index(frames_per_item)].total_items;
+    if (warp_item_stride > 1 && items > 0 && std::gcd(static_cast<long long>(warp_item_stride), items) != 1)
+      throw gh::ValueError("warp_item_stride must be coprime to the number of warp work items");
+    check(hipStreamSynchronize(b->stream), "synchronize(batch)");
+    if (b->exec) check(hipGraphExecDestroy(b->exec), "hipGraphExecDestroy");
+    if (b->graph) check(hipGraphDestroy(b->graph), "hipGraphDestroy");
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->sched.warp_grid = static_cast<unsigned>(warp_grid);
+    b->sched.warp_stride = warp_item_stride;
+    b->sched.frames_per_item = frames_per_item;
+    b->sched.classes = integrate_classes;
+    b->pinned = true;
+    b->tuned = false;  // the next run captures the graph with this schedule
   });
 }
 

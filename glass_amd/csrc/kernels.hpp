@@ -137,10 +137,12 @@ struct BatchWarpJob {
   float* coverage;           // nullable, accumulated atomically
   long long first_item;      // prefix of the items of earlier jobs
   int first_frame, n_frames; // range in the frame array
+  int frames_per_item;       // frames of one work item (a divisor of kWarpBatch)
+  int pad;
 };
 // Output tile width / height of the tiled warp kernels.
 void warp_tile_shape(int* tile_w, int* tile_h);
-long long warp_batched_items(const BatchWarpJob& job);  // ceil(frames / kWarpBatch) * tiles
+long long warp_batched_items(const BatchWarpJob& job);  // ceil(frames / frames_per_item) * tiles
 unsigned warp_batched_max_blocks();                     // resident blocks that fill the device
 // item_job[i]: the job of work item i (jobs are in first_item order).
 void launch_warp_batched(const BatchWarpJob* jobs, const unsigned short* item_job, const BatchWarpFrame* frames,
@@ -270,6 +272,32 @@ void launch_apply_grid(const LnBatch& b, int n, const float* scales, const float
 int integrate_max_frames();         // positive-weight frames the winsorized path accepts
 int integrate_register_capacity();  // frames the register kernels hold; frame tables pad to it
 void launch_integrate(const IntegrateArgs& a, hipStream_t stream);
+
+// Job table of one register-kernel launch over the pixels of many jobs: block
+// b belongs to job block_job[b] (arguments jobs[job]) and covers its pixels
+// [256 (b - first_block[job]), +256).  All null: one job, the kernel argument.
+struct IntegrateJobTable {
+  const IntegrateArgs* jobs;
+  const unsigned short* block_job;
+  const unsigned* first_block;
+};
+// Capacities of the register-sort winsorized kernels (samples held in VGPRs);
+// a job takes the smallest one holding its positive-weight frames.
+constexpr int kIntegrateCaps[] = {16, 24, 32, 48, 64, 80, 96, 112, 128, 160, 200, 256};
+constexpr int kIntegrateCapCount = sizeof(kIntegrateCaps) / sizeof(kIntegrateCaps[0]);
+int integrate_cap_for(int n_active);  // smallest capacity >= n_active (n_active <= 256)
+// The register kernel of capacity `cap` (one of kIntegrateCaps) over `grid`
+// blocks of 256 pixels, for one job (empty table) or the jobs of the table.
+void launch_integrate_cap(int cap, const IntegrateArgs& a, const IntegrateJobTable& t, unsigned grid,
+                          hipStream_t stream);
+// Per capacity set (one translation unit each, compiled in parallel): false
+// when `cap` is not in the set.
+bool launch_integrate_caps_small(int cap, const IntegrateArgs& a, const IntegrateJobTable& t, unsigned grid,
+                                 hipStream_t stream);
+bool launch_integrate_caps_mid(int cap, const IntegrateArgs& a, const IntegrateJobTable& t, unsigned grid,
+                               hipStream_t stream);
+bool launch_integrate_caps_large(int cap, const IntegrateArgs& a, const IntegrateJobTable& t, unsigned grid,
+                                 hipStream_t stream);
 
 // Per-tile "any non-finite sample" flags over n_active frames (for the
 // CPUStackEngine per-tile statistics choice); flags[t] = 1 if all finite.

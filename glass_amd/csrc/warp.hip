@@ -1325,8 +1325,8 @@ __global__ __launch_bounds__(kTiledThreads) __attribute__((amdgpu_waves_per_eu(3
     }
     const long long local = item - job.first_item;
     const int batch = static_cast<int>(local / job.tiles), t = static_cast<int>(local % job.tiles);
-    const int f0 = job.first_frame + batch * kWarpBatch;
-    const int count = min(kWarpBatch, job.first_frame + job.n_frames - f0);
+    const int f0 = job.first_frame + batch * job.frames_per_item;
+    const int count = min(job.frames_per_item, job.first_frame + job.n_frames - f0);
     const int ox0 = (t % job.tiles_x) * kOutW, oy0 = (t / job.tiles_x) * kOutH;
     WarpParams p{};
     p.height = job.height;
@@ -1677,7 +1677,7 @@ void warp_tile_shape(int* tile_w, int* tile_h) {
 }
 
 long long warp_batched_items(const BatchWarpJob& job) {
-  return static_cast<long long>((job.n_frames + kWarpBatch - 1) / kWarpBatch) * job.tiles;
+  return static_cast<long long>((job.n_frames + job.frames_per_item - 1) / job.frames_per_item) * job.tiles;
 }
 
 unsigned warp_batched_max_blocks() {

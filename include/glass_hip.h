@@ -648,13 +648,16 @@ int gh_refine_affine(gh_stack* stack, int64_t reference_index, int64_t moving_in
  * No counterpart in the reference (which runs one resident job per call):
  * many stacking jobs -- each a loaded gh_stack, the frames to warp with their
  * FP64 inverses, frame weights and integration parameters/outputs -- run as
- * one batched warp launch over all jobs' (frame batch, tile) items, one
+ * one batched warp launch over all jobs' (frame group, tile) items, one
  * launch that decides each job's statistics arithmetic on the device from the
- * warped frames' non-finite flags, and one integration launch per job that
- * reads that decision, optionally captured once into a hipGraph and replayed.  Results equal the per-job
- * calls (gh_warp_frames + gh_integrate with statistics_variant -1, tile_size 0).
- * The batch owns the warped copies of the listed frames; the stacks' frames
- * are read, not modified. */
+ * warped frames' non-finite flags, and one integration launch per
+ * sort-network capacity class over the pixels of all jobs of the class (a
+ * device job table; rejection-free jobs and jobs of more than 256
+ * positive-weight frames launch on their own), optionally captured once into
+ * a hipGraph and replayed.  Results equal the per-job calls (gh_warp_frames +
+ * gh_integrate with statistics_variant -1, tile_size 0) whatever the
+ * schedule.  The batch owns the warped copies of the listed frames; the
+ * stacks' frames are read, not modified. */
 typedef struct gh_batch gh_batch;
 
 typedef struct gh_batch_job {
@@ -673,16 +676,25 @@ typedef struct gh_batch_job {
 } gh_batch_job;
 
 #define GH_BATCH_USE_GRAPH 1u  /* capture the launch sequence once, replay it */
-#define GH_BATCH_AUTOTUNE 2u   /* time candidate schedules on the first run */
+#define GH_BATCH_AUTOTUNE 2u   /* time candidate schedules on the first run (warp grid, item
+                                  order and frames per item; integration capacity classes) */
 
 int gh_batch_create(const gh_batch_job* jobs, int64_t n_jobs, uint32_t flags, gh_batch** out);
 /* Enqueue one execution on `stream`, asynchronously; NULL runs it on the
    batch's own stream and returns when it has finished. */
 int gh_batch_run(gh_batch* batch, void* stream);
-/* out[8]: jobs, warp work items, integration launches, graph captured (0/1),
+/* out[10]: jobs, warp work items, integration launches, graph captured (0/1),
  * warp grid, warp item stride, jobs whose statistics arithmetic is decided
- * on the device, autotune trials run. */
+ * on the device, autotune trials run, frames per warp work item, integration
+ * capacity class set (0 each job at its smallest register capacity, 1 powers
+ * of two, 2 one launch at the largest). */
 int gh_batch_info(gh_batch* batch, int64_t* out);
+/* Fixes the schedule (no autotuning; the next run captures the graph anew):
+ * warp_grid workgroups in [1, 2^31 - 1], warp_item_stride 1 or coprime to
+ * the work-item count, frames_per_item 2, 4 or 8, integrate_classes 0, 1 or
+ * 2 (as in gh_batch_info).  Results do not depend on it. */
+int gh_batch_set_schedule(gh_batch* batch, int64_t warp_grid, int32_t warp_item_stride, int32_t frames_per_item,
+                          int32_t integrate_classes);
 int gh_batch_destroy(gh_batch* batch);
 
 #ifdef __cplusplus

@@ -40,6 +40,7 @@ SPECS = [
     (128, 160, 17, "bilinear", "cpu", 0.0, "zero", True),
     (33, 47, 9, "lanczos3", "resident", math.nan, None, True),
     (200, 130, 40, "lanczos3", "cpu", 0.0, "varied", False),
+    (80, 100, 70, "lanczos3", "cpu", 0.0, "zero", False),
 ]
 
 
@@ -120,3 +121,47 @@ def test_graph_replays_match_eager(gpu):
                     assert np.array_equal(g[k], e[k], equal_nan=True), (rep, k)
     for j in jobs:
         j["stack"].close()
+
+
+def _coprime_stride(n):
+    for p in (97, 89, 83, 79, 73, 71):
+        if math.gcd(p, n) == 1 and p < n:
+            return p
+    return 1
+
+
+@pytest.mark.parametrize("frames_per_item,classes,interleave", [(2, 0, False), (4, 1, True), (8, 2, True),
+                                                                (2, 2, False)])
+def test_pinned_schedules_match_per_job_path(gpu, frames_per_item, classes, interleave):
+    # Every schedule the autotuner may pick gives the per-job maps bit for bit:
+    # frames per warp work item, item order, and the integration's capacity
+    # classes (each job at its smallest capacity, powers of two, one launch).
+    jobs = _make_jobs(3)
+    with StackingBatch(jobs, use_graph=True, autotune=True) as b:
+        b.set_schedule(512, 1, frames_per_item, classes)
+        if interleave:
+            b.set_schedule(512, _coprime_stride(b.info()["warp_items"]), frames_per_item, classes)
+        b.run()
+        got = [_host(j["outputs"]) for j in jobs]
+        info = b.info()
+    assert info["warp_frames_per_item"] == frames_per_item and info["integrate_classes"] == classes
+    assert info["autotune_trials"] == 0 and info["graph"] == 1
+    # one register launch per class and weight kind: 5, 11, 16, 9 and 69
+    # positive-weight frames with unit weights and 40 with varied ones make
+    # classes 16 / 48 / 80 at the smallest capacities, 16 / 64 / 128 at
+    # powers of two, or 80 for both weight kinds in the single-capacity set
+    assert info["integrate_launches"] == {0: 3, 1: 3, 2: 2}[classes]
+    for job, g in zip(jobs, got):
+        exp = _host(_serial(job))
+        for k in KEYS:
+            assert np.array_equal(g[k], exp[k], equal_nan=True), (k, job["interpolation"], frames_per_item, classes)
+        job["stack"].close()
+
+
+def test_schedule_arguments_are_checked(gpu):
+    jobs = _make_jobs(4)[:1]
+    with StackingBatch(jobs, use_graph=False, autotune=False) as b:
+        for bad in [(512, 1, 3, 0), (512, 1, 8, 3), (0, 1, 8, 0), (512, 0, 8, 0)]:
+            with pytest.raises(ValueError):
+                b.set_schedule(*bad)
+    jobs[0]["stack"].close()
