@@ -101,7 +101,9 @@ struct gh_batch {
     DeviceBuffer<const unsigned*> flag_ptrs;
   };
   std::vector<Job> jobs;
-  WarpPlan wplans[kItemFrameOptions];
+  // [0]: the Lanczos3 jobs' items, [1]: the bilinear jobs' items (a launch
+  // each), per frames-per-item choice
+  WarpPlan wplans[2][kItemFrameOptions];
   IntegratePlan iplans[kClassSets];
   DeviceBuffer<gh::BatchWarpFrame> warp_frames;
   DeviceBuffer<gh::VariantJob> variant_jobs;
@@ -132,9 +134,11 @@ struct gh_batch {
       j.weights.release();
       j.flag_ptrs.release();
     }
-    for (WarpPlan& w : wplans) {
-      w.jobs.release();
-      w.item_job.release();
+    for (auto& kind : wplans) {
+      for (WarpPlan& w : kind) {
+        w.jobs.release();
+        w.item_job.release();
+      }
     }
     for (IntegratePlan& ip : iplans) {
       for (ClassLaunch& l : ip.launches) {
@@ -165,11 +169,19 @@ struct gh_batch {
     integrate(st, sc);
   }
 
+  // The Lanczos3 launch takes the schedule's grid and item order; the
+  // bilinear launch runs one workgroup per item in order.
   void warp(hipStream_t st, const Schedule& sc) {
-    const WarpPlan& w = wplans[item_frames_index(sc.frames_per_item)];
-    if (w.total_items) {
-      gh::launch_warp_batched(w.jobs.ptr, w.item_job.ptr, warp_frames.ptr, w.total_items,
+    const int o = item_frames_index(sc.frames_per_item);
+    const WarpPlan& l = wplans[0][o];
+    if (l.total_items) {
+      gh::launch_warp_batched(true, l.jobs.ptr, l.item_job.ptr, warp_frames.ptr, l.total_items,
                               reinterpret_cast<const float4*>(lut.ptr), sc.warp_grid, sc.warp_stride, st);
+    }
+    const WarpPlan& b = wplans[1][o];
+    if (b.total_items) {
+      gh::launch_warp_batched(false, b.jobs.ptr, b.item_job.ptr, warp_frames.ptr, b.total_items, nullptr,
+                              static_cast<unsigned>(std::min<long long>(b.total_items, 0x7fffffffLL)), 1, st);
     }
   }
 
@@ -182,7 +194,13 @@ struct gh_batch {
     for (size_t j : ip.per_job) gh::launch_integrate(jobs[j].args, st);
   }
 
-  long long total_items() const { return wplans[item_frames_index(sched.frames_per_item)].total_items; }
+  long long lanczos_items(int frames_per_item) const {
+    return wplans[0][item_frames_index(frames_per_item)].total_items;
+  }
+  long long total_items() const {
+    const int o = item_frames_index(sched.frames_per_item);
+    return wplans[0][o].total_items + wplans[1][o].total_items;
+  }
   int integrate_launches() const {
     const IntegratePlan& ip = iplans[sched.classes];
     return static_cast<int>(ip.launches.size() + ip.per_job.size());
@@ -218,8 +236,8 @@ struct gh_batch {
   void autotune() {
     enqueue(stream, sched);  // warm-up
     check(hipStreamSynchronize(stream), "synchronize(batch)");
-    if (wplans[item_frames_index(gh::kWarpBatch)].total_items) {
-      const unsigned base_w = gh::warp_batched_max_blocks();
+    if (lanczos_items(gh::kWarpBatch) + wplans[1][item_frames_index(gh::kWarpBatch)].total_items) {
+      const unsigned base_w = gh::warp_batched_max_blocks(true);
       auto warp_time = [&](const Schedule& sc) { return time_launch([&] { warp(stream, sc); }); };
       float best = warp_time(sched);
       for (int fpi : kItemFrames) {
@@ -232,8 +250,8 @@ struct gh_batch {
           sched = sc;
         }
       }
-      const long long items = total_items();
-      const unsigned all_items = static_cast<unsigned>(std::min<long long>(items, 0x7fffffffLL));
+      const long long items = lanczos_items(sched.frames_per_item);
+      const unsigned all_items = static_cast<unsigned>(std::min<long long>(std::max(items, 1LL), 0x7fffffffLL));
       const Schedule base = sched;
       for (unsigned grid : {base_w, base_w * 4u, all_items}) {
         for (int stride : {1, coprime_stride(items)}) {
@@ -454,29 +472,35 @@ int gh_batch_create(const gh_batch_job* jobs, int64_t n_jobs, uint32_t flags, gh
       check(hipMemcpy(b->warp_frames.ptr, wframes.data(), wframes.size() * sizeof(gh::BatchWarpFrame),
                       hipMemcpyHostToDevice),
             "hipMemcpy(batch warp frames)");
-      for (int o = 0; o < kItemFrameOptions; ++o) {
-        std::vector<gh::BatchWarpJob> pj = wjobs;
-        long long items = 0;
-        for (gh::BatchWarpJob& wj : pj) {
-          wj.frames_per_item = kItemFrames[o];
-          wj.first_item = items;
-          items += gh::warp_batched_items(wj);
+      for (int kind = 0; kind < 2; ++kind) {
+        std::vector<gh::BatchWarpJob> kj;
+        for (const gh::BatchWarpJob& wj : wjobs)
+          if ((wj.interpolation == GH_INTERP_LANCZOS3) == (kind == 0)) kj.push_back(wj);
+        if (kj.empty()) continue;
+        for (int o = 0; o < kItemFrameOptions; ++o) {
+          std::vector<gh::BatchWarpJob> pj = kj;
+          long long items = 0;
+          for (gh::BatchWarpJob& wj : pj) {
+            wj.frames_per_item = kItemFrames[o];
+            wj.first_item = items;
+            items += gh::warp_batched_items(wj);
+          }
+          // Work item -> warp job, so a workgroup finds its job with one load.
+          std::vector<unsigned short> item_job(static_cast<size_t>(items));
+          for (size_t k = 0; k < pj.size(); ++k) {
+            const long long end = k + 1 < pj.size() ? pj[k + 1].first_item : items;
+            std::fill(item_job.begin() + pj[k].first_item, item_job.begin() + end, static_cast<unsigned short>(k));
+          }
+          gh_batch::WarpPlan& w = b->wplans[kind][o];
+          w.total_items = items;
+          w.item_job.ensure(item_job.size(), "hipMalloc(batch item table)");
+          check(hipMemcpy(w.item_job.ptr, item_job.data(), item_job.size() * sizeof(unsigned short),
+                          hipMemcpyHostToDevice),
+                "hipMemcpy(batch item table)");
+          w.jobs.ensure(pj.size(), "hipMalloc(batch warp jobs)");
+          check(hipMemcpy(w.jobs.ptr, pj.data(), pj.size() * sizeof(gh::BatchWarpJob), hipMemcpyHostToDevice),
+                "hipMemcpy(batch warp jobs)");
         }
-        // Work item -> warp job, so a workgroup finds its job with one load.
-        std::vector<unsigned short> item_job(static_cast<size_t>(items));
-        for (size_t k = 0; k < pj.size(); ++k) {
-          const long long end = k + 1 < pj.size() ? pj[k + 1].first_item : items;
-          std::fill(item_job.begin() + pj[k].first_item, item_job.begin() + end, static_cast<unsigned short>(k));
-        }
-        gh_batch::WarpPlan& w = b->wplans[o];
-        w.total_items = items;
-        w.item_job.ensure(item_job.size(), "hipMalloc(batch item table)");
-        check(hipMemcpy(w.item_job.ptr, item_job.data(), item_job.size() * sizeof(unsigned short),
-                        hipMemcpyHostToDevice),
-              "hipMemcpy(batch item table)");
-        w.jobs.ensure(pj.size(), "hipMalloc(batch warp jobs)");
-        check(hipMemcpy(w.jobs.ptr, pj.data(), pj.size() * sizeof(gh::BatchWarpJob), hipMemcpyHostToDevice),
-              "hipMemcpy(batch warp jobs)");
       }
     }
     b->n_variant_jobs = static_cast<int>(vjobs.size());
@@ -538,7 +562,7 @@ int gh_batch_create(const gh_batch_job* jobs, int64_t n_jobs, uint32_t flags, gh
         ip.launches.push_back(l);
       }
     }
-    b->sched.warp_grid = gh::warp_batched_max_blocks();
+    b->sched.warp_grid = gh::warp_batched_max_blocks(true);
     b->sched.warp_stride = 1;
     b->signature = sig.str();
     *out = b.release();
@@ -621,7 +645,7 @@ int gh_batch_set_schedule(gh_batch* b, int64_t warp_grid, int32_t warp_item_stri
       throw gh::ValueError("integrate_classes must be 0, 1 or 2");
     if (warp_grid < 1 || warp_grid > 0x7fffffffLL) throw gh::ValueError("warp_grid must be in [1, 2^31 - 1]");
     if (warp_item_stride < 1) throw gh::ValueError("warp_item_stride must be positive");
-    const long long items = b->wplans[item_frames_index(frames_per_item)].total_items;
+    const long long items = b->lanczos_items(frames_per_item);
     if (warp_item_stride > 1 && items > 0 && std::gcd(static_cast<long long>(warp_item_stride), items) != 1)
       throw gh::ValueError("warp_item_stride must be coprime to the number of warp work items");
     check(hipStreamSynchronize(b->stream), "synchronize(batch)");

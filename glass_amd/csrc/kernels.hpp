@@ -143,11 +143,12 @@ struct BatchWarpJob {
 // Output tile width / height of the tiled warp kernels.
 void warp_tile_shape(int* tile_w, int* tile_h);
 long long warp_batched_items(const BatchWarpJob& job);  // ceil(frames / frames_per_item) * tiles
-unsigned warp_batched_max_blocks();                     // resident blocks that fill the device
-// item_job[i]: the job of work item i (jobs are in first_item order).
-void launch_warp_batched(const BatchWarpJob* jobs, const unsigned short* item_job, const BatchWarpFrame* frames,
-                         long long total_items, const float4* lut, unsigned grid, int order_stride,
-                         hipStream_t stream);
+unsigned warp_batched_max_blocks(bool lanczos);  // resident blocks that fill the device
+// The items of the Lanczos3 jobs (lanczos) or of the bilinear jobs; item_job[i]
+// is the job of work item i (jobs in first_item order).
+void launch_warp_batched(bool lanczos, const BatchWarpJob* jobs, const unsigned short* item_job,
+                         const BatchWarpFrame* frames, long long total_items, const float4* lut, unsigned grid,
+                         int order_stride, hipStream_t stream);
 
 void launch_fill(float* dst, long long n, float value, hipStream_t stream);
 void launch_add_scalar(float* dst, long long n, float value, hipStream_t stream);

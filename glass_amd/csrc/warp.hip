@@ -1302,16 +1302,15 @@ warp_lanczos3_stream_kernel(WarpBatch b, int count, WarpParams p, int tiles_x) {
 }
 
 // ------------------------------------------------- batched multi-job warp
-// Kernel over the work items of many jobs: item = (job, batch of up to
+// Kernels over the work items of many jobs: item = (job, group of up to
 // kWarpBatch frames, output tile).  Blocks stride over the item list (one
 // item per block when the grid covers it); item_job[] names each item's job,
 // so a block re-reads the job descriptor only when the job changes.
-__global__ __launch_bounds__(kTiledThreads) __attribute__((amdgpu_waves_per_eu(3))) void warp_batched_kernel(
-    const BatchWarpJob* __restrict__ jobs, const unsigned short* __restrict__ item_job,
-    const BatchWarpFrame* __restrict__ frames, long long total_items, const float4* lut_g, int order_stride) {
-  __shared__ float4 lut[kLutRows * kLutVec];
-  __shared__ float tile[kMaxTile];
-  load_lut(lut_g, lut);
+template <bool LANCZOS>
+__device__ __forceinline__ void warp_batched_items(const BatchWarpJob* __restrict__ jobs,
+                                                   const unsigned short* __restrict__ item_job,
+                                                   const BatchWarpFrame* __restrict__ frames, long long total_items,
+                                                   int order_stride, const float4* lut, float* tile) {
   int cur = -1;
   BatchWarpJob job{};
   for (long long it = blockIdx.x; it < total_items; it += gridDim.x) {
@@ -1342,7 +1341,7 @@ __global__ __launch_bounds__(kTiledThreads) __attribute__((amdgpu_waves_per_eu(3
       return FrameRef{fr.src, fr.dst, {fr.inv[0], fr.inv[1], fr.inv[2], fr.inv[3], fr.inv[4], fr.inv[5]},
                       static_cast<unsigned>(__builtin_amdgcn_readfirstlane(*(const global_u32*)fr.src_flag)), fr.dst_flag};
     };
-    if (job.interpolation == 3) {
+    if constexpr (LANCZOS) {
       // the same kernel choice as a per-job launch of this 8-frame group
       const int streamed = __builtin_amdgcn_readfirstlane(*(const global_u32*)&frames[f0].stream);
       if (streamed) lanczos_stream_tile<true>(get, count, p, ox0, oy0, lut, tile);
@@ -1351,6 +1350,25 @@ __global__ __launch_bounds__(kTiledThreads) __attribute__((amdgpu_waves_per_eu(3
       bilinear_tile(get, count, p, ox0, oy0);
     }
   }
+}
+
+// The Lanczos3 jobs' items and the bilinear jobs' items run as two launches:
+// the bilinear taps are a few gathers per pixel, and a kernel holding the
+// Lanczos3 paths' registers and LDS would run them at a third of the
+// occupancy they need to hide the gathers' latency.
+__global__ __launch_bounds__(kTiledThreads) __attribute__((amdgpu_waves_per_eu(3))) void warp_batched_kernel(
+    const BatchWarpJob* __restrict__ jobs, const unsigned short* __restrict__ item_job,
+    const BatchWarpFrame* __restrict__ frames, long long total_items, const float4* lut_g, int order_stride) {
+  __shared__ float4 lut[kLutRows * kLutVec];
+  __shared__ float tile[kMaxTile];
+  load_lut(lut_g, lut);
+  warp_batched_items<true>(jobs, item_job, frames, total_items, order_stride, lut, tile);
+}
+
+__global__ __launch_bounds__(kTiledThreads) void warp_batched_bilinear_kernel(
+    const BatchWarpJob* __restrict__ jobs, const unsigned short* __restrict__ item_job,
+    const BatchWarpFrame* __restrict__ frames, long long total_items, int order_stride) {
+  warp_batched_items<false>(jobs, item_job, frames, total_items, order_stride, nullptr, nullptr);
 }
 
 __global__ void snapshot_warp_flags_kernel(WarpBatch b, int count, unsigned* snap) {
@@ -1680,21 +1698,30 @@ long long warp_batched_items(const BatchWarpJob& job) {
   return static_cast<long long>((job.n_frames + job.frames_per_item - 1) / job.frames_per_item) * job.tiles;
 }
 
-unsigned warp_batched_max_blocks() {
+unsigned warp_batched_max_blocks(bool lanczos) {
   int dev = 0, cus = 0, per_cu = 0;
   check(hipGetDevice(&dev), "hipGetDevice");
   check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "device CU count");
-  check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, warp_batched_kernel, kTiledThreads, 0),
-        "warp occupancy");
+  if (lanczos) {
+    check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, warp_batched_kernel, kTiledThreads, 0),
+          "warp occupancy");
+  } else {
+    check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, warp_batched_bilinear_kernel, kTiledThreads, 0),
+          "warp occupancy");
+  }
   return static_cast<unsigned>(std::max(1, cus) * std::max(1, per_cu));
 }
 
-void launch_warp_batched(const BatchWarpJob* jobs, const unsigned short* item_job, const BatchWarpFrame* frames,
-                         long long total_items, const float4* lut, unsigned grid, int order_stride,
-                         hipStream_t stream) {
+void launch_warp_batched(bool lanczos, const BatchWarpJob* jobs, const unsigned short* item_job,
+                         const BatchWarpFrame* frames, long long total_items, const float4* lut, unsigned grid,
+                         int order_stride, hipStream_t stream) {
   if (total_items <= 0) return;
   const unsigned g = static_cast<unsigned>(std::min<long long>(std::max(grid, 1u), total_items));
-  warp_batched_kernel<<<g, kTiledThreads, 0, stream>>>(jobs, item_job, frames, total_items, lut, order_stride);
+  if (lanczos) {
+    warp_batched_kernel<<<g, kTiledThreads, 0, stream>>>(jobs, item_job, frames, total_items, lut, order_stride);
+  } else {
+    warp_batched_bilinear_kernel<<<g, kTiledThreads, 0, stream>>>(jobs, item_job, frames, total_items, order_stride);
+  }
   check(hipGetLastError(), "batched warp launch");
 }
 

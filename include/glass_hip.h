@@ -683,16 +683,19 @@ int gh_batch_create(const gh_batch_job* jobs, int64_t n_jobs, uint32_t flags, gh
 /* Enqueue one execution on `stream`, asynchronously; NULL runs it on the
    batch's own stream and returns when it has finished. */
 int gh_batch_run(gh_batch* batch, void* stream);
-/* out[10]: jobs, warp work items, integration launches, graph captured (0/1),
- * warp grid, warp item stride, jobs whose statistics arithmetic is decided
+/* out[10]: jobs, warp work items (both warp launches), integration launches,
+ * graph captured (0/1), Lanczos3 warp grid, Lanczos3 warp item stride, jobs
+ * whose statistics arithmetic is decided
  * on the device, autotune trials run, frames per warp work item, integration
  * capacity class set (0 each job at its smallest register capacity, 1 powers
  * of two, 2 one launch at the largest). */
 int gh_batch_info(gh_batch* batch, int64_t* out);
 /* Fixes the schedule (no autotuning; the next run captures the graph anew):
- * warp_grid workgroups in [1, 2^31 - 1], warp_item_stride 1 or coprime to
- * the work-item count, frames_per_item 2, 4 or 8, integrate_classes 0, 1 or
- * 2 (as in gh_batch_info).  Results do not depend on it. */
+ * warp_grid workgroups in [1, 2^31 - 1] and warp_item_stride 1 or coprime to
+ * the Lanczos3 jobs' work-item count (their launch; the bilinear jobs' items
+ * run one workgroup each, in order), frames_per_item 2, 4 or 8,
+ * integrate_classes 0, 1 or 2 (as in gh_batch_info).  Results do not depend
+ * on it. */
 int gh_batch_set_schedule(gh_batch* batch, int64_t warp_grid, int32_t warp_item_stride, int32_t frames_per_item,
                           int32_t integrate_classes);
 int gh_batch_destroy(gh_batch* batch);
