@@ -54,6 +54,11 @@ __device__ __forceinline__ bool source_coords(const double* m, int affine, int x
   return true;
 }
 
+// One float at byte offset `off` of a raw buffer (out-of-range offsets read 0).
+__device__ __forceinline__ float ld_rsrc_b32(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+
 // Six Lanczos3 weights for taps at floor(s) + k, k = -2..3, fractional part f.
 // With d = f - k:  sin(pi d) = (-1)^k sin(pi f)  and
 // sin(pi d / 3) = sin(pi f/3) cos(pi k/3) - cos(pi f/3) sin(pi k/3),
@@ -667,9 +672,15 @@ __device__ __forceinline__ void lanczos_tile(const GetFrame& get, int count, con
   }
 }
 
-// Bilinear over one output tile (same thread layout, no LDS), for batched
-// launches that mix interpolations.
-template <typename GetFrame>
+// Bilinear over one output tile (the tiled kernels' thread layout, no LDS),
+// for affine maps of frames under 2^30 pixels.  The arithmetic is the CPU
+// path's (src/glass/engine/warp.py:99-118: FP64 coordinates, f32 lerps), so
+// the output is the same bit for bit.  A lane computes four rows at a time
+// and issues their 16 taps before any lerp, so 16 gathers are in flight per
+// lane rather than 4.  Taps are buffer loads at 32-bit byte offsets; a pixel
+// outside the footprint reads pixel (0, 0) and discards it.
+// ATOMIC_COVERAGE: other blocks may add to the same tile's coverage.
+template <bool ATOMIC_COVERAGE, typename GetFrame>
 __device__ __forceinline__ void bilinear_tile(const GetFrame& get, int count, const WarpParams& p, int ox0, int oy0) {
   const int tx = threadIdx.x & 63;
   const int ty = threadIdx.x >> 6;
@@ -678,37 +689,61 @@ __device__ __forceinline__ void bilinear_tile(const GetFrame& get, int count, co
   const int W = p.width, H = p.height;
   if (x >= W) return;
   unsigned long long cov = 0;
+  const double xs = static_cast<double>(x);
+  const double xmax = static_cast<double>(W - 1), ymax = static_cast<double>(H - 1);
+  // frames hold fewer than 2^30 pixels (checked by the host): byte offsets fit 32 bits
+  const unsigned bytes = static_cast<unsigned>(W) * static_cast<unsigned>(H) * 4u;
+  constexpr int kGroup = 4;
+  static_assert(kLaneRows % kGroup == 0, "a lane's rows split into groups");
   for (int f = 0; f < count; ++f) {
     const FrameRef ref = get(f);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(ref.src), 0,
+                                                                        static_cast<int>(bytes), 0x00020000);
+    const double px = ref.inv[0] * xs, py = ref.inv[3] * xs;
     bool bad = false;
 #pragma unroll 1
-    for (int k = 0; k < kLaneRows; ++k) {
-      const int y = y_first + k;
-      if (y >= H) break;
-      double sx, sy;
-      source_coords(ref.inv, 1, x, y, sx, sy);
-      const bool ok = isfinite(sx) && isfinite(sy) && sx >= 0.0 && sx <= static_cast<double>(W - 1) && sy >= 0.0 &&
-                      sy <= static_cast<double>(H - 1);
-      float value = p.fill;
-      if (ok) {
+    for (int k0 = 0; k0 < kLaneRows; k0 += kGroup) {
+      float t[kGroup][4], fx[kGroup], fy[kGroup];
+      bool ok[kGroup];
+#pragma unroll
+      for (int g = 0; g < kGroup; ++g) {
+        const int y = y_first + k0 + g;
+        const double ys = static_cast<double>(y);
+        const double sx = (px + ref.inv[1] * ys) + ref.inv[2];
+        const double sy = (py + ref.inv[4] * ys) + ref.inv[5];
+        ok[g] = y < H && isfinite(sx) && isfinite(sy) && sx >= 0.0 && sx <= xmax && sy >= 0.0 && sy <= ymax;
         const double fx0 = floor(sx), fy0 = floor(sy);
-        const int x0 = static_cast<int>(fx0), y0 = static_cast<int>(fy0);
-        const int x1 = min(x0 + 1, W - 1), y1 = min(y0 + 1, H - 1);
-        const float tx_ = static_cast<float>(sx - fx0), ty_ = static_cast<float>(sy - fy0);
-        const float* src = ref.src;
-        const float a00 = ld_global(src, static_cast<long long>(y0) * W + x0);
-        const float a01 = ld_global(src, static_cast<long long>(y0) * W + x1);
-        const float a10 = ld_global(src, static_cast<long long>(y1) * W + x0);
-        const float a11 = ld_global(src, static_cast<long long>(y1) * W + x1);
-        const float top = a00 * (1.0f - tx_) + a01 * tx_;
-        const float bottom = a10 * (1.0f - tx_) + a11 * tx_;
-        value = top * (1.0f - ty_) + bottom * ty_;
-        cov += 1ull << (4 * k);
+        int ix, iy;  // the hardware conversion saturates (ignored outside the footprint)
+        asm("v_cvt_i32_f64 %0, %1" : "=v"(ix) : "v"(fx0));
+        asm("v_cvt_i32_f64 %0, %1" : "=v"(iy) : "v"(fy0));
+        const int x0 = ok[g] ? ix : 0, y0 = ok[g] ? iy : 0;
+        const unsigned dx = x0 + 1 < W ? 4u : 0u;
+        const unsigned dy = y0 + 1 < H ? static_cast<unsigned>(W) * 4u : 0u;
+        const unsigned o00 = (static_cast<unsigned>(y0) * static_cast<unsigned>(W) + static_cast<unsigned>(x0)) * 4u;
+        t[g][0] = ld_rsrc_b32(rs, o00);
+        t[g][1] = ld_rsrc_b32(rs, o00 + dx);
+        t[g][2] = ld_rsrc_b32(rs, o00 + dy);
+        t[g][3] = ld_rsrc_b32(rs, o00 + dy + dx);
+        fx[g] = static_cast<float>(sx - fx0);
+        fy[g] = static_cast<float>(sy - fy0);
       }
-      st_global(ref.dst, static_cast<long long>(y) * W + x, value);
-      bad |= !isfinite(value);
+#pragma unroll
+      for (int g = 0; g < kGroup; ++g) {
+        const int y = y_first + k0 + g;
+        float value = p.fill;
+        if (ok[g]) {
+          const float top = t[g][0] * (1.0f - fx[g]) + t[g][1] * fx[g];
+          const float bottom = t[g][2] * (1.0f - fx[g]) + t[g][3] * fx[g];
+          value = top * (1.0f - fy[g]) + bottom * fy[g];
+          cov += 1ull << (4 * (k0 + g));
+        }
+        if (y < H) {
+          st_global(ref.dst, static_cast<long long>(y) * W + x, value);
+          bad |= !isfinite(value);
+        }
+      }
     }
-    if (bad) atomicOr(ref.dst_flag, 1u);
+    if (__any(bad) && tx == 0) atomicOr(ref.dst_flag, 1u);
   }
   if (p.coverage) {
 #pragma unroll 1
@@ -716,9 +751,29 @@ __device__ __forceinline__ void bilinear_tile(const GetFrame& get, int count, co
       const int y = y_first + k;
       if (y >= H) break;
       const float add = static_cast<float>((cov >> (4 * k)) & 15ull);
-      if (add != 0.0f) atomicAdd(p.coverage + static_cast<long long>(y) * W + x, add);
+      const long long o = static_cast<long long>(y) * W + x;
+      if constexpr (ATOMIC_COVERAGE) {
+        if (add != 0.0f) atomicAdd(p.coverage + o, add);
+      } else {
+        st_global(p.coverage, o, ld_global(p.coverage, o) + add);
+      }
     }
   }
+}
+
+// Affine bilinear warp of a launch's frames over 128 x 32 output tiles.
+__global__ __launch_bounds__(kTiledThreads) void warp_bilinear_tiled_kernel(WarpBatch b, int count, WarpParams p,
+                                                                            int tiles_x) {
+  const unsigned nb = gridDim.x, bid = blockIdx.x;
+  const unsigned xcd = bid & 7u, q = nb >> 3, r = nb & 7u;
+  const unsigned logical = xcd * q + min(xcd, r) + (bid >> 3);
+  const int ox0 = static_cast<int>(logical % tiles_x) * kOutW;
+  const int oy0 = static_cast<int>(logical / tiles_x) * kOutH;
+  auto get = [&](int f) {
+    const double* m = b.inv[f];
+    return FrameRef{b.src[f], b.dst[f], {m[0], m[1], m[2], m[3], m[4], m[5]}, b.src_flags[f], b.nonfinite[f]};
+  };
+  bilinear_tile<false>(get, count, p, ox0, oy0);
 }
 
 __device__ __forceinline__ void load_lut(const float4* lut_g, float4* lut) {
@@ -1347,7 +1402,7 @@ __device__ __forceinline__ void warp_batched_items(const BatchWarpJob* __restric
       if (streamed) lanczos_stream_tile<true>(get, count, p, ox0, oy0, lut, tile);
       else lanczos_tile<true>(get, count, p, ox0, oy0, lut, tile);
     } else {
-      bilinear_tile(get, count, p, ox0, oy0);
+      bilinear_tile<true>(get, count, p, ox0, oy0);
     }
   }
 }
@@ -1765,8 +1820,15 @@ void launch_warp(const WarpBatch& batch, int count, const WarpParams& p, hipStre
     const int tiles_y = (p.height + kOutH - 1) / kOutH;
     warp_lanczos3_tiled_kernel<<<tiles_x * tiles_y, kTiledThreads, 0, stream>>>(batch, count, p, tiles_x);
   } else if (p.interpolation == 1) {
-    if (p.affine) warp_bilinear_kernel<true><<<grid, block, 0, stream>>>(batch, count, p);
-    else warp_bilinear_kernel<false><<<grid, block, 0, stream>>>(batch, count, p);
+    if (p.affine && static_cast<long long>(p.width) * p.height < (1ll << 30)) {
+      const int tiles_x = (p.width + kOutW - 1) / kOutW;
+      const int tiles_y = (p.height + kOutH - 1) / kOutH;
+      warp_bilinear_tiled_kernel<<<tiles_x * tiles_y, kTiledThreads, 0, stream>>>(batch, count, p, tiles_x);
+    } else if (p.affine) {
+      warp_bilinear_kernel<true><<<grid, block, 0, stream>>>(batch, count, p);
+    } else {
+      warp_bilinear_kernel<false><<<grid, block, 0, stream>>>(batch, count, p);
+    }
   } else {
     if (p.affine) warp_lanczos3_kernel<true><<<grid, block, 0, stream>>>(batch, count, p);
     else warp_lanczos3_kernel<false><<<grid, block, 0, stream>>>(batch, count, p);
