@@ -160,7 +160,7 @@ __device__ __forceinline__ void compare_exchange(Samples<CAP>& v) {
   // separates them, both inputs stay live in between and a 200-sample network
   // needs twice the registers.  Keys are never NaN (invalid samples are +inf).
   float lo, hi;
-  asm("v_min_f32 %0, %2, %3\n\tv_max_f32 %1, %2, %3" : "=&v"(lo), "=v"(hi) : "v"(x), "v"(y));
+  asm volatile("v_min_f32 %0, %2, %3\n\tv_max_f32 %1, %2, %3" : "=&v"(lo), "=v"(hi) : "v"(x), "v"(y));
   v.template set<i>(lo);
   v.template set<j>(hi);
 }
@@ -331,8 +331,7 @@ __device__ __attribute__((noinline)) float frame_axis_std(const IntegrateArgs& a
 // Winsorized clip + weighted mean of pixel p (p >= npix: no output).
 template <int CAP, bool UNIT>
 __device__ __forceinline__ void winsorized_pixel(const IntegrateArgs& a, long long p) {
-  const bool in_range = p < a.npix;
-  const long long pp = in_range ? p : 0;
+  const long long pp0 = p < a.npix ? p : 0;
 
   // Per-tile choice of the statistics arithmetic (CPUStackEngine tiles) or
   // one choice for the whole frame (weighted_integrate_stack).  Decided
@@ -341,8 +340,8 @@ __device__ __forceinline__ void winsorized_pixel(const IntegrateArgs& a, long lo
   bool all_valid = (a.variant_ptr ? *(const global_u32*)a.variant_ptr : static_cast<unsigned>(a.variant_all_valid)) != 0u;
   if (a.tile_flags) {
     const unsigned w = static_cast<unsigned>(a.width);
-    const unsigned y = static_cast<unsigned>(pp) / w;  // npix < 2^30
-    const unsigned x = static_cast<unsigned>(pp) - y * w;
+    const unsigned y = static_cast<unsigned>(pp0) / w;  // npix < 2^30
+    const unsigned x = static_cast<unsigned>(pp0) - y * w;
     const unsigned t = static_cast<unsigned>(a.tile_size);
     all_valid = a.tile_flags[(y / t) * static_cast<unsigned>(a.tiles_x) + (x / t)] != 0;
   }
@@ -359,7 +358,7 @@ __device__ __forceinline__ void winsorized_pixel(const IntegrateArgs& a, long lo
   // 16 table entries is addressed through a fresh opaque zero so the scalar
   // loads of the pointers stay next to their use: hoisted to the kernel entry,
   // CAP 64-bit pointers overflow the SGPRs and spill.
-  const unsigned byte_offset = static_cast<unsigned>(pp) * 4u;  // host guarantees npix < 2^30
+  const unsigned byte_offset = static_cast<unsigned>(pp0) * 4u;  // host guarantees npix < 2^30
   int zero = 0;
   static_for<CAP>([&](auto K) GH_ALWAYS_INLINE {
     constexpr int k = decltype(K)::value;
@@ -392,12 +391,38 @@ __device__ __forceinline__ void winsorized_pixel(const IntegrateArgs& a, long lo
   } else {
     asm volatile("" ::"v"(swx), "v"(sw), "v"(su), "v"(cnt));
   }
+  // The load-pass sums, the valid count, the pixel index and the statistics
+  // choice wait out the sort in LDS.  Held in VGPRs they are the registers
+  // that the capacity-200 network lacks, and the allocator spills them to
+  // scratch (16 B written per pixel, as HBM traffic).  The slot is
+  // addressed by the lane's position in the block recomputed from mbcnt and a
+  // scalar wave index, so threadIdx.x is not held across the sort either; the
+  // memory clobber makes the compiler reload the values instead of keeping
+  // the stored registers alive.
+  __shared__ double park_sums[UNIT ? 1 : 3][kBlock];
+  __shared__ unsigned park_pixel[kBlock];  // p < 2^31 (npix < 2^30), bit 31: all_valid
+  __shared__ int park_count[kBlock];
+  const int wave_base = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x)) & ~63;
+  {
+    const int slot = static_cast<int>(threadIdx.x);
+    park_sums[0][slot] = swx;
+    if constexpr (!UNIT) {
+      park_sums[1][slot] = sw;
+      park_sums[2][slot] = su;
+    }
+    park_pixel[slot] = static_cast<unsigned>(p) | (all_valid ? 0x80000000u : 0u);
+    park_count[slot] = cnt;
+  }
+  asm volatile("" ::: "memory");
 
   // No branch between the load pass and the network: a branch here lets the
   // compiler sink the key selects past it and keep raw values and masks alive
   // together.  Pixels without samples (cnt == 0) run the network on +inf keys
   // and are written as zeros at the end.
   sort_registers<CAP>(v);
+  asm volatile("" ::: "memory");
+  const int slot = wave_base + static_cast<int>(__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)));
+  cnt = park_count[slot];
 
   // Waterfall over the distinct valid counts of the wave; each pass fetches
   // the six rank registers with a wave-uniform register index.
@@ -416,6 +441,17 @@ __device__ __forceinline__ void winsorized_pixel(const IntegrateArgs& a, long lo
       pending = false;
     }
   }
+
+  swx = park_sums[0][slot];
+  if constexpr (!UNIT) {
+    sw = park_sums[1][slot];
+    su = park_sums[2][slot];
+  }
+  const unsigned parked = park_pixel[slot];
+  p = static_cast<long long>(parked & 0x7fffffffu);
+  all_valid = (parked >> 31) != 0u;
+  const bool in_range = p < a.npix;
+  const long long pp = in_range ? p : 0;
 
   const float center0 = lerp_percentile(q.a50, q.b50, q.t50, all_valid);
   const float q25 = lerp_percentile(q.a25, q.b25, q.t25, all_valid);
@@ -472,7 +508,7 @@ __device__ __forceinline__ void winsorized_pixel(const IntegrateArgs& a, long lo
   static_for<CAP>([&](auto K) GH_ALWAYS_INLINE {
     constexpr int k = decltype(K)::value;
     const double d = static_cast<double>(__builtin_amdgcn_fmed3f(v.template get<k>(), lo_clip2, hi_clip2)) - wmean;
-    wss += d * d;
+    wss = __builtin_fma(d, d, wss);
   });
   const double d_pad = static_cast<double>(hi_clip2) - wmean;
   wss -= pad * (d_pad * d_pad);
