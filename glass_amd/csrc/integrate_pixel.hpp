@@ -37,36 +37,104 @@ constexpr int net_size(int n) {
   return c;
 }
 
+constexpr int net_passes(int n) {
+  if (n < 2) return 0;
+  int t = 0;
+  while ((1 << t) < n) ++t;
+  return t * (t + 1) / 2;
+}
+
+// Pruning for waves whose valid counts all lie in (CAP - kFastWindow, CAP]:
+// the integration reads the sorted keys only at the quartile ranks of such a
+// count, at its largest valid sample and at the ends, where the scans of the
+// clip bounds walk.  A comparator neither of whose outputs reaches one of
+// those slots (through the comparators after it) can be skipped; the slots
+// not read may then hold their keys in any order, but the multiset of all
+// slots is unchanged.  Exact slots: the bottom kExactEnd, the top
+// kFastWindow + kExactEnd (the +inf keys of the invalid samples sit above the
+// valid ones) and the rank slots of every count in the window.
+constexpr int kFastWindow = 8;
+constexpr int kExactEnd = 16;
+constexpr int kExactTop = kFastWindow + kExactEnd;
+
+constexpr bool exact_slot(int cap, int k) {
+  if (k < kExactEnd || k >= cap - kExactTop) return true;
+  for (int c = cap - kFastWindow + 1; c <= cap; ++c) {
+    if (c < 1) continue;
+    const int last = c - 1;
+    if (k == last) return true;
+    for (int m = 1; m <= 3; ++m) {
+      const int l = (last * m) >> 2;  // the ranks pick_ranks reads
+      if (k == l || k == (l + 1 < last ? l + 1 : last)) return true;
+    }
+  }
+  return false;
+}
+
+// Algorithm M's comparators in pass order (the comparators of one pass touch
+// disjoint slots, so their order within the pass is free).  Within pass p the
+// comparators [begin[p], mid[p]) feed an exact slot and [mid[p], begin[p+1])
+// do not.
 template <int N>
 struct Network {
   static constexpr int kSize = net_size(N);
+  static constexpr int kPasses = net_passes(N);
   short a[kSize > 0 ? kSize : 1];
   short b[kSize > 0 ? kSize : 1];
+  short begin[kPasses + 1];
+  short mid[kPasses > 0 ? kPasses : 1];
 };
 
 template <int N>
 constexpr Network<N> make_network() {
   Network<N> net{};
   if (N < 2) return net;
+  constexpr int kSize = Network<N>::kSize;
+  short pa[kSize]{}, pb[kSize]{}, pass[kSize]{};
   int t = 0;
   while ((1 << t) < N) ++t;
-  int c = 0;
+  int c = 0, np = 0;
   for (int p = 1 << (t - 1); p > 0; p >>= 1) {
     int q = 1 << (t - 1), r = 0, d = p;
     while (true) {
       for (int i = 0; i < N - d; ++i) {
         if ((i & p) == r) {
-          net.a[c] = static_cast<short>(i);
-          net.b[c] = static_cast<short>(i + d);
+          pa[c] = static_cast<short>(i);
+          pb[c] = static_cast<short>(i + d);
+          pass[c] = static_cast<short>(np);
           ++c;
         }
       }
+      ++np;
       if (q == p) break;
       d = q - p;
       q >>= 1;
       r = p;
     }
   }
+  // Walk back from the exact slots: a comparator is kept when one of its
+  // outputs is needed, and then both of its inputs are.
+  bool need[N]{}, keep[kSize]{};
+  for (int k = 0; k < N; ++k) need[k] = exact_slot(N, k);
+  for (int i = kSize - 1; i >= 0; --i) {
+    keep[i] = need[pa[i]] || need[pb[i]];
+    if (keep[i]) need[pa[i]] = need[pb[i]] = true;
+  }
+  int out = 0;
+  for (int p = 0; p < np; ++p) {
+    net.begin[p] = static_cast<short>(out);
+    for (int pruned = 0; pruned < 2; ++pruned) {
+      if (pruned) net.mid[p] = static_cast<short>(out);
+      for (int i = 0; i < kSize; ++i) {
+        if (pass[i] == p && keep[i] == (pruned == 0)) {
+          net.a[out] = pa[i];
+          net.b[out] = pb[i];
+          ++out;
+        }
+      }
+    }
+  }
+  net.begin[np] = static_cast<short>(out);
   return net;
 }
 
@@ -165,14 +233,24 @@ __device__ __forceinline__ void compare_exchange(Samples<CAP>& v) {
   v.template set<j>(hi);
 }
 
-template <int CAP, int... I>
-__device__ __forceinline__ void sort_network(Samples<CAP>& v, std::integer_sequence<int, I...>) {
-  (compare_exchange<CAP, I>(v), ...);
+template <int CAP, int FIRST, int... I>
+__device__ __forceinline__ void run_comparators(Samples<CAP>& v, std::integer_sequence<int, I...>) {
+  (compare_exchange<CAP, FIRST + I>(v), ...);
 }
 
+// The network pass by pass; `fast` (wave-uniform: every lane's valid count in
+// (CAP - kFastWindow, CAP]) skips each pass's comparators that feed no exact
+// slot, behind one scalar branch per pass.
 template <int CAP>
-__device__ __forceinline__ void sort_registers(Samples<CAP>& v) {
-  sort_network<CAP>(v, std::make_integer_sequence<int, Network<CAP>::kSize>{});
+__device__ __forceinline__ void sort_registers(Samples<CAP>& v, bool fast) {
+  static_for<Network<CAP>::kPasses>([&](auto P) GH_ALWAYS_INLINE {
+    constexpr int pi = decltype(P)::value;
+    constexpr int b = kNetwork<CAP>.begin[pi], m = kNetwork<CAP>.mid[pi], e = kNetwork<CAP>.begin[pi + 1];
+    run_comparators<CAP, b>(v, std::make_integer_sequence<int, m - b>{});
+    if constexpr (e > m) {
+      if (!fast) run_comparators<CAP, m>(v, std::make_integer_sequence<int, e - m>{});
+    }
+  });
 }
 
 // ------------------------------------------------------ scans of the ends
@@ -186,8 +264,13 @@ __device__ __forceinline__ void sort_registers(Samples<CAP>& v) {
 constexpr int kScanGroup = 4;
 
 template <int CAP, bool FROM_TOP, typename Hit, typename Body>
-__device__ __forceinline__ void scan_ends(const Samples<CAP>& v, Hit&& hit, Body&& body) {
+__device__ __forceinline__ void scan_ends(const Samples<CAP>& v, bool fast, Hit&& hit, Body&& body) {
   constexpr int kGroups = (CAP + kScanGroup - 1) / kScanGroup;
+  // After the pruned network (`fast`) only the end's exact slots are in
+  // order: a group without hits there still ends the scan, but once the scan
+  // has walked past them (some lane has more samples beyond its bound than
+  // the exact end holds) it visits every remaining group.
+  constexpr int kExactGroups = (FROM_TOP ? kExactTop : kExactEnd) / kScanGroup;
   bool go = true;
   static_for<kGroups>([&](auto G) GH_ALWAYS_INLINE {
     constexpr int g = decltype(G)::value;
@@ -204,19 +287,19 @@ __device__ __forceinline__ void scan_ends(const Samples<CAP>& v, Hit&& hit, Body
           body(x, h);
         }
       });
-      if (__ballot(any) == 0ull) go = false;
+      if (__ballot(any) == 0ull && !(fast && g >= kExactGroups)) go = false;
     }
   });
 }
 
 template <int CAP, typename Hit, typename Body>
-__device__ __forceinline__ void scan_from_bottom(const Samples<CAP>& v, Hit&& hit, Body&& body) {
-  scan_ends<CAP, false>(v, hit, body);
+__device__ __forceinline__ void scan_from_bottom(const Samples<CAP>& v, bool fast, Hit&& hit, Body&& body) {
+  scan_ends<CAP, false>(v, fast, hit, body);
 }
 
 template <int CAP, typename Hit, typename Body>
-__device__ __forceinline__ void scan_from_top(const Samples<CAP>& v, Hit&& hit, Body&& body) {
-  scan_ends<CAP, true>(v, hit, body);
+__device__ __forceinline__ void scan_from_top(const Samples<CAP>& v, bool fast, Hit&& hit, Body&& body) {
+  scan_ends<CAP, true>(v, fast, hit, body);
 }
 
 // ------------------------------------------------------- quartile extraction
@@ -413,13 +496,16 @@ __device__ __forceinline__ void winsorized_pixel(const IntegrateArgs& a, long lo
     park_pixel[slot] = static_cast<unsigned>(p) | (all_valid ? 0x80000000u : 0u);
     park_count[slot] = cnt;
   }
+  // Pruned network when every lane's valid count is within kFastWindow of
+  // the capacity (wave-uniform; counts below take the full network).
+  const bool fast = __ballot(cnt <= CAP - kFastWindow) == 0ull;
   asm volatile("" ::: "memory");
 
   // No branch between the load pass and the network: a branch here lets the
   // compiler sink the key selects past it and keep raw values and masks alive
   // together.  Pixels without samples (cnt == 0) run the network on +inf keys
   // and are written as zeros at the end.
-  sort_registers<CAP>(v);
+  sort_registers<CAP>(v, fast);
   asm volatile("" ::: "memory");
   const int slot = wave_base + static_cast<int>(__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)));
   cnt = park_count[slot];
@@ -479,12 +565,12 @@ __device__ __forceinline__ void winsorized_pixel(const IntegrateArgs& a, long lo
   // FP64 bits, which moves a float32 result only at a rounding boundary.
   double sum_lo = 0.0, sum_hi = 0.0;
   int c_lo = 0, c_hi = 0;
-  scan_from_bottom<CAP>(v, [&](float x) { return x < lo_clip; }, [&](float x, bool h) {
+  scan_from_bottom<CAP>(v, fast, [&](float x) { return x < lo_clip; }, [&](float x, bool h) {
     c_lo += h ? 1 : 0;
     sum_lo += static_cast<double>(h ? x : 0.0f);
   });
   // from the top: the +inf keys (slots past the valid count) come first
-  scan_from_top<CAP>(v, [&](float x) { return x > hi_clip; }, [&](float x, bool h) {
+  scan_from_top<CAP>(v, fast, [&](float x) { return x > hi_clip; }, [&](float x, bool h) {
     const bool real = h && x != INFINITY;
     c_hi += real ? 1 : 0;
     sum_hi += static_cast<double>(real ? x : 0.0f);
@@ -530,11 +616,11 @@ __device__ __forceinline__ void winsorized_pixel(const IntegrateArgs& a, long lo
   // hi_t capped at FLT_MAX: the +inf keys past the valid count stay above it
   // (the scan walks through them) without being counted.
   const float hi_c = fminf(hi_t, 3.402823466e38f);
-  scan_from_bottom<CAP>(v, [&](float x) { return usable && x < lo_t; }, [&](float x, bool h) {
+  scan_from_bottom<CAP>(v, fast, [&](float x) { return usable && x < lo_t; }, [&](float x, bool h) {
     n_low += h ? 1 : 0;
     if constexpr (UNIT) rej_sum += static_cast<double>(h ? x : 0.0f);
   });
-  scan_from_top<CAP>(v, [&](float x) { return usable && x > hi_c; }, [&](float x, bool h) {
+  scan_from_top<CAP>(v, fast, [&](float x) { return usable && x > hi_c; }, [&](float x, bool h) {
     const bool real = h && x != INFINITY;
     n_high += real ? 1 : 0;
     if constexpr (UNIT) rej_sum += static_cast<double>(real ? x : 0.0f);

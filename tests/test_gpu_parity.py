@@ -429,6 +429,42 @@ def test_winsorized_sums_straddling_zero_and_wide_magnitudes(gpu, frames, weight
     _compare(got, exp)
 
 
+@pytest.mark.parametrize("frames,weighted", [(16, False), (48, True), (64, False), (128, True), (160, False),
+                                             (193, False), (200, False), (200, True), (256, False)])
+def test_winsorized_pruned_network_waves(gpu, frames, weighted):
+    # Waves whose valid counts all lie within 8 of the sort capacity take a
+    # pruned network that orders only the quartile ranks and the ends; each
+    # 64-pixel row below is one wave.  Rows: clean; 20-40 low and 25-35 high
+    # outliers per pixel (the clip scans walk past the ordered ends into the
+    # unordered middle); the same with one NaN in the row (full network);
+    # integer-valued samples (ties); three NaNs per pixel (+inf keys inside
+    # the window when the capacity allows); outliers in every other pixel.
+    rng = np.random.default_rng(900 + frames + weighted)
+    h, w = 6, 64
+    stack = rng.normal(1000.0, 4.0, size=(frames, h, w)).astype(np.float32)
+    n_lo, n_hi = min(40, frames // 5), min(35, frames // 6)
+    for row in (1, 2, 5):
+        for x in range(0 if row != 5 else 1, w, 1 if row != 5 else 2):
+            k_lo = int(rng.integers(n_lo // 2, n_lo + 1))
+            k_hi = int(rng.integers(n_hi // 2, n_hi + 1))
+            idx = rng.permutation(frames)
+            stack[idx[:k_lo], row, x] -= rng.uniform(50, 300, k_lo).astype(np.float32)
+            stack[idx[k_lo:k_lo + k_hi], row, x] += rng.uniform(50, 300, k_hi).astype(np.float32)
+    stack[int(rng.integers(frames)), 2, 17] = np.nan
+    stack[:, 3, :] = np.round(stack[:, 3, :] / 3.0)
+    for x in range(w):
+        stack[rng.permutation(frames)[:3], 4, x] = np.nan
+    weights = rng.uniform(0.5, 2.0, frames).astype(np.float32) if weighted else None
+    print(f"winsorized pruned-network waves frames={frames} weighted={weighted}", flush=True)
+    with _stack_of(stack) as s:
+        got = s.integrate_hardened_winsorized_sigma(weights, 3.0, 3.0, 3, 0.5)
+    exp = oracle.weighted_integrate_stack(stack, weights=weights, rejection="winsorized_sigma")
+    _compare(got, exp)
+    # every outlier was rejected: from 200 frames on, more than the 16 / 24
+    # ordered end slots
+    assert np.min(exp[3][1]) >= n_lo // 2 and np.min(exp[4][1]) >= n_hi // 2
+
+
 @pytest.mark.parametrize("frames", [4, 20, 40, 120])
 def test_winsorized_weights_and_zero_weight_frames(gpu, frames):
     rng = np.random.default_rng(100 + frames)
