@@ -64,6 +64,24 @@ __device__ __forceinline__ T load_uniform(const T* p) {
   return out;
 }
 
+// A wave-uniform descriptor read by scalar loads (constant address space):
+// unlike load_uniform it issues no vector memory operation, so it does not
+// wait behind a wave's outstanding vector loads.  Valid only where every
+// lane reads the same entry and no earlier work of this kernel wrote it.
+template <typename T>
+__device__ __forceinline__ T load_scalar(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "descriptor must be whole 32-bit words");
+  constexpr int kWords = static_cast<int>(sizeof(T) / 4);
+  typedef __attribute__((address_space(4))) const unsigned const_u32_word;
+  const const_u32_word* src = (const const_u32_word*)p;
+  unsigned w[kWords];
+#pragma unroll
+  for (int i = 0; i < kWords; ++i) w[i] = src[i];
+  T out;
+  __builtin_memcpy(&out, w, sizeof(T));
+  return out;
+}
+
 // Entry k of a device table of pointers, as a wave-uniform pointer.  The
 // table is read through the constant address space, so a wave-uniform k
 // becomes a scalar load into SGPRs (a global load would fetch the same 8

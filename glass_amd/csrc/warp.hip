@@ -251,41 +251,41 @@ static_assert(kWaves % kColGroups == 0 && kOutH % (kWaves / kColGroups) == 0, "w
 constexpr int kLutVec = kLanczosLutStride / 4;
 constexpr int kLutRows = kLanczosLutIntervals + 1;
 static_assert(kLanczosLutStride % 4 == 0, "table intervals are whole float4s");
+static_assert(kLutVec == 5, "an interval's 18 coefficients are four float4s and a float2");
+// LDS table: four planes of kLutRows float4s, then one plane of kLutRows
+// float2s (floats 16 and 17; the host table's floats 18 and 19 are padding).
+constexpr int kLutLdsVec4 = 4 * kLutRows + (kLutRows + 1) / 2;
 static_assert(kLaneRows * 4 <= 64 && kWarpBatch <= 15, "coverage counts are packed 4 bits per row");
 
-// Six normalised tap weights for fractional offset f in [0, 1].  The LDS table
-// is stored as kLutVec planes of kLutRows float4s (plane v holds floats
-// 4v .. 4v+3 of every interval), so lanes reading distinct intervals touch
-// consecutive 16-byte slots; row kLanczosLutIntervals serves f == 1 exactly.
-__device__ __forceinline__ void lut_weights(const float4* lut, float f, float (&w)[6]) {
-  const float t = f * static_cast<float>(kLanczosLutIntervals);
-  const int i = static_cast<int>(t);
-  const float u = __builtin_amdgcn_fractf(t);
+// The 18 quadratic coefficients of table interval i.  The LDS table is
+// stored as planes (plane v < 4 holds floats 4v .. 4v+3 of every interval as
+// float4s, the last plane floats 16 and 17 as float2s), so lanes reading
+// distinct intervals touch consecutive slots; row kLanczosLutIntervals serves
+// f == 1 exactly.
+__device__ __forceinline__ void lut_coefficients(const float4* lut, int i, float (&c)[18]) {
   const float4* e = lut + i;
-  float c[kLanczosLutStride];
 #pragma unroll
-  for (int v = 0; v < kLutVec; ++v) {
+  for (int v = 0; v < 4; ++v) {
     const float4 q = e[v * kLutRows];
     c[4 * v + 0] = q.x;
     c[4 * v + 1] = q.y;
     c[4 * v + 2] = q.z;
     c[4 * v + 3] = q.w;
   }
-#pragma unroll
-  for (int k = 0; k < 6; ++k) w[k] = fmaf(u, fmaf(u, c[3 * k + 2], c[3 * k + 1]), c[3 * k]);
+  const float2 q = reinterpret_cast<const float2*>(lut + 4 * kLutRows)[i];
+  c[16] = q.x;
+  c[17] = q.y;
 }
 
-// The 18 quadratic coefficients of table interval i (LDS planes, see above).
-__device__ __forceinline__ void lut_coefficients(const float4* lut, int i, float (&c)[18]) {
-  const float4* e = lut + i;
+// Six normalised tap weights for fractional offset f in [0, 1].
+__device__ __forceinline__ void lut_weights(const float4* lut, float f, float (&w)[6]) {
+  const float t = f * static_cast<float>(kLanczosLutIntervals);
+  const int i = static_cast<int>(t);
+  const float u = __builtin_amdgcn_fractf(t);
+  float c[18];
+  lut_coefficients(lut, i, c);
 #pragma unroll
-  for (int v = 0; v < kLutVec; ++v) {
-    const float4 q = e[v * kLutRows];
-    if (4 * v + 0 < 18) c[4 * v + 0] = q.x;
-    if (4 * v + 1 < 18) c[4 * v + 1] = q.y;
-    if (4 * v + 2 < 18) c[4 * v + 2] = q.z;
-    if (4 * v + 3 < 18) c[4 * v + 3] = q.w;
-  }
+  for (int k = 0; k < 6; ++k) w[k] = fmaf(u, fmaf(u, c[3 * k + 2], c[3 * k + 1]), c[3 * k]);
 }
 
 // T: an LDS pointer (staged tile) or a global-address-space frame pointer.
@@ -779,13 +779,16 @@ __global__ __launch_bounds__(kTiledThreads) void warp_bilinear_tiled_kernel(Warp
 __device__ __forceinline__ void load_lut(const float4* lut_g, float4* lut) {
   // host layout [interval][vec] -> LDS planes [vec][interval]
   for (int i = threadIdx.x; i < kLutRows * kLutVec; i += kTiledThreads) {
-    lut[(i % kLutVec) * kLutRows + i / kLutVec] = lut_g[i];
+    const int row = i / kLutVec, v = i % kLutVec;
+    const float4 q = lut_g[i];
+    if (v < 4) lut[v * kLutRows + row] = q;
+    else reinterpret_cast<float2*>(lut + 4 * kLutRows)[row] = make_float2(q.x, q.y);
   }
 }
 
 __global__ __launch_bounds__(kTiledThreads) __attribute__((amdgpu_waves_per_eu(3))) void warp_lanczos3_tiled_kernel(WarpBatch b, int count, WarpParams p,
                                                                              int tiles_x) {
-  __shared__ float4 lut[kLutRows * kLutVec];
+  __shared__ float4 lut[kLutLdsVec4];
   __shared__ float tile[kMaxTile];
   load_lut(p.lanczos_lut, lut);
   // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
@@ -869,6 +872,7 @@ __device__ __forceinline__ float ld_rsrc(frame_rsrc_t r, unsigned voff, unsigned
 // Per-frame uniform values of the streamed path.
 struct StreamFrame {
   frame_rsrc_t src;
+  frame_rsrc_t dst_rs;  // the output frame, for stores at 32-bit lane offsets
   float* dst;
   double m1, m2, m4, m5;
   float dtx, dty;  // per-row drift of the x / y table coordinate
@@ -879,7 +883,7 @@ struct StreamFrame {
 // Staged source box of one output tile and frame (LDS, row stride
 // kBoxStride floats): rows by0 .. by0 + bh - 1, columns bx0 .. bx0 + bw - 1.
 #ifndef GH_WARP_BOX_ROWS
-#define GH_WARP_BOX_ROWS 48
+#define GH_WARP_BOX_ROWS 40
 #endif
 #ifndef GH_WARP_BOX_AHEAD
 #define GH_WARP_BOX_AHEAD 1
@@ -888,13 +892,21 @@ struct StreamFrame {
 #define GH_WARP_LUT_AHEAD 1
 #endif
 #ifndef GH_WARP_BOX_STRIDE
-#define GH_WARP_BOX_STRIDE 144
+#define GH_WARP_BOX_STRIDE 138
 #endif
 constexpr int kBoxStride = GH_WARP_BOX_STRIDE;
 static_assert(kBoxStride > 128 && kBoxStride <= 192, "a box row is two 64-lane segments and a partial third");
 constexpr int kBoxRows = GH_WARP_BOX_ROWS;
 constexpr int kBoxFloats = kBoxStride * kBoxRows;
-static_assert(kBoxFloats <= kMaxTile, "the batched kernel stages boxes in the tiled kernel's buffer");
+// Two boxes (the next frame's is staged while this frame's is read); with
+// the table they fit three blocks into the CU's 160 KB of LDS.
+constexpr int kStreamLdsFloats = 2 * kBoxFloats > kMaxTile ? 2 * kBoxFloats : kMaxTile;
+// LDS is allocated in 1280-byte granules (measured: a 54480-byte block runs
+// two per CU, a 53448-byte one three).
+constexpr int kLdsGranule = 1280;
+static_assert(((kLutLdsVec4 * 16 + kStreamLdsFloats * 4 + kLdsGranule - 1) / kLdsGranule) * kLdsGranule * 3 <=
+                  160 * 1024,
+              "three blocks of the streamed kernel per CU");
 static_assert(kSOutW == kOutW && kSOutH == kOutH, "both affine Lanczos3 kernels use the same output tiles");
 struct BoxGeom {
   int bx0, by0, bw, bh;
@@ -926,6 +938,15 @@ struct RowsFromBox {
   __device__ __forceinline__ void pin() { asm volatile("" : "+v"(base)); }
 };
 
+// s_waitcnt vmcnt(0) as a builtin (a wait the compiler accounts for), at the
+// ends of the streamed kernel's paths that read the frame with vector loads
+// (rows from the frame, exact rows, non-finite frames).  Without it the
+// compiler's wait bookkeeping carries those loads around the strip and frame
+// loops and places waits into the staged-box strips, where they would also
+// wait for the next frame's box loads in flight (a wave's vector memory
+// operations complete in order).
+__device__ __forceinline__ void drain_vector_memory() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // The 18 quadratic coefficients of interval i and the six weights at u.
 __device__ __forceinline__ void lut_eval(const float4* lut, int i, float u, float (&w)[6]) {
   float c[18];
@@ -954,7 +975,8 @@ __device__ __forceinline__ bool stream_strip_rows(const StreamFrame& fr, const f
   };
   static_for<5 + kAhead>([&](auto J) __attribute__((always_inline)) { load_row(J); });
   float nf = 0.0f;
-  float* const dst_row0 = fr.dst + static_cast<long long>(y0) * W;
+  const unsigned lane_byte = static_cast<unsigned>(x) * 4u;  // frames of < 2^31 bytes
+  const unsigned row0_byte = static_cast<unsigned>(y0) * static_cast<unsigned>(W) * 4u;  // wave-uniform
   // x table interval, position in it and column choice of row r
   auto x_coord = [&](int r, int& ix, float& u, bool& sel) __attribute__((always_inline)) {
     const float txr = fmaf(static_cast<float>(r), fr.dtx, tx0);
@@ -1033,15 +1055,17 @@ __device__ __forceinline__ bool stream_strip_rows(const StreamFrame& fr, const f
     value = live ? value : fill;
     // 0 * value is NaN exactly when value is not finite: a vector-only test
     nf = fmaf(value, 0.0f, nf);
-    if (present) st_global_u32(dst_row0 + static_cast<long long>(r) * W, static_cast<unsigned>(x), value);
+    if (present)
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, value), fr.dst_rs, lane_byte,
+                                            row0_byte + static_cast<unsigned>(r * W) * 4u, 0);
   });
+  if constexpr (!Rows::kLutAhead) drain_vector_memory();  // rows read from the frame
   return present && !isfinite(nf);
 }
 
 // One strip row by row: FP64 coordinates as the CPU path, table weights, taps
-// read from the frame (finite source) or, BOX, from the staged box (which
-// holds every tap of the tile).  Row r's coverage count goes to bits 4 r of
-// cov.
+// read from the frame (finite source) or, BOX, from the staged box when they
+// lie inside it.  Row r's coverage count goes to bits 4 r of cov.
 template <int S, bool BOX>
 __device__ __forceinline__ bool stream_strip_exact(const FrameRef& ref, const float4* lut, double px, double py,
                                                    int x, int y0, int W, int H, float fill, unsigned long long& cov,
@@ -1062,7 +1086,11 @@ __device__ __forceinline__ bool stream_strip_exact(const FrameRef& ref, const fl
       float wx[6], wy[6];
       lut_weights(lut, static_cast<float>(sx - fx0), wx);
       lut_weights(lut, static_cast<float>(sy - fy0), wy);
+      bool from_box = BOX;
       if constexpr (BOX) {
+        from_box = x0 - 2 >= g.bx0 && x0 + 3 < g.bx0 + g.bw && yy0 - 2 >= g.by0 && yy0 + 3 < g.by0 + g.bh;
+      }
+      if (from_box) {
         value = lanczos_taps(box + ((yy0 - 2 - g.by0) * kBoxStride + (x0 - 2 - g.bx0)), kBoxStride, wx, wy);
       } else {
         value = lanczos_taps((const global_cf32*)(ref.src + static_cast<long long>(yy0 - 2) * W + (x0 - 2)), W, wx,
@@ -1073,6 +1101,7 @@ __device__ __forceinline__ bool stream_strip_exact(const FrameRef& ref, const fl
     st_global(ref.dst, static_cast<long long>(y) * W + x, value);
     bad |= !isfinite(value);
   }
+  drain_vector_memory();
   return bad;
 }
 
@@ -1146,9 +1175,12 @@ __device__ __forceinline__ bool stream_strip(const FrameRef& ref, const StreamFr
 }
 
 // Source box of output tile [ox0, ox1] x [oy0, oy1] for frame `ref`: the
-// affine map takes its extremes at the corners; one pixel of margin beyond
-// the 7-column window and the strip rows covers the FP64 rounding.  Every
-// value is wave-uniform.
+// affine map takes its extremes at the corners, and the box holds the
+// 7-column window (columns floor(sx) - 3 .. + 4) and the six tap rows
+// (floor(sy) - 2 .. + 3) of each.  An interior pixel whose FP64 coordinate
+// rounds past a corner's can need a row or column more: the streamed strips
+// check their windows against the box, and the exact rows read such taps
+// from the frame.  Every value is wave-uniform.
 __device__ __forceinline__ BoxGeom box_geometry(const FrameRef& ref, bool eligible, int ox0, int oy0, int ox1,
                                                 int oy1, int W, int H) {
   BoxGeom g{0, 0, 0, 0, false};
@@ -1164,7 +1196,7 @@ __device__ __forceinline__ BoxGeom box_geometry(const FrameRef& ref, bool eligib
   asm("v_cvt_i32_f64 %0, %1" : "=v"(b) : "v"(floor(sxh)));
   asm("v_cvt_i32_f64 %0, %1" : "=v"(c) : "v"(floor(syl)));
   asm("v_cvt_i32_f64 %0, %1" : "=v"(d) : "v"(floor(syh)));
-  const int bx0 = max(a - 4, 0), bx1 = min(b + 5, W - 1), by0 = max(c - 3, 0), by1 = min(d + 4, H - 1);
+  const int bx0 = max(a - 3, 0), bx1 = min(b + 4, W - 1), by0 = max(c - 2, 0), by1 = min(d + 3, H - 1);
   g.bx0 = __builtin_amdgcn_readfirstlane(bx0);
   g.by0 = __builtin_amdgcn_readfirstlane(by0);
   g.bw = __builtin_amdgcn_readfirstlane(bx1 - bx0 + 1);
@@ -1227,6 +1259,7 @@ __device__ __forceinline__ void stage_box_lds(const FrameRef& ref, const BoxGeom
 __device__ __forceinline__ StreamFrame stream_frame(const FrameRef& ref, int bytes) {
   StreamFrame fr;
   fr.src = __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(ref.src), 0, bytes, 0x00020000);
+  fr.dst_rs = __builtin_amdgcn_make_buffer_rsrc(ref.dst, 0, bytes, 0x00020000);
   fr.dst = ref.dst;
   fr.m1 = ref.inv[1];
   fr.m2 = ref.inv[2];
@@ -1241,16 +1274,15 @@ __device__ __forceinline__ StreamFrame stream_frame(const FrameRef& ref, int byt
 
 // The streamed Lanczos3 work of one output tile over `count` frames; every
 // thread of the block calls it.  A finite-source frame whose box fits (no
-// clamp) is staged into LDS by loads that write LDS directly; the block waits
-// for them (every wave's loads: s_waitcnt vmcnt(0) and a barrier) and reads
-// its window rows from the box.  The staging latency is hidden by the other
-// blocks on the CU, not inside the block: a second box would halve the
-// blocks per CU, which measured slower.  Other finite-source frames stream
-// their window rows from the frame; the rest take the per-row paths of the
-// tiled kernel.
+// clamp) is staged into LDS by loads that write LDS directly, one frame
+// ahead into the second of two boxes (`boxes`: 2 * kBoxFloats floats), and
+// the waves read their window rows from the box.  138 x 40 boxes keep three
+// blocks per CU (a 1-degree rotation of a 128 x 32 tile needs up to 136 x 40).
+// Other finite-source frames stream their window rows from the frame; the
+// rest take the per-row paths of the tiled kernel.
 template <bool ATOMIC_COVERAGE, typename GetFrame>
 __device__ __forceinline__ void lanczos_stream_tile(const GetFrame& get, int count, const WarpParams& p, int ox0,
-                                                    int oy0, const float4* lut, float* box) {
+                                                    int oy0, const float4* lut, float* boxes) {
   const int tx = threadIdx.x & 63;
   const int ty = threadIdx.x >> 6;
   const int x = ox0 + 64 * (ty % kSColGroups) + tx;
@@ -1267,17 +1299,28 @@ __device__ __forceinline__ void lanczos_stream_tile(const GetFrame& get, int cou
   const int bytes = W * H * 4;  // < 2^31 (checked by the host)
   const double xd = static_cast<double>(x);
 
+  // Frame f's box is boxes[f & 1]; frame f + 1's is staged while frame f is
+  // computed.  One barrier per frame: after it every wave's loads of frame
+  // f's box have landed (each wave waits for its own first) and every wave is
+  // done with frame f - 1, whose buffer frame f + 1's box then takes.
+  auto geometry = [&](const FrameRef& r) {
+    return box_geometry(r, !clamp_on && r.src_flag == 0u, ox0, oy0, ox1, oy1, W, H);
+  };
+  __syncthreads();  // the buffers' previous users (an earlier item of a batched launch) are done
+  BoxGeom geo = geometry(get(0));
+  if (geo.ok) stage_box_lds(get(0), geo, W, bytes, boxes);
   for (int f = 0; f < count; ++f) {
     const FrameRef ref = get(f);
     const bool finite = !clamp_on && ref.src_flag == 0u;
-    const BoxGeom geo = box_geometry(ref, finite, ox0, oy0, ox1, oy1, W, H);
-    bool bad = false;
-    if (geo.ok) {
-      __syncthreads();  // every wave is done with the previous frame's box
-      stage_box_lds(ref, geo, W, bytes, box);
+    BoxGeom next{0, 0, 0, 0, false};
+    if (f + 1 < count) next = geometry(get(f + 1));
+    if (geo.ok || next.ok) {
       asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-      __syncthreads();  // every wave's part of the box has landed
+      __syncthreads();
     }
+    if (next.ok) stage_box_lds(get(f + 1), next, W, bytes, boxes + ((f + 1) & 1) * kBoxFloats);
+    const float* box = boxes + (f & 1) * kBoxFloats;
+    bool bad = false;
     if (finite) {
       const StreamFrame fr = stream_frame(ref, bytes);
       const double px = ref.inv[0] * xd, py = ref.inv[3] * xd;
@@ -1313,8 +1356,10 @@ __device__ __forceinline__ void lanczos_stream_tile(const GetFrame& get, int cou
         if (tf.fast && b) b = recheck_column(tf, x, yg, W, H);
         bad |= b;
       }
+      drain_vector_memory();
     }
     if (__any(bad) && tx == 0) atomicOr(ref.dst_flag, 1u);
+    geo = next;
   }
   if (p.coverage && x < W) {
 #pragma unroll
@@ -1340,20 +1385,25 @@ __device__ __forceinline__ void lanczos_stream_tile(const GetFrame& get, int cou
 #endif
 __global__ __launch_bounds__(kSThreads) __attribute__((amdgpu_waves_per_eu(GH_WARP_STREAM_WAVES_PER_EU))) void
 warp_lanczos3_stream_kernel(WarpBatch b, int count, WarpParams p, int tiles_x) {
-  __shared__ float4 lut[kLutRows * kLutVec];
-  __shared__ float box[kBoxFloats];
+  __shared__ float4 lut[kLutLdsVec4];
+  __shared__ float boxes[2 * kBoxFloats];
   load_lut(p.lanczos_lut, lut);
   const unsigned nb = gridDim.x, bid = blockIdx.x;
   const unsigned xcd = bid & 7u, q = nb >> 3, r = nb & 7u;
   const unsigned logical = xcd * q + min(xcd, r) + (bid >> 3);
   const int ox0 = static_cast<int>(logical % tiles_x) * kSOutW;
   const int oy0 = static_cast<int>(logical / tiles_x) * kSOutH;
+  // The source flags are read by scalar loads: a vector load issued while a
+  // box is being staged would wait for the staging loads (vector memory
+  // operations of a wave complete in order).
+  typedef __attribute__((address_space(4))) const unsigned const_u32;
   auto get = [&](int f) {
     const double* m = b.inv[f];
-    return FrameRef{b.src[f], b.dst[f], {m[0], m[1], m[2], m[3], m[4], m[5]}, b.src_flags[f], b.nonfinite[f]};
+    return FrameRef{b.src[f], b.dst[f], {m[0], m[1], m[2], m[3], m[4], m[5]}, ((const const_u32*)b.src_flags)[f],
+                    b.nonfinite[f]};
   };
   __syncthreads();  // the table
-  lanczos_stream_tile<false>(get, count, p, ox0, oy0, lut, box);
+  lanczos_stream_tile<false>(get, count, p, ox0, oy0, lut, boxes);
 }
 
 // ------------------------------------------------- batched multi-job warp
@@ -1391,10 +1441,12 @@ __device__ __forceinline__ void warp_batched_items(const BatchWarpJob* __restric
     p.skip_nonfinite = job.skip_nonfinite;
     p.affine = 1;
     p.coverage = job.coverage;
+    // Descriptors and source flags by scalar loads (see the streamed kernel).
+    typedef __attribute__((address_space(4))) const unsigned const_u32;
     auto get = [&](int f) {
-      const BatchWarpFrame fr = load_uniform(frames + (f0 + f));
+      const BatchWarpFrame fr = load_scalar(frames + (f0 + f));
       return FrameRef{fr.src, fr.dst, {fr.inv[0], fr.inv[1], fr.inv[2], fr.inv[3], fr.inv[4], fr.inv[5]},
-                      static_cast<unsigned>(__builtin_amdgcn_readfirstlane(*(const global_u32*)fr.src_flag)), fr.dst_flag};
+                      *(const const_u32*)fr.src_flag, fr.dst_flag};
     };
     if constexpr (LANCZOS) {
       // the same kernel choice as a per-job launch of this 8-frame group
@@ -1414,8 +1466,8 @@ __device__ __forceinline__ void warp_batched_items(const BatchWarpJob* __restric
 __global__ __launch_bounds__(kTiledThreads) __attribute__((amdgpu_waves_per_eu(3))) void warp_batched_kernel(
     const BatchWarpJob* __restrict__ jobs, const unsigned short* __restrict__ item_job,
     const BatchWarpFrame* __restrict__ frames, long long total_items, const float4* lut_g, int order_stride) {
-  __shared__ float4 lut[kLutRows * kLutVec];
-  __shared__ float tile[kMaxTile];
+  __shared__ float4 lut[kLutLdsVec4];
+  __shared__ float tile[kStreamLdsFloats];
   load_lut(lut_g, lut);
   warp_batched_items<true>(jobs, item_job, frames, total_items, order_stride, lut, tile);
 }
