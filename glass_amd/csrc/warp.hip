@@ -910,7 +910,8 @@ static_assert(((kLutLdsVec4 * 16 + kStreamLdsFloats * 4 + kLdsGranule - 1) / kLd
 static_assert(kSOutW == kOutW && kSOutH == kOutH, "both affine Lanczos3 kernels use the same output tiles");
 struct BoxGeom {
   int bx0, by0, bw, bh;
-  bool ok;  // staged (finite source, no clamp, and the box fits)
+  bool ok;      // staged (finite source, no clamp, and the box fits)
+  bool inside;  // every pixel of the tile maps inside the footprint (strip tests skipped)
 };
 
 // Window rows of a strip: loaded from the frame with buffer loads into
@@ -1116,14 +1117,17 @@ __device__ __forceinline__ bool stream_strip(const FrameRef& ref, const StreamFr
   const double yd0 = static_cast<double>(y0), yde = static_cast<double>(y0 + S - 1);
   const double sx0 = (px + fr.m1 * yd0) + fr.m2, sy0 = (py + fr.m4 * yd0) + fr.m5;
   const double sxe = (px + fr.m1 * yde) + fr.m2, sye = (py + fr.m4 * yde) + fr.m5;
-  const double lox = fmin(sx0, sxe), hix = fmax(sx0, sxe), loy = fmin(sy0, sye), hiy = fmax(sy0, sye);
   // Exact footprint decisions need a margin far above the FP64 rounding of
   // the coordinates (~1e-12 px); x keeps one more pixel so that the 7-column
-  // window stays inside the row.
-  constexpr double kEdge = 1.0e-7;
-  const double xmax = static_cast<double>(W - 3), ymax = static_cast<double>(H - 3);
-  const bool inside = lox >= 3.0 && hix < xmax - 1.0 && loy >= 2.0 + kEdge && hiy < ymax - kEdge;
-  const bool outside = hix < 2.0 - kEdge || lox >= xmax + kEdge || hiy < 2.0 - kEdge || loy >= ymax + kEdge;
+  // window stays inside the row.  Tiles wholly inside skip the tests.
+  bool inside = true, outside = false;
+  if (!g.inside) {
+    const double lox = fmin(sx0, sxe), hix = fmax(sx0, sxe), loy = fmin(sy0, sye), hiy = fmax(sy0, sye);
+    constexpr double kEdge = 1.0e-7;
+    const double xmax = static_cast<double>(W - 3), ymax = static_cast<double>(H - 3);
+    inside = lox >= 3.0 && hix < xmax - 1.0 && loy >= 2.0 + kEdge && hiy < ymax - kEdge;
+    outside = hix < 2.0 - kEdge || lox >= xmax + kEdge || hiy < 2.0 - kEdge || loy >= ymax + kEdge;
+  }
   // In the footprint s >= 2, so truncation is floor and fract(s) = s - floor(s)
   // (the hardware conversion saturates coordinates far outside).
   int ix0, iy0, iye;
@@ -1183,20 +1187,37 @@ __device__ __forceinline__ bool stream_strip(const FrameRef& ref, const StreamFr
 // from the frame.  Every value is wave-uniform.
 __device__ __forceinline__ BoxGeom box_geometry(const FrameRef& ref, bool eligible, int ox0, int oy0, int ox1,
                                                 int oy1, int W, int H) {
-  BoxGeom g{0, 0, 0, 0, false};
+  BoxGeom g{0, 0, 0, 0, false, false};
   if (!eligible) return g;
   const double m0 = ref.inv[0], m1 = ref.inv[1], m2 = ref.inv[2], m3 = ref.inv[3], m4 = ref.inv[4], m5 = ref.inv[5];
-  const double x0 = ox0, x1 = ox1, y0 = oy0, y1 = oy1;
-  const double sxl = (m0 * (m0 >= 0.0 ? x0 : x1) + m1 * (m1 >= 0.0 ? y0 : y1)) + m2;
-  const double sxh = (m0 * (m0 >= 0.0 ? x1 : x0) + m1 * (m1 >= 0.0 ? y1 : y0)) + m2;
-  const double syl = (m3 * (m3 >= 0.0 ? x0 : x1) + m4 * (m4 >= 0.0 ? y0 : y1)) + m5;
-  const double syh = (m3 * (m3 >= 0.0 ? x1 : x0) + m4 * (m4 >= 0.0 ? y1 : y0)) + m5;
+  // Corners chosen by the coefficients' sign bits: scalar tests of the
+  // uniform values instead of FP64 compares and 64-bit selects (a zero
+  // coefficient gives the same product at either corner).
+  auto nonneg = [](double m) { return __builtin_bit_cast(long long, m) >= 0; };
+  const int cx0 = nonneg(m0) ? ox0 : ox1, cx1 = nonneg(m0) ? ox1 : ox0;
+  const int cy0 = nonneg(m1) ? oy0 : oy1, cy1 = nonneg(m1) ? oy1 : oy0;
+  const int dx0 = nonneg(m3) ? ox0 : ox1, dx1 = nonneg(m3) ? ox1 : ox0;
+  const int dy0 = nonneg(m4) ? oy0 : oy1, dy1 = nonneg(m4) ? oy1 : oy0;
+  const double sxl = (m0 * static_cast<double>(cx0) + m1 * static_cast<double>(cy0)) + m2;
+  const double sxh = (m0 * static_cast<double>(cx1) + m1 * static_cast<double>(cy1)) + m2;
+  const double syl = (m3 * static_cast<double>(dx0) + m4 * static_cast<double>(dy0)) + m5;
+  const double syh = (m3 * static_cast<double>(dx1) + m4 * static_cast<double>(dy1)) + m5;
   int a, b, c, d;
   asm("v_cvt_i32_f64 %0, %1" : "=v"(a) : "v"(floor(sxl)));
   asm("v_cvt_i32_f64 %0, %1" : "=v"(b) : "v"(floor(sxh)));
   asm("v_cvt_i32_f64 %0, %1" : "=v"(c) : "v"(floor(syl)));
   asm("v_cvt_i32_f64 %0, %1" : "=v"(d) : "v"(floor(syh)));
+  a = __builtin_amdgcn_readfirstlane(a);
+  b = __builtin_amdgcn_readfirstlane(b);
+  c = __builtin_amdgcn_readfirstlane(c);
+  d = __builtin_amdgcn_readfirstlane(d);
   const int bx0 = max(a - 3, 0), bx1 = min(b + 4, W - 1), by0 = max(c - 2, 0), by1 = min(d + 3, H - 1);
+  // Every pixel of the tile lies inside the footprint with the streamed
+  // strips' margins: their tests at the corner extremes, which bound every
+  // pixel's FP64 coordinate (the rounded products and sums are monotonic).
+  constexpr double kEdge = 1.0e-7;
+  g.inside = sxl >= 3.0 && sxh < static_cast<double>(W - 3) - 1.0 && syl >= 2.0 + kEdge &&
+             syh < static_cast<double>(H - 3) - kEdge;
   g.bx0 = __builtin_amdgcn_readfirstlane(bx0);
   g.by0 = __builtin_amdgcn_readfirstlane(by0);
   g.bw = __builtin_amdgcn_readfirstlane(bx1 - bx0 + 1);
@@ -1312,7 +1333,7 @@ __device__ __forceinline__ void lanczos_stream_tile(const GetFrame& get, int cou
   for (int f = 0; f < count; ++f) {
     const FrameRef ref = get(f);
     const bool finite = !clamp_on && ref.src_flag == 0u;
-    BoxGeom next{0, 0, 0, 0, false};
+    BoxGeom next{0, 0, 0, 0, false, false};
     if (f + 1 < count) next = geometry(get(f + 1));
     if (geo.ok || next.ok) {
       asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
