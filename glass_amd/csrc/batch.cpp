@@ -120,11 +120,18 @@ struct gh_batch {
   bool pinned = false;  // schedule set by gh_batch_set_schedule (no autotune)
   int trials = 0;
   hipStream_t stream = nullptr;
+  // Side streams and events of the fork / join in enqueue (default priority).
+  static constexpr int kSides = 3;
+  hipStream_t side[kSides] = {};
+  hipEvent_t ev_fork = nullptr, ev_warp = nullptr, ev_decided = nullptr;
+  hipEvent_t ev_join[kSides] = {};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 
   ~gh_batch() {
     if (stream) (void)hipStreamSynchronize(stream);
+    for (hipStream_t s : side)
+      if (s) (void)hipStreamSynchronize(s);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     for (Job& j : jobs) {
@@ -152,10 +159,21 @@ struct gh_batch {
     variants.release();
     warped_flags.release();
     lut.release();
+    for (hipEvent_t e : {ev_fork, ev_warp, ev_decided})
+      if (e) (void)hipEventDestroy(e);
+    for (int k = 0; k < kSides; ++k) {
+      if (ev_join[k]) (void)hipEventDestroy(ev_join[k]);
+      if (side[k]) (void)hipStreamDestroy(side[k]);
+    }
     if (stream) (void)hipStreamDestroy(stream);
   }
 
-  // The whole batch on `st`: clear flags / coverage, warp, integrate.
+  // The whole batch, ordered after the work already queued on `st` and
+  // complete when `st` reaches its end: clear flags / coverage, warp,
+  // integrate.  The launches that do not depend on each other run side by
+  // side (a fork / join over `st` and the side streams, recorded as parallel
+  // branches when captured into the graph): the Lanczos3 and the bilinear
+  // warp launches (different jobs), then the integration class launches.
   void enqueue(hipStream_t st, const Schedule& sc) {
     if (total_warped) {
       gh::launch_fill(reinterpret_cast<float*>(warped_flags.ptr), static_cast<long long>(total_warped), 0.0f, st);
@@ -164,34 +182,66 @@ struct gh_batch {
       if (!j.coverage) continue;
       gh::launch_fill(j.coverage, j.stack->npix, static_cast<float>(j.n_unwarped), st);
     }
-    warp(st, sc);
+    const int o = item_frames_index(sc.frames_per_item);
+    const bool both = wplans[0][o].total_items && wplans[1][o].total_items;
+    if (both) {
+      check(hipEventRecord(ev_fork, st), "hipEventRecord(batch fork)");
+      check(hipStreamWaitEvent(side[0], ev_fork, 0), "hipStreamWaitEvent(batch fork)");
+      warp_kind(0, side[0], sc);
+      check(hipEventRecord(ev_warp, side[0]), "hipEventRecord(batch warp)");
+      warp_kind(1, st, sc);
+      check(hipStreamWaitEvent(st, ev_warp, 0), "hipStreamWaitEvent(batch warp)");
+    } else {
+      warp(st, sc);
+    }
     gh::launch_decide_variants(variant_jobs.ptr, n_variant_jobs, variants.ptr, st);
-    integrate(st, sc);
+    integrate(st, sc, true);
   }
 
   // The Lanczos3 launch takes the schedule's grid and item order; the
   // bilinear launch runs one workgroup per item in order.
-  void warp(hipStream_t st, const Schedule& sc) {
+  void warp_kind(int kind, hipStream_t st, const Schedule& sc) {
     const int o = item_frames_index(sc.frames_per_item);
-    const WarpPlan& l = wplans[0][o];
-    if (l.total_items) {
-      gh::launch_warp_batched(true, l.jobs.ptr, l.item_job.ptr, warp_frames.ptr, l.total_items,
+    const WarpPlan& w = wplans[kind][o];
+    if (!w.total_items) return;
+    if (kind == 0) {
+      gh::launch_warp_batched(true, w.jobs.ptr, w.item_job.ptr, warp_frames.ptr, w.total_items,
                               reinterpret_cast<const float4*>(lut.ptr), sc.warp_grid, sc.warp_stride, st);
-    }
-    const WarpPlan& b = wplans[1][o];
-    if (b.total_items) {
-      gh::launch_warp_batched(false, b.jobs.ptr, b.item_job.ptr, warp_frames.ptr, b.total_items, nullptr,
-                              static_cast<unsigned>(std::min<long long>(b.total_items, 0x7fffffffLL)), 1, st);
+    } else {
+      gh::launch_warp_batched(false, w.jobs.ptr, w.item_job.ptr, warp_frames.ptr, w.total_items, nullptr,
+                              static_cast<unsigned>(std::min<long long>(w.total_items, 0x7fffffffLL)), 1, st);
     }
   }
+  void warp(hipStream_t st, const Schedule& sc) {
+    warp_kind(0, st, sc);
+    warp_kind(1, st, sc);
+  }
 
-  void integrate(hipStream_t st, const Schedule& sc) {
+  // The integration launches; with `fork`, class launch i runs on stream
+  // i mod (1 + kSides) (st or a side stream, each after the launches queued
+  // on st so far) and st waits for all of them at the end.
+  void integrate(hipStream_t st, const Schedule& sc, bool fork = false) {
     const IntegratePlan& ip = iplans[sc.classes];
-    for (const ClassLaunch& l : ip.launches) {
+    bool used[kSides] = {};
+    if (fork && ip.launches.size() > 1) check(hipEventRecord(ev_decided, st), "hipEventRecord(batch decided)");
+    for (size_t i = 0; i < ip.launches.size(); ++i) {
+      const ClassLaunch& l = ip.launches[i];
+      hipStream_t s = st;
+      const int k = static_cast<int>(i % (1 + kSides)) - 1;
+      if (fork && ip.launches.size() > 1 && k >= 0) {
+        s = side[k];
+        if (!used[k]) check(hipStreamWaitEvent(s, ev_decided, 0), "hipStreamWaitEvent(batch decided)");
+        used[k] = true;
+      }
       gh::launch_integrate_cap(l.cap, l.first, gh::IntegrateJobTable{l.args.ptr, l.block_job.ptr, l.first_block.ptr},
-                               l.grid, st);
+                               l.grid, s);
     }
     for (size_t j : ip.per_job) gh::launch_integrate(jobs[j].args, st);
+    for (int k = 0; k < kSides; ++k) {
+      if (!used[k]) continue;
+      check(hipEventRecord(ev_join[k], side[k]), "hipEventRecord(batch join)");
+      check(hipStreamWaitEvent(st, ev_join[k], 0), "hipStreamWaitEvent(batch join)");
+    }
   }
 
   long long lanczos_items(int frames_per_item) const {
@@ -269,13 +319,13 @@ struct gh_batch {
     }
     warp(stream, sched);  // the integration candidates read these warped frames
     gh::launch_decide_variants(variant_jobs.ptr, n_variant_jobs, variants.ptr, stream);
-    float best_i = time_launch([&] { integrate(stream, sched); });
+    float best_i = time_launch([&] { integrate(stream, sched, true); });
     const Schedule base = sched;
     for (int set = 0; set < kClassSets; ++set) {
       if (set == base.classes) continue;
       Schedule sc = base;
       sc.classes = set;
-      const float ms = time_launch([&] { integrate(stream, sc); });
+      const float ms = time_launch([&] { integrate(stream, sc, true); });
       if (ms < best_i) {
         best_i = ms;
         sched = sc;
@@ -294,6 +344,12 @@ int gh_batch_create(const gh_batch_job* jobs, int64_t n_jobs, uint32_t flags, gh
     auto b = std::make_unique<gh_batch>();
     b->flags = flags;
     check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate(batch)");
+    for (int k = 0; k < gh_batch::kSides; ++k) {
+      check(hipStreamCreateWithFlags(&b->side[k], hipStreamNonBlocking), "hipStreamCreate(batch side)");
+      check(hipEventCreateWithFlags(&b->ev_join[k], hipEventDisableTiming), "hipEventCreate(batch)");
+    }
+    for (hipEvent_t* e : {&b->ev_fork, &b->ev_warp, &b->ev_decided})
+      check(hipEventCreateWithFlags(e, hipEventDisableTiming), "hipEventCreate(batch)");
     // Lanczos3 weight table shared by all jobs.
     const size_t lut_floats = static_cast<size_t>(gh::kLanczosLutIntervals + 1) * gh::kLanczosLutStride;
     std::vector<float> lut(lut_floats);

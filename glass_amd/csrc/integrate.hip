@@ -277,7 +277,7 @@ int integrate_cap_for(int n_active) {
 void launch_integrate_cap(int cap, const IntegrateArgs& a, const IntegrateJobTable& t, unsigned grid,
                           hipStream_t stream) {
   if (!(launch_integrate_caps_small(cap, a, t, grid, stream) || launch_integrate_caps_mid(cap, a, t, grid, stream) ||
-        launch_integrate_caps_large(cap, a, t, grid, stream)))
+        launch_integrate_caps_upper(cap, a, t, grid, stream) || launch_integrate_caps_large(cap, a, t, grid, stream)))
     throw ValueError("no register kernel of that capacity");
   check(hipGetLastError(), "integrate kernel launch");
 }

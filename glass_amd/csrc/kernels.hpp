@@ -284,7 +284,7 @@ struct IntegrateJobTable {
 };
 // Capacities of the register-sort winsorized kernels (samples held in VGPRs);
 // a job takes the smallest one holding its positive-weight frames.
-constexpr int kIntegrateCaps[] = {16, 24, 32, 48, 64, 80, 96, 112, 128, 160, 200, 256};
+constexpr int kIntegrateCaps[] = {16, 24, 32, 40, 48, 56, 64, 72, 80, 88, 96, 104, 112, 120, 128, 160, 200, 256};
 constexpr int kIntegrateCapCount = sizeof(kIntegrateCaps) / sizeof(kIntegrateCaps[0]);
 int integrate_cap_for(int n_active);  // smallest capacity >= n_active (n_active <= 256)
 // The register kernel of capacity `cap` (one of kIntegrateCaps) over `grid`
@@ -297,6 +297,8 @@ bool launch_integrate_caps_small(int cap, const IntegrateArgs& a, const Integrat
                                  hipStream_t stream);
 bool launch_integrate_caps_mid(int cap, const IntegrateArgs& a, const IntegrateJobTable& t, unsigned grid,
                                hipStream_t stream);
+bool launch_integrate_caps_upper(int cap, const IntegrateArgs& a, const IntegrateJobTable& t, unsigned grid,
+                                 hipStream_t stream);
 bool launch_integrate_caps_large(int cap, const IntegrateArgs& a, const IntegrateJobTable& t, unsigned grid,
                                  hipStream_t stream);
 
