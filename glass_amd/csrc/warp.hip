@@ -1096,13 +1096,13 @@ __device__ __forceinline__ bool stream_strip_exact(const FrameRef& ref, const fl
       } else {
         value = lanczos_taps((const global_cf32*)(ref.src + static_cast<long long>(yy0 - 2) * W + (x0 - 2)), W, wx,
                              wy);
+        drain_vector_memory();
       }
       cov += 1ull << (4 * r);
     }
     st_global(ref.dst, static_cast<long long>(y) * W + x, value);
     bad |= !isfinite(value);
   }
-  drain_vector_memory();
   return bad;
 }
 
@@ -1321,9 +1321,10 @@ __device__ __forceinline__ void lanczos_stream_tile(const GetFrame& get, int cou
   const double xd = static_cast<double>(x);
 
   // Frame f's box is boxes[f & 1]; frame f + 1's is staged while frame f is
-  // computed.  One barrier per frame: after it every wave's loads of frame
-  // f's box have landed (each wave waits for its own first) and every wave is
-  // done with frame f - 1, whose buffer frame f + 1's box then takes.
+  // computed (or right after it, see below).  One barrier per frame: after it
+  // every wave's loads of frame f's box have landed (each wave waits for its
+  // own first) and every wave is done with frame f - 1, whose buffer frame
+  // f + 1's box then takes.
   auto geometry = [&](const FrameRef& r) {
     return box_geometry(r, !clamp_on && r.src_flag == 0u, ox0, oy0, ox1, oy1, W, H);
   };
@@ -1339,7 +1340,12 @@ __device__ __forceinline__ void lanczos_stream_tile(const GetFrame& get, int cou
       asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
       __syncthreads();
     }
-    if (next.ok) stage_box_lds(get(f + 1), next, W, bytes, boxes + ((f + 1) & 1) * kBoxFloats);
+    // The next box streams in during this frame only when this frame's
+    // strips read nothing but their box (it fits and the tile lies inside
+    // the footprint): a vector load of the frame issued meanwhile would wait
+    // for the staging loads.  Otherwise it is staged after this frame.
+    const bool ahead = geo.ok && geo.inside;
+    if (next.ok && ahead) stage_box_lds(get(f + 1), next, W, bytes, boxes + ((f + 1) & 1) * kBoxFloats);
     const float* box = boxes + (f & 1) * kBoxFloats;
     bool bad = false;
     if (finite) {
@@ -1380,6 +1386,7 @@ __device__ __forceinline__ void lanczos_stream_tile(const GetFrame& get, int cou
       drain_vector_memory();
     }
     if (__any(bad) && tx == 0) atomicOr(ref.dst_flag, 1u);
+    if (next.ok && !ahead) stage_box_lds(get(f + 1), next, W, bytes, boxes + ((f + 1) & 1) * kBoxFloats);
     geo = next;
   }
   if (p.coverage && x < W) {
