@@ -211,21 +211,36 @@ def refine_stack_vs_oracle():
     d = np.abs(c["master"][has].astype(np.float64) - ref)
     # Weight map (kept samples after rejection) against the oracle's; a pixel
     # may differ only where one of its samples lies within 1e-5 sigma of one
-    # of the oracle's rejection thresholds (the tolerance contract).
+    # of the oracle's rejection thresholds (the tolerance contract).  The
+    # scene is noise-free, so sigma is the frames' interpolation spread
+    # (~1e-6 at values ~0.05, a few hundred float32 ulps) and 1e-5 sigma is
+    # far below one ulp; the GPU's Lanczos3 samples themselves differ from
+    # the oracle's FP64-weight samples by a few ulps.  So a sample within
+    # 4 ulps of a threshold also counts as a threshold case here; the strict
+    # 1e-5-sigma figure is reported beside it.
     from oracle.rejection import center_and_scale
 
     ok = (cov > 0) & np.isfinite(samples)
     center, scale = center_and_scale(samples, ok, method="winsorized_sigma", low_sigma=3.0, high_sigma=3.0)
-    lo = (center - np.float32(3.0) * scale).astype(np.float64)
-    hi = (center + np.float32(3.0) * scale).astype(np.float64)
-    tol = 1e-5 * np.abs(scale.astype(np.float64))
+    lo32 = center - np.float32(3.0) * scale
+    hi32 = center + np.float32(3.0) * scale
+    lo, hi = lo32.astype(np.float64), hi32.astype(np.float64)
     xd = samples.astype(np.float64)
-    near = np.any(ok & ((np.abs(xd - lo) <= tol) | (np.abs(xd - hi) <= tol)), axis=0)
+
+    def near_within(tol_lo, tol_hi):
+        return np.any(ok & ((np.abs(xd - lo) <= tol_lo) | (np.abs(xd - hi) <= tol_hi)), axis=0)
+
+    tol = 1e-5 * np.abs(scale.astype(np.float64))
+    near_strict = near_within(tol, tol)
+    near = near_within(np.maximum(tol, 4.0 * np.spacing(np.abs(lo32)).astype(np.float64)),
+                       np.maximum(tol, 4.0 * np.spacing(np.abs(hi32)).astype(np.float64)))
     differ = c["weight"] != exp[1]
     return {"band": f"rows {y0}..{y0 + h - 1} x {w}", "pixels": int(has.sum()),
             "rms_rel": float(np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(ref * ref))),
             "p99_of_range": float(np.percentile(d, 99) / (ref.max() - ref.min())),
+            "sigma_median": float(np.median(scale)),
             "weight_map_mismatches": int(differ.sum()),
+            "weight_map_mismatches_off_1e-5_sigma": int((differ & ~near_strict).sum()),
             "weight_map_mismatches_off_threshold": int((differ & ~near).sum())}
 
 
