@@ -11,6 +11,8 @@
 // (8 B / 16 B raw loads, 16 B stores).  Optional fused FP64 moments replace a
 // separate stats pass over the stack.
 
+#include <algorithm>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -557,8 +559,13 @@ void launch_calibrate(const CalibBatch& batch, int count, const CalibParams& p, 
   const bool vec4 = (npix % 4 == 0);
   const dim3 grid(calibrate_blocks_per_frame(npix), count);
 #define GH_CAL(FMT, VEC) calibrate_kernel<FMT, VEC><<<grid, kBlock, 0, stream>>>(batch, p, npix, partials)
+  // Without statistics partials the batched kernel takes one block per 256
+  // pixel groups (up to 65,536 blocks): a u16 -> f32 stream measured 4.83 TB/s
+  // with such grids against 4.5 TB/s with 2,048 grid-stride blocks.
+  const unsigned grid_stream = static_cast<unsigned>(
+      std::min<long long>(std::max<long long>((npix / kCalPx + kBlock - 1) / kBlock, 1), 65536));
 #define GH_CALB(FMT, STATS) \
-  calibrate_batch_kernel<FMT, STATS><<<grid.x, kBlock, 0, stream>>>(batch, count, p, npix / kCalPx, partials)
+  calibrate_batch_kernel<FMT, STATS><<<grid_stream, kBlock, 0, stream>>>(batch, count, p, npix / kCalPx, partials)
 #define GH_CALS(FMT)                                                                                      \
   calibrate_batch_kernel<FMT, true><<<grid.x, kBlock, 0, stream>>>(sub, n, p, npix / kCalPx,                   \
                                                                    partials + static_cast<long long>(k0) * grid.x)
