@@ -119,18 +119,21 @@ __global__ __launch_bounds__(kTileX* kTileY) void warp_bilinear_kernel(WarpBatch
 }
 
 
-// Lanczos3 sample of one pixel read straight from the frame, covering every
-// semantic variant (CPU: no tap skipping, threshold 1e-8; resident: non-finite
-// taps skipped, threshold 1e-6, optional clamp).  `covered` reports whether
-// the pixel counts toward geometric coverage.
-__device__ __forceinline__ float lanczos_direct(const float* frame, int width, double sx, double sy,
-                                                const WarpParams& p, float thr, bool& covered) {
-  const double fx0 = floor(sx), fy0 = floor(sy);
-  const int x0 = static_cast<int>(fx0), y0 = static_cast<int>(fy0);
+// Lanczos3 sample of one pixel, covering every semantic variant (CPU: no tap
+// skipping, threshold 1e-8; resident: non-finite taps skipped, threshold
+// 1e-6, optional clamp).  `src` points at the window's top-left tap (row
+// floor(sy) - 2, column floor(sx) - 2) of rows `stride` floats apart: the
+// frame in global memory, or a staged copy in LDS (same taps, same
+// arithmetic, same value).  `covered` reports whether the pixel counts toward
+// geometric coverage.
+// CLAMP: -1 follows p.clamp at run time; 0 / 1 fix it (p.clamp < 0 / >= 0)
+// at compile time, so the unclamped form carries no per-tap extremes.
+template <typename P, int CLAMP = -1>
+__device__ __forceinline__ float lanczos_direct_window(P src, int stride, double sx, double sy, double fx0, double fy0,
+                                                       const WarpParams& p, float thr, bool& covered) {
   float wx[6], wy[6];
   lanczos_axis(static_cast<float>(sx - fx0), thr, wx);
   lanczos_axis(static_cast<float>(sy - fy0), thr, wy);
-  const global_cf32* src = (const global_cf32*)(frame + static_cast<long long>(y0 - 2) * width + (x0 - 2));
   float acc = 0.0f, wsum = 0.0f;
   float vmin = INFINITY, vmax = -INFINITY;
   // CPU semantics take a tap when its FP64 weight is non-zero
@@ -141,7 +144,7 @@ __device__ __forceinline__ float lanczos_direct(const float* frame, int width, d
   const bool last_x = sx != fx0, last_y = sy != fy0;
 #pragma unroll
   for (int j = 0; j < 6; ++j) {
-    const global_cf32* row = src + static_cast<long long>(j) * width;
+    const P row = src + static_cast<long long>(j) * stride;
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
       const float v = row[i];
@@ -150,8 +153,10 @@ __device__ __forceinline__ float lanczos_direct(const float* frame, int width, d
         if (isfinite(v)) {
           acc = fmaf(v, w, acc);
           wsum += w;
-          vmin = fminf(vmin, v);
-          vmax = fmaxf(vmax, v);
+          if constexpr (CLAMP != 0) {
+            vmin = fminf(vmin, v);
+            vmax = fmaxf(vmax, v);
+          }
         }
       } else if ((i < 5 || last_x) && (j < 5 || last_y)) {
         acc = fmaf(v, w, acc);
@@ -164,7 +169,7 @@ __device__ __forceinline__ float lanczos_direct(const float* frame, int width, d
   if (p.skip_nonfinite) {
     if (fabsf(wsum) > 1.0e-12f) {
       value = acc / wsum;
-      if (p.clamp >= 0.0f && vmin <= vmax) {
+      if (CLAMP != 0 && p.clamp >= 0.0f && vmin <= vmax) {
         const float r = vmax - vmin;
         value = fminf(vmax + p.clamp * r, fmaxf(vmin - p.clamp * r, value));
       }
@@ -175,6 +180,15 @@ __device__ __forceinline__ float lanczos_direct(const float* frame, int width, d
     covered = true;
   }
   return value;
+}
+
+// The sample read straight from the frame.
+__device__ __forceinline__ float lanczos_direct(const float* frame, int width, double sx, double sy,
+                                                const WarpParams& p, float thr, bool& covered) {
+  const double fx0 = floor(sx), fy0 = floor(sy);
+  const int x0 = static_cast<int>(fx0), y0 = static_cast<int>(fy0);
+  const global_cf32* src = (const global_cf32*)(frame + static_cast<long long>(y0 - 2) * width + (x0 - 2));
+  return lanczos_direct_window(src, width, sx, sy, fx0, fy0, p, thr, covered);
 }
 
 __device__ __forceinline__ bool lanczos_footprint(double sx, double sy, int width, int height) {
@@ -1735,6 +1749,190 @@ __global__ __launch_bounds__(256) void clip_integrate_kernel(ClipArgs a) {
 }
 
 // ------------------------------------------------------------------------
+// The fused Lanczos3 integrations of affine maps over 128 x 16 output tiles
+// (the same statistics as clip_integrate_kernel<2, ...>, sample for sample
+// and sum for sum).  Per frame the tile's source box is staged into LDS
+// (box_geometry / stage_box_lds of the streamed warp) and every lane samples
+// its column of 8 rows from it with the resident tap rules
+// (lanczos_direct_window: the same taps and arithmetic as from the frame,
+// which a window outside the box, or a frame whose box does not fit, reads
+// instead).  The per-pixel state of the lane's 8 rows stays in registers; the
+// sigma clip takes its passes (mean, variance, winsorized mean and variance,
+// final) over the frames again, sampling each time, where the per-pixel
+// kernel gathers 36 taps per sample from global memory and keeps the samples
+// in LDS.  MODE 0: weighted mean, 1: sigma clip.
+#ifndef GH_CLIP_ROWS
+#define GH_CLIP_ROWS 8
+#endif
+constexpr int kClipRows = GH_CLIP_ROWS;
+constexpr int kClipTileH = kClipRows * (kSThreads / 64 / kSColGroups);
+
+template <int MODE, bool CLAMP>
+__global__ __launch_bounds__(kSThreads) __attribute__((amdgpu_waves_per_eu(2, MODE == 0 ? 4 : 3))) void clip_lanczos_tile_kernel(
+    ClipArgs a, int tiles_x) {
+  __shared__ float box[kBoxFloats];
+  const unsigned nb = gridDim.x, bid = blockIdx.x;
+  const unsigned xcd = bid & 7u, q = nb >> 3, rr = nb & 7u;
+  const unsigned logical = xcd * q + min(xcd, rr) + (bid >> 3);
+  const int ox0 = static_cast<int>(logical % tiles_x) * kSOutW;
+  const int oy0 = static_cast<int>(logical / tiles_x) * kClipTileH;
+  const int W = a.width, H = a.height;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int x = ox0 + 64 * (ty % kSColGroups) + tx;
+  const int y_first = oy0 + kClipRows * (ty / kSColGroups);
+  const int ox1 = min(ox0 + kSOutW, W) - 1, oy1 = min(oy0 + kClipTileH, H) - 1;
+  const int bytes = W * H * 4;  // frames of < 2^31 bytes (checked by the host)
+  WarpParams p{};
+  p.fill = NAN;
+  p.clamp = a.clamp;
+  p.skip_nonfinite = 1;
+  typedef __attribute__((address_space(4))) const double const_f64;
+
+  // The sample of frame k at row r of this lane's column (NaN when it does
+  // not count); `covered` reports its geometric coverage.
+  auto sample = [&](int k, const BoxGeom& g, const FrameRef& ref, int r, bool& covered) -> float {
+    covered = false;
+    const int y = y_first + r;
+    if (x >= W || y >= H) return NAN;
+    double sx, sy;
+    source_coords(ref.inv, 1, x, y, sx, sy);
+    if (!lanczos_footprint(sx, sy, W, H)) return NAN;
+    const double fx0 = floor(sx), fy0 = floor(sy);
+    const int x0 = static_cast<int>(fx0), y0 = static_cast<int>(fy0);
+    float v;
+    if (g.ok && x0 - 2 >= g.bx0 && x0 + 3 < g.bx0 + g.bw && y0 - 2 >= g.by0 && y0 + 3 < g.by0 + g.bh) {
+      v = lanczos_direct_window<const float*, CLAMP ? 1 : 0>(box + ((y0 - 2 - g.by0) * kBoxStride + (x0 - 2 - g.bx0)),
+                                                             kBoxStride, sx, sy, fx0, fy0, p, 1.0e-6f, covered);
+    } else {
+      v = lanczos_direct_window<const global_cf32*, CLAMP ? 1 : 0>(
+          (const global_cf32*)(ref.src + static_cast<long long>(y0 - 2) * W + (x0 - 2)), W, sx, sy, fx0, fy0, p,
+          1.0e-6f, covered);
+    }
+    return (covered && isfinite(v)) ? v : NAN;
+  };
+  // One pass over the frames in order: body(k, weight, row, sample, covered).
+  auto pass = [&](auto&& body) {
+#pragma unroll 1
+    for (int k = 0; k < a.n; ++k) {
+      const double* m = a.inverses + 9 * k;
+      FrameRef ref{};
+      ref.src = table_at(a.frames, k);
+      for (int i = 0; i < 6; ++i) ref.inv[i] = ((const const_f64*)m)[i];
+      const float w = ((const const_f32*)a.weights)[k];
+      const BoxGeom g = box_geometry(ref, true, ox0, oy0, ox1, oy1, W, H);
+      if (g.ok) {
+        __syncthreads();  // every wave is done with the previous frame's box
+        stage_box_lds(ref, g, W, bytes, box);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();  // every wave's part of the box has landed
+      }
+      // compile-time rows: the per-pixel arrays stay in registers
+      static_for<kClipRows>([&](auto R) __attribute__((always_inline)) {
+        constexpr int r = decltype(R)::value;
+        bool covered;
+        const float v = sample(k, g, ref, r, covered);
+        body(w, r, v, covered);
+      });
+    }
+  };
+
+  // The pass bodies add +0.0 instead of skipping a sample (branch-free, so
+  // the unrolled rows keep compile-time indices); no sum here is ever -0.0,
+  // so the sums are the per-pixel kernel's bit for bit.
+  float geometric[kClipRows], s0[kClipRows], s1[kClipRows], s2[kClipRows];
+#pragma unroll
+  for (int r = 0; r < kClipRows; ++r) geometric[r] = s0[r] = s1[r] = s2[r] = 0.0f;
+  float master[kClipRows], wsum[kClipRows], coverage[kClipRows], low[kClipRows], high[kClipRows];
+  if constexpr (MODE == 0) {
+    // s0 = sum w x, s1 = sum w, s2 = count
+    pass([&](float w, int r, float v, bool covered) {
+      const bool fin = isfinite(v);
+      geometric[r] += covered ? 1.0f : 0.0f;
+      s0[r] += fin ? v * w : 0.0f;
+      s1[r] += fin ? w : 0.0f;
+      s2[r] += fin ? 1.0f : 0.0f;
+    });
+#pragma unroll
+    for (int r = 0; r < kClipRows; ++r) {
+      master[r] = s1[r] > 0.0f ? s0[r] / s1[r] : 0.0f;
+      wsum[r] = s1[r];
+      coverage[r] = s2[r];
+      low[r] = high[r] = 0.0f;
+    }
+  } else {
+    // mean and count (s0, s1), then the variance (s2)
+    pass([&](float, int r, float v, bool covered) {
+      const bool fin = isfinite(v);
+      geometric[r] += covered ? 1.0f : 0.0f;
+      s0[r] += fin ? v : 0.0f;
+      s1[r] += fin ? 1.0f : 0.0f;
+    });
+    float center[kClipRows], lo[kClipRows], hi[kClipRows];
+#pragma unroll
+    for (int r = 0; r < kClipRows; ++r) center[r] = s1[r] > 0.0f ? s0[r] / s1[r] : 0.0f;
+    pass([&](float, int r, float v, bool) {
+      const float d = v - center[r];
+      s2[r] += isfinite(v) ? d * d : 0.0f;
+    });
+#pragma unroll
+    for (int r = 0; r < kClipRows; ++r) {
+      const float scale = s1[r] > 0.0f ? sqrtf(s2[r] / s1[r]) : 0.0f;
+      lo[r] = center[r] - a.low_sigma * scale;
+      hi[r] = center[r] + a.high_sigma * scale;
+    }
+    if (a.winsorize) {
+#pragma unroll
+      for (int r = 0; r < kClipRows; ++r) s0[r] = s2[r] = 0.0f;
+      pass([&](float, int r, float v, bool) {
+        const float c = v < lo[r] ? lo[r] : (v > hi[r] ? hi[r] : v);
+        s0[r] += isfinite(v) ? c : 0.0f;
+      });
+#pragma unroll
+      for (int r = 0; r < kClipRows; ++r) center[r] = s1[r] > 0.0f ? s0[r] / s1[r] : 0.0f;
+      pass([&](float, int r, float v, bool) {
+        const float c = v < lo[r] ? lo[r] : (v > hi[r] ? hi[r] : v);
+        const float d = c - center[r];
+        s2[r] += isfinite(v) ? d * d : 0.0f;
+      });
+#pragma unroll
+      for (int r = 0; r < kClipRows; ++r) {
+        const float scale = s1[r] > 0.0f ? sqrtf(s2[r] / s1[r]) : 0.0f;
+        lo[r] = center[r] - a.low_sigma * scale;
+        hi[r] = center[r] + a.high_sigma * scale;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kClipRows; ++r) master[r] = wsum[r] = coverage[r] = low[r] = high[r] = 0.0f;
+    pass([&](float w, int r, float v, bool) {
+      const bool fin = isfinite(v) && s1[r] > 0.0f;
+      const bool below = fin && v < lo[r];
+      const bool above = fin && !below && v > hi[r];
+      const bool keep = fin && !below && !above;
+      low[r] += below ? 1.0f : 0.0f;
+      high[r] += above ? 1.0f : 0.0f;
+      master[r] += keep ? v * w : 0.0f;
+      wsum[r] += keep ? w : 0.0f;
+      coverage[r] += keep ? 1.0f : 0.0f;
+    });
+#pragma unroll
+    for (int r = 0; r < kClipRows; ++r) master[r] = wsum[r] > 0.0f ? master[r] / wsum[r] : 0.0f;
+  }
+  if (x >= W) return;
+#pragma unroll
+  for (int r = 0; r < kClipRows; ++r) {
+    const int y = y_first + r;
+    if (y >= H) break;
+    const long long o = static_cast<long long>(y) * W + x;
+    if (a.master) st_global(a.master, o, master[r]);
+    if (a.weight) st_global(a.weight, o, wsum[r]);
+    if (a.coverage) st_global(a.coverage, o, coverage[r]);
+    if (a.low) st_global(a.low, o, low[r]);
+    if (a.high) st_global(a.high, o, high[r]);
+    if (a.geometric) st_global(a.geometric, o, geometric[r]);
+  }
+}
+
+// ------------------------------------------------------------------------
 // Translations (cpp/cuda/warp_kernels.cu:103-190): the integer form copies
 // in[y - dy][x - dx] (fill outside); the bilinear form samples at
 // (x - dx, y - dy) computed in float32, valid 0 <= s <= dim - 1, the far
@@ -1928,6 +2126,18 @@ void launch_clip_integrate(const ClipArgs& a, int sampler, hipStream_t stream) {
     else if (a.n <= 64) clip_integrate_kernel<0, 64, 0><<<grid256, 256, 0, stream>>>(a);
     else if (a.n <= 128) clip_integrate_kernel<0, 128, 0><<<grid256, 256, 0, stream>>>(a);
     else GH_CLIP_AGAIN(0);
+  } else if (sampler == 2 && a.affine && !a.tile_map && static_cast<long long>(a.width) * a.height * 4 < (1ll << 31)) {
+    // affine Lanczos3: staged source boxes per output tile
+    const int tiles_x = (a.width + kSOutW - 1) / kSOutW, tiles_y = (a.height + kClipTileH - 1) / kClipTileH;
+    const unsigned grid = static_cast<unsigned>(tiles_x * tiles_y);
+    const bool clamp = a.clamp >= 0.0f;
+    if (a.mode == 0) {
+      if (clamp) clip_lanczos_tile_kernel<0, true><<<grid, kSThreads, 0, stream>>>(a, tiles_x);
+      else clip_lanczos_tile_kernel<0, false><<<grid, kSThreads, 0, stream>>>(a, tiles_x);
+    } else {
+      if (clamp) clip_lanczos_tile_kernel<1, true><<<grid, kSThreads, 0, stream>>>(a, tiles_x);
+      else clip_lanczos_tile_kernel<1, false><<<grid, kSThreads, 0, stream>>>(a, tiles_x);
+    }
   } else if (a.mode == 0) {
     // the weighted mean is a single pass: nothing to store
     if (sampler == 1) GH_CLIP_AGAIN(1);
