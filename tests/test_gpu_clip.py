@@ -132,6 +132,55 @@ def test_matrix_warped_lanczos3(gpu, f, clamp):
     np.testing.assert_allclose(mean[0], ref["master"], rtol=0, atol=2e-3)
 
 
+@pytest.mark.parametrize("clamp", [-1.0, 0.3])
+def test_matrix_warped_lanczos3_staged_and_unstaged_boxes(gpu, clamp):
+    # Affine Lanczos3 maps take the tile kernel: each frame's source box of a
+    # 128 x 16 output tile is staged into LDS when it fits (near-identity
+    # maps), and the taps come from the frame when it does not (35-degree
+    # rotations, a zoom, a shear), within one launch and one tile; partial
+    # tiles at the right and bottom edges.  Samples and sums must equal the
+    # library's resident warp followed by the oracle's clip bit for bit.
+    h, w = 70, 300
+    x = _frames(8, h, w, seed=79)
+    x[3, 30:33, 100:110] = np.nan
+    def rot(deg, tx, ty, s=1.0):
+        a = np.deg2rad(deg)
+        return np.array([[s * np.cos(a), -s * np.sin(a), tx], [s * np.sin(a), s * np.cos(a), ty], [0, 0, 1.0]])
+    mats = np.stack([rot(0.3, 1.5, -2.0), rot(35.0, 60.0, -120.0), rot(-0.2, -2.5, 1.0), rot(-35.0, -20.0, 90.0),
+                     rot(0.0, 0.5, 0.5, 1.8), np.array([[1.0, 0.6, -15.0], [0.0, 1.0, 3.0], [0, 0, 1.0]]),
+                     rot(0.5, 3.0, 2.0), rot(0.0, -4.0, -1.5)])
+    weights = np.array([1, 0.5, 2, 1, 1, 0, 1, 1.5], np.float32)
+    active = [k for k in range(8) if weights[k] > 0]
+    print(f"fused lanczos3 staged/unstaged boxes clamp={clamp}", flush=True)
+    with _stack_of(x) as s:
+        clip = s.integrate_matrix_warped_sigma_clip(mats, weights, "lanczos3", clamp, 2.0, 2.0, True)
+        plain = s.integrate_matrix_warped_sigma_clip(mats, weights, "lanczos3", clamp, 2.0, 2.0, False)
+        mean = s.integrate_matrix_warped_mean(mats, weights, "lanczos3", clamp)
+    with _stack_of(x) as s:
+        s.reset_warp_coverage()
+        s.warp_frames(active, mats[active], "lanczos3", "resident", fill=np.nan, clamping_threshold=clamp)
+        warped = s.download_frames_tile(list(range(8)), 0, 0, w, h)
+        geo = s.warp_coverage_map()
+    if clamp >= 0:
+        # the clamped resident warp samples every pixel with the same direct
+        # arithmetic (unclamped, finite frames take the table-weight warp)
+        for got, wins in ((clip, True), (plain, False)):
+            exp = SC.clip_integrate(warped, weights, "sigma_clip", 2.0, 2.0, wins)
+            for name, g in zip(MAPS, got[:5]):
+                assert np.array_equal(g, exp[name]), (name, wins)
+            assert np.array_equal(got[5], geo)
+        exp_mean = SC.clip_integrate(warped, weights, "mean")
+        assert np.array_equal(mean[0], exp_mean["master"]) and np.array_equal(mean[1], exp_mean["weight"])
+        assert np.array_equal(mean[2], exp_mean["coverage"])
+    assert np.array_equal(mean[3], geo)
+    # the FP64-weight oracle, within the Lanczos3 tolerance: float32 direct
+    # weights and float32 tap sums, whose rounding grows with the 500-valued
+    # outliers' ringing (relative 1e-5 beside the absolute 2e-3 at ~100)
+    ref = SC.matrix_warped_integrate(x, mats, weights, "lanczos3", clamp, "mean")
+    assert np.array_equal(mean[2], ref["coverage"]) and np.array_equal(mean[3], ref["geometric"])
+    np.testing.assert_allclose(mean[0], ref["master"], rtol=1e-5, atol=2e-3)
+
+
 def test_clip_argument_errors(gpu):
     x = _frames(3, 8, 8, seed=1)
     with _stack_of(x) as s:
