@@ -228,7 +228,7 @@ __global__ __launch_bounds__(kBlock) void calibrate_batch_kernel(CalibBatch batc
       if ((threadIdx.x & 63) == 0) s_part[wave][k] = StatPartial{ws, wq, wf, wn};
     }
     __syncthreads();
-    if (threadIdx.x < count) {
+    if (static_cast<int>(threadIdx.x) < count) {
       const int k = threadIdx.x;
       StatPartial out{0.0, 0.0, 0, 0};
       for (int w = 0; w < kBlock / 64; ++w) {

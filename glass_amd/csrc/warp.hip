@@ -1259,7 +1259,10 @@ __device__ __forceinline__ rsrc_words_t rsrc_words(const float* base, int bytes)
 // being read from the box being filled); nothing is returned in a register,
 // and the caller waits (s_waitcnt vmcnt(0) and a barrier) before reading the
 // data.  Waits the compiler places for its own loads stay correct: vector
-// memory operations of a wave complete in order.
+// memory operations of a wave complete in order.  M0 is a reserved register
+// (the compiler warns that the clobber is not tracked); nothing the compiler
+// generates in the kernels that stage boxes reads or writes M0 (their ISA
+// holds M0 only inside these statements).
 __device__ __forceinline__ void load_dword_to_lds(rsrc_words_t rs, unsigned lds_byte, unsigned voff, unsigned soff) {
   asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tbuffer_load_dword %1, %2, %3 offen lds"
                :
@@ -1790,7 +1793,7 @@ __global__ __launch_bounds__(kSThreads) __attribute__((amdgpu_waves_per_eu(2, MO
 
   // The sample of frame k at row r of this lane's column (NaN when it does
   // not count); `covered` reports its geometric coverage.
-  auto sample = [&](int k, const BoxGeom& g, const FrameRef& ref, int r, bool& covered) -> float {
+  auto sample = [&](const BoxGeom& g, const FrameRef& ref, int r, bool& covered) -> float {
     covered = false;
     const int y = y_first + r;
     if (x >= W || y >= H) return NAN;
@@ -1830,7 +1833,7 @@ __global__ __launch_bounds__(kSThreads) __attribute__((amdgpu_waves_per_eu(2, MO
       static_for<kClipRows>([&](auto R) __attribute__((always_inline)) {
         constexpr int r = decltype(R)::value;
         bool covered;
-        const float v = sample(k, g, ref, r, covered);
+        const float v = sample(g, ref, r, covered);
         body(w, r, v, covered);
       });
     }
