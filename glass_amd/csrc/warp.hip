@@ -1276,7 +1276,12 @@ __device__ __forceinline__ void load_dword_to_lds(rsrc_words_t rs, unsigned lds_
 // address lies inside the frame.
 __device__ __forceinline__ void stage_box_lds(const FrameRef& ref, const BoxGeom& g, int W, int bytes, float* box) {
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-  const int lane = threadIdx.x & 63;
+  // The lane index recomputed here from an opaque zero: hoisted to the kernel
+  // entry, the staging's lane offsets are kept across the whole frame loop
+  // and spilled, and their reloads would sit before every staging.
+  unsigned zero = 0u;
+  asm volatile("" : "+s"(zero));
+  const int lane = static_cast<int>(__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, zero)));
   const rsrc_words_t rs = rsrc_words(ref.src, bytes);
   const unsigned v0 = static_cast<unsigned>(min(lane, g.bw - 1) * 4);
   const unsigned v1 = static_cast<unsigned>(min(64 + lane, g.bw - 1) * 4);
