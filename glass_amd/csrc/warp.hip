@@ -277,16 +277,21 @@ static_assert(kLaneRows * 4 <= 64 && kWarpBatch <= 15, "coverage counts are pack
 // distinct intervals touch consecutive slots; row kLanczosLutIntervals serves
 // f == 1 exactly.
 __device__ __forceinline__ void lut_coefficients(const float4* lut, int i, float (&c)[18]) {
-  const float4* e = lut + i;
+  // One address register: interval i's float4s lie at byte 16 i of their
+  // planes and its float2 at byte 8 i = (16 i) / 2 of the last plane (the
+  // plane bases are immediate offsets).
+  unsigned b = static_cast<unsigned>(i) << 4;
+  asm("" : "+v"(b));
+  const char* base = reinterpret_cast<const char*>(lut);
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    const float4 q = e[v * kLutRows];
+    const float4 q = *reinterpret_cast<const float4*>(base + b + v * kLutRows * 16);
     c[4 * v + 0] = q.x;
     c[4 * v + 1] = q.y;
     c[4 * v + 2] = q.z;
     c[4 * v + 3] = q.w;
   }
-  const float2 q = reinterpret_cast<const float2*>(lut + 4 * kLutRows)[i];
+  const float2 q = *reinterpret_cast<const float2*>(base + 4 * kLutRows * 16 + (b >> 1));
   c[16] = q.x;
   c[17] = q.y;
 }
@@ -939,18 +944,17 @@ struct RowsFromFrame {
   __device__ __forceinline__ float at(int j, int c) const { return ld_rsrc(rs, voff + 4u * c, row_bytes * j); }
   __device__ __forceinline__ void pin() { asm volatile("" : "+v"(voff)); }
 };
+typedef __attribute__((address_space(3))) const float lds_f32;
 struct RowsFromBox {
-  const float* box;
-  int base;
+  lds_f32* row0;  // the lane's window row 0, column 0 in the box
   static constexpr int kAhead = GH_WARP_BOX_AHEAD;
   static constexpr bool kLutAhead = GH_WARP_LUT_AHEAD;
-  // one address per row; its columns are immediate offsets
   __device__ __forceinline__ float at(int j, int c) const {
-    int o = base + j * kBoxStride;
-    asm("" : "+v"(o));  // one base register per row; its columns are immediate offsets
-    return box[o + c];
+    lds_f32* p = row0 + j * kBoxStride;
+    asm("" : "+v"(p));  // one LDS byte address per row; its columns are immediate offsets
+    return p[c];
   }
-  __device__ __forceinline__ void pin() { asm volatile("" : "+v"(base)); }
+  __device__ __forceinline__ void pin() { asm volatile("" : "+v"(row0)); }
 };
 
 // s_waitcnt vmcnt(0) as a builtin (a wait the compiler accounts for), at the
@@ -1000,9 +1004,11 @@ __device__ __forceinline__ bool stream_strip_rows(const StreamFrame& fr, const f
       ix = static_cast<int>(txr);
       u = __builtin_amdgcn_fractf(txr);
     } else {
-      const float fl = floorf(txr);
-      ix = static_cast<int>(fl) & (kLanczosLutIntervals - 1);
-      u = txr - fl;
+      // |txr| < 2^8 here; floor and conversion in one instruction
+      int fl;
+      asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(fl) : "v"(txr));
+      ix = fl & (kLanczosLutIntervals - 1);
+      u = __builtin_amdgcn_fractf(txr);
       sel = txr >= fr.thr;
     }
   };
@@ -1177,7 +1183,7 @@ __device__ __forceinline__ bool stream_strip(const FrameRef& ref, const StreamFr
     const int cb = ix0 - 2 - (seven ? fr.neg : 0);
     bool bad;
     if constexpr (BOX) {
-      const RowsFromBox rows{box, live ? (iy0 - 2 - g.by0) * kBoxStride + (cb - g.bx0) : 0};
+      const RowsFromBox rows{(lds_f32*)box + (live ? (iy0 - 2 - g.by0) * kBoxStride + (cb - g.bx0) : 0)};
       bad = seven ? stream_strip_rows<S, 7>(fr, lut, rows, tx0, uy0, cy, live, present, x, y0, W, fill)
                   : stream_strip_rows<S, 6>(fr, lut, rows, tx0, uy0, cy, live, present, x, y0, W, fill);
     } else {
