@@ -130,7 +130,7 @@ def test_integrate_signature_matches_reference_keywords():
 
 class _ReferenceLoader:
     """The reference's `_native()` caching loader (glass_cuda.py:31-43) with
-    the one-line binding of INTEGRATION.md: return glass_amd.native_module."""
+    the drop-in's one-line binding: it returns glass_amd.native_module."""
 
     def __init__(self):
         self.attempted = False
